@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: SU(3) pure-gauge Wilson HMC at 32^4, beta = 6.0, on one MI355X.
+
+Workload (one "step" = one HMC trajectory over the whole lattice):
+  SU(3) 32^4 periodic, beta = 6.0, Haar-random start (Philox seed 2),
+  50 thermalisation trajectories without Metropolis, then W warm-up and K
+  timed trajectories with Metropolis; Omelyan 2nd-order integrator (2mn),
+  epsilon = 0.5 (tau = 1), 12 substeps (49 steps: 25 link, 24 force).
+Metric: site-updates/s = V * K / wall time of the K timed trajectories.
+The headline value is the FP64 path; the FP32-mixed path (FP32 links and
+momenta, FP64 sums and Metropolis) runs the same workload and is reported
+beside it.  All inputs are resident in HBM when the timed region starts.
+
+Roofline: the dominant kernel is the fused force update (staples + TA +
+momentum step).  Its algorithmic traffic is 216 reals per site (read U and P,
+write P: 1728 B FP64, 864 B FP32); its average duration comes from HIP events
+recorded around every launch on the library's stream during the timed region.
+
+cpu_baseline: the CPU oracle (FP64, OpenMP) running one trajectory of the
+same workload on the host cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_site_update(R, n_force, n_link):
+    return R * (216 * (n_force + n_link) + 288) + 16
+
+
+def run_gpu(prec, L, steps, warmup, therm, seed, scheme, eps, substeps):
+    import numpy as np
+
+    import mesh_amd
+    from mesh_amd import gauge
+
+    ctx = mesh_amd.default_context()
+    g = mesh_amd.Grid((L, L, L, L))
+    deltas = gauge.makeDeltas(scheme, eps, substeps)
+    hmc = gauge.Hmc(g, prec, seed=seed, ctx=ctx)
+    hmc.randomizeGaugeField()
+    for _ in range(therm):
+        hmc.runHmcUpdate(6.0, deltas, metropolis=False)
+    for _ in range(warmup):
+        hmc.runHmcUpdate(6.0, deltas)
+    hmc.reset_observables()
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hmc.runHmcUpdate(6.0, deltas)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ctx.profile(False)
+    kern = {}
+    for k in ("force_update", "link_update", "delta_h", "momenta"):
+        ms, n = ctx.profile_read(k)
+        kern[k] = dict(avg_ms=ms / n if n else None, launches=n)
+    s = hmc.summary()
+    return dict(hmc=hmc, seconds=dt, V=g.size(), kernels=kern, acceptance=s["acceptance"],
+                plaquette=s["plaquette"], exp_mdH=s["exp_mdH"], dH=list(hmc.deltaH_history),
+                n_force=(len(deltas) - 1) // 2, n_link=(len(deltas) + 1) // 2)
+
+
+def same_config_compare(U64, seed, scheme, eps, substeps):
+    """One trajectory from the same FP64 config and momenta in both precisions."""
+    import mesh_amd
+    from mesh_amd import F32, F64, gauge
+
+    deltas = gauge.makeDeltas(scheme, eps, substeps)
+    out = {}
+    for prec in (F64, F32):
+        h = gauge.Hmc(U64.grid, prec, seed=seed + 1000, ctx=U64.ctx)
+        h.U = U64.convert(prec)
+        r = h.runHmcUpdate(6.0, deltas, metropolis=False)
+        out[prec] = (r["dH"], gauge.plaquette(h.U))
+    return dict(dplaq=abs(out[F32][1] - out[F64][1]), ddH=abs(out[F32][0] - out[F64][0]),
+                dH_f64=out[F64][0], plaq_f64=out[F64][1])
+
+
+def cpu_baseline(L, seed, scheme, eps, substeps):
+    from oracle import oracle as O
+
+    dims = [L] * 4
+    U = O.random_gauge(dims, seed)
+    d = O.make_deltas(scheme, eps, substeps)
+    t0 = time.perf_counter()
+    O.hmc_trajectory(U, dims, 6.0, d, seed, 0, metropolis=False)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=L ** 4 / dt, unit="site-updates/s", cores=cores, kind="port",
+                sample=f"one FP64 trajectory of the same workload ({L}^4, {scheme}, eps={eps}, "
+                       f"{substeps} substeps) from the Haar start, CPU oracle with OpenMP; {dt:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--therm", type=int, default=50)
+    ap.add_argument("--L", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--scheme", default="2mn")
+    ap.add_argument("--epsilon", type=float, default=0.5)
+    ap.add_argument("--substeps", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32-mixed run")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
+              "(the lattice is not domain-decomposed)", file=sys.stderr)
+        sys.exit(2)
+
+    from mesh_amd import F32, F64
+
+    r64 = run_gpu(F64, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
+    V = r64["V"]
+    value = V * a.steps / r64["seconds"]
+    fk = r64["kernels"]["force_update"]
+    R = 8
+    alg = 216 * R * V
+    achieved = alg / (fk["avg_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "SU(3) HMC site-updates/s at 32^4 (FP64)",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * r64["seconds"] / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: Philox-seeded Haar-random SU(3) start, seed 2, thermalised",
+        "config": {"workload": f"SU(3) Wilson HMC {a.L}^4 beta=6.0 {a.scheme} eps={a.epsilon} "
+                               f"substeps={a.substeps} (tau=1), FP64",
+                   "L": a.L, "beta": 6.0, "thermalisation": a.therm},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_force_update<double>", "avg_ms": fk["avg_ms"],
+                     "algorithmic_bytes_per_launch": alg},
+        "trajectory_roofline_frac": value * bytes_per_site_update(8, r64["n_force"], r64["n_link"]) / 8e12,
+        "acceptance": r64["acceptance"],
+        "plaquette": r64["plaquette"],
+        "exp_minus_dH": r64["exp_mdH"],
+        "kernels_f64": r64["kernels"],
+    }
+    if not a.no_f32:
+        r32 = run_gpu(F32, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
+        v32 = V * a.steps / r32["seconds"]
+        fk32 = r32["kernels"]["force_update"]
+        ach32 = 216 * 4 * V / (fk32["avg_ms"] * 1e-3) / 1e9
+        cmp = same_config_compare(r64["hmc"].U, a.seed, a.scheme, a.epsilon, a.substeps)
+        line["f32_mixed"] = {
+            "value": v32, "ms_per_step": 1e3 * r32["seconds"] / a.steps, "ratio_vs_f64": v32 / value,
+            "acceptance": r32["acceptance"], "plaquette": r32["plaquette"],
+            "force_update_frac_of_hbm": ach32 / HBM_PEAK_GBS,
+            "trajectory_roofline_frac": v32 * bytes_per_site_update(4, r32["n_force"], r32["n_link"]) / 8e12,
+            "same_config_vs_f64": cmp, "kernels": r32["kernels"],
+        }
+    if not a.no_cpu:
+        cb = cpu_baseline(a.L, a.seed, a.scheme, a.epsilon, a.substeps)
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,234 @@
+/*
+ * mesh_hip.h -- C ABI of libmesh_hip.so, the MI355X (gfx950) lattice library.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes.
+ * Device buffers are allocated through mesh_malloc (or by any HIP allocator)
+ * and hold fields in the structure-of-arrays layout described below.
+ *
+ * Each entry point names the reference (krox/mesh) interface it replaces;
+ * paths are relative to the reference root.
+ *
+ * ---------------------------------------------------------------------------
+ * Device layout of an SU(3) field with `nf` matrices per site (nf = Nd for a
+ * gauge or momentum field, 1 for a single-direction lattice such as a staple
+ * sum, a force or a gauge transformation):
+ *
+ *     plane (f, k), f < nf, k < 9 (row-major matrix entry k = 3 i + j),
+ *     holds the complex entry k of matrix f for all sites:
+ *         elem(f, k, x) = base[(f * 9 + k) * stride + x]    (complex: re, im)
+ *     stride = mesh_site_stride(grid) = V rounded up to a multiple of 64.
+ *
+ * i.e. consecutive lanes of a wavefront read consecutive sites of one plane
+ * (16 B per lane in FP64, 8 B in FP32).  Sites are row-major with dimension 0
+ * slowest (src/lattice/grid.h:39-49).
+ *
+ * Host layout (mesh_*_upload / download): the reference's AoS order,
+ * host[f][x][3][3] complex interleaved (src/lattice/lattice.h:159-192: one
+ * buffer per Lorentz direction, one Matrix<complex,3> per site).
+ *
+ * Precision: MESH_F64 stores and computes in double.  MESH_F32 stores links
+ * and momenta in float and computes the molecular dynamics in float; the
+ * per-site action, kinetic energy, Delta H and every global sum are computed
+ * in double in both modes.
+ *
+ * Every call is asynchronous on the context's stream unless it returns a
+ * host scalar (reductions), in which case it synchronises that stream.
+ * ---------------------------------------------------------------------------
+ */
+#ifndef MESH_HIP_H
+#define MESH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (src/lattice/device.cpp:10-19 throws instead) */
+enum
+{
+	MESH_OK = 0,
+	MESH_E_ARG = -1,   /* null pointer, bad direction, bad scheme ... */
+	MESH_E_SIZE = -2,  /* unsupported geometry or size */
+	MESH_E_ALLOC = -3, /* device allocation failed */
+	MESH_E_HIP = -4,   /* HIP runtime error (message in mesh_last_error) */
+	MESH_E_STATE = -5  /* invalid context */
+};
+
+enum
+{
+	MESH_F64 = 0,
+	MESH_F32 = 1
+};
+
+typedef struct mesh_ctx mesh_ctx;
+
+/* Lattice geometry: nd in [2, 4], dims[0..nd-1] >= 2 (src/lattice/grid.h:13-58). */
+typedef struct
+{
+	int nd;
+	int dims[4];
+} mesh_grid;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device` with its own stream and reduction
+ * scratch.  Replaces the implicit global CUDA state of
+ * src/lattice/device.cpp:1-76. */
+int mesh_ctx_create(mesh_ctx **ctx, int device);
+int mesh_ctx_destroy(mesh_ctx *ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char *mesh_last_error(mesh_ctx *ctx);
+/* Waits for all work queued on the context (device_synchronize,
+ * src/lattice/device.cpp:21). */
+int mesh_synchronize(mesh_ctx *ctx);
+/* The context's hipStream_t, for callers that order their own work. */
+void *mesh_stream(mesh_ctx *ctx);
+/* Kernel launch tuning: xcd_remap != 0 maps consecutive workgroups of the
+ * stencil kernels onto one XCD's contiguous site range (default 1). */
+int mesh_set_option(mesh_ctx *ctx, const char *name, long value);
+/* Per-kernel timing (replaces the util::Stopwatch globals swStaples, swExp,
+ * swPlaquette ... of src/gauge/utils.h:23, src/gauge/hmc.h:16, printed at
+ * src/apps/hmc.cpp:138-155): when enabled, launches of the HMC and Landau
+ * kernels are bracketed by HIP events on the context stream and resolved at
+ * the next synchronisation.  Kernel names: "force_update", "link_update",
+ * "delta_h", "momenta", "plaquette", "landau_or". */
+int mesh_profile_enable(mesh_ctx *ctx, int on);
+int mesh_profile_reset(mesh_ctx *ctx);
+int mesh_profile_read(mesh_ctx *ctx, const char *kernel, double *total_ms, long long *count);
+
+/* ---- memory: the untyped backend of src/lattice/device.h:14-24 ---------- */
+
+int mesh_malloc(mesh_ctx *ctx, void **ptr, size_t bytes);     /* device_malloc  device.cpp:23-31 */
+int mesh_free(mesh_ctx *ctx, void *ptr);                      /* device_free    device.cpp:33-38 */
+int mesh_memcpy(mesh_ctx *ctx, void *dst, const void *src, size_t bytes); /* device_memcpy device.cpp:40-45 */
+int mesh_memcpy_2d(mesh_ctx *ctx, void *dst, size_t dpitch, const void *src, size_t spitch,
+                   size_t width, size_t height);              /* device_memcpy_2d device.cpp:47-54 */
+int mesh_memclear(mesh_ctx *ctx, void *ptr, size_t bytes);    /* device_memclear device.cpp:70-75 */
+int mesh_memcpy_to_host(mesh_ctx *ctx, void *dst, const void *src, size_t bytes);   /* device.cpp:56-61 */
+int mesh_memcpy_from_host(mesh_ctx *ctx, void *dst, const void *src, size_t bytes); /* device.cpp:63-68 */
+
+/* ---- geometry and SU(3) field transfer ---------------------------------- */
+
+long long mesh_volume(const mesh_grid *g);
+long long mesh_site_stride(const mesh_grid *g);
+/* bytes of an SU(3) field with nf matrices per site in precision prec */
+size_t mesh_su3_field_bytes(const mesh_grid *g, int nf, int prec);
+/* host AoS (double for MESH_F64, float for MESH_F32) <-> device SoA;
+ * replaces Lattice::copy_from_host / copy_to_host (src/lattice/lattice.h:102-115) */
+int mesh_su3_upload(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *host, void *dev);
+int mesh_su3_download(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *dev, void *host);
+/* dst = src converted between precisions (same grid and nf) */
+int mesh_su3_convert(mesh_ctx *ctx, const mesh_grid *g, int nf, int dst_prec, void *dst, int src_prec,
+                     const void *src);
+/* dst[x] = 1 for every matrix (cold start, Landau::reset src/gauge/landau.h:14) */
+int mesh_su3_unit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *dst);
+
+/* ---- random fields (Philox4x32-10, counter = (site, dir, draw, stream)) --- */
+
+/* Haar-random SU(3) field: randomizeGaugeField / random_gauge_field
+ * (src/gauge/hmc.cpp:60-63, src/gauge/utils.h:43-52, groups.h:255-261).
+ * nf matrices per site. */
+int mesh_su3_random_gauge(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U, uint64_t seed,
+                          uint32_t stream);
+/* Gaussian su(3) momenta P = sum_a p_a i T^a, p_a ~ N(0,1), for trajectory
+ * `traj`; K_sites (device, V doubles, may be NULL) receives
+ * k(x) = sum_mu ||P_mu(x)||_F^2.  randomizeMomenta (src/gauge/hmc.cpp:65-71,
+ * utils.h:54-62, groups.h:245-253). */
+int mesh_su3_momenta(mesh_ctx *ctx, const mesh_grid *g, int prec, void *P, double *K_sites, uint64_t seed,
+                     uint32_t traj);
+
+/* ---- observables --------------------------------------------------------- */
+
+/* *out = normalised average plaquette in [0,1]: plaquette(U)
+ * (src/gauge/utils.h:93-108). */
+int mesh_su3_plaquette(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double *out);
+/* *out = Wilson action beta * sum_P (1 - Re tr P / 3): wilsonAction
+ * (src/gauge/utils.h:141-146). */
+int mesh_su3_wilson_action(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double beta,
+                           double *out);
+/* *out = sum_{x,mu} ||P_mu(x)||_F^2: norm2(GaugeField) (src/gauge/utils.h:35-41) */
+int mesh_su3_norm2(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *P, double *out);
+/* *out = sum_{x,mu} Re tr A_mu(x) B_mu(x): sum_real_trace(A*B) (utils.h:29) */
+int mesh_su3_sum_real_trace_prod(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *A,
+                                 const void *B, double *out);
+
+/* ---- molecular dynamics --------------------------------------------------- */
+
+/* S = staple sum of direction mu (1 matrix per site): stapleSum(U, mu)
+ * (src/gauge/utils.h:110-139). */
+int mesh_su3_staple(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, int mu, void *S);
+/* F = (beta/6) TA(U_mu S_mu) (1 matrix per site): wilsonDeriv(U, mu, beta)
+ * (src/gauge/utils.h:148-153). */
+int mesh_su3_force(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, int mu, double beta,
+                   void *F);
+/* Momentum step, all directions: P_mu -= delta (beta/6) TA(U_mu S_mu),
+ * fused staple + force + update; the loop body of runHmd's odd steps
+ * (src/gauge/hmc.h:54-58). */
+int mesh_su3_force_update(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, void *P, double beta,
+                          double delta);
+/* Link step, all directions: Uout_mu = exp(t TA(P_mu)) Uin_mu, then
+ * project_on_group_fast if reunit != 0; evolve (src/gauge/hmc.h:26-36) fused
+ * with reunitize (src/gauge/utils.h:80-91).  Uout may equal Uin. */
+int mesh_su3_link_update(mesh_ctx *ctx, const mesh_grid *g, int prec, void *Uout, const void *Uin,
+                         const void *P, double t, int reunit);
+/* U = project_on_group_fast(U) for nf matrices per site: reunitize
+ * (src/gauge/utils.h:80-91, groups.h:237-243). */
+int mesh_su3_reunit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U);
+/* U = exp(t TA(F)) U for nf = 1 fields (gauge transformation update,
+ * src/gauge/landau.h:131). */
+int mesh_su3_exp_mul(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U, const void *F, double t);
+
+/* out[5] = {Delta H, sum_x ps_old(x), sum_x ps_new(x), K_old, K_new} with
+ * per-site differencing d(x) = -(beta/3)(ps_new - ps_old)(x) + (k_new - k_old)(x),
+ * ps(x) = sum_{mu<nu} Re tr of the plaquettes at x, computed in double.
+ * Replaces H_new - H_old of src/gauge/hmc.cpp:85-90. */
+int mesh_su3_delta_h(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U_old, const void *U_new,
+                     const void *P, const double *K_old_sites, double beta, double *out);
+
+/* Integrator step table: makeDeltas (src/gauge/hmc.cpp:7-51).  scheme is
+ * "2lf", "2mn" or "4mn"; returns the length (> 0) or MESH_E_ARG; writes at
+ * most cap values. */
+int mesh_make_deltas(const char *scheme, double epsilon, int substeps, double *out, int cap);
+
+/* One HMC trajectory: runHmcUpdate (src/gauge/hmc.cpp:75-107).
+ *   *U      current field (device), *U_work a second field of the same size;
+ *           on acceptance the two pointers are swapped (std::swap, hmc.cpp:96).
+ *   P       momentum field, K_sites V doubles (device), both workspace.
+ *   metropolis == 0 accepts unconditionally (thermalisation).
+ * out[6] = {Delta H, accepted, plaquette of the resulting U, H_old, H_new, u}.
+ * The Metropolis uniform u is Philox(seed, traj); accept iff u < exp(-Delta H). */
+int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **U, void **U_work, void *P,
+                            double *K_sites, double beta, const double *deltas, int ndeltas, uint64_t seed,
+                            uint32_t traj, int metropolis, double *out);
+
+/* ---- Landau gauge (src/gauge/landau.h) ------------------------------------ */
+
+/* *out = (1/(3 Nd V)) sum_{x,mu} Re tr h(x) U_mu(x) h(x+mu)^+; h == NULL
+ * means h = 1.  Landau::operator() (src/gauge/landau.h:23-34). */
+int mesh_landau_functional(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, const void *h,
+                           double *out);
+/* *out = sum_x ||Delta(x)||_F^2, Delta(x) = sum_mu TA(U^h_mu(x)) - TA(U^h_mu(x-mu)).
+ * Landau::condition() = *out / (Nd V) (src/gauge/landau.h:37-49); the
+ * Landau condition theta = (1/V) sum_x |div A|^2 = *out / V. */
+int mesh_landau_divsum(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, const void *h,
+                       double *out);
+/* F(x) = -0.5/(3 Nd V) TA(h(x) d(x)), d(x) = sum_mu U_mu(x) h(x+mu)^+ + (h(x-mu) U_mu(x-mu))^+:
+ * Landau::deriv (src/gauge/landau.h:52-65). */
+int mesh_landau_deriv(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, const void *h, void *F);
+/* One overrelaxation half-sweep over sites of the given parity, in place on
+ * the gauge-transformed links U (and on h if not NULL): replaces the update
+ * step of Landau::run (src/gauge/landau.h:100-136); closest reference
+ * ancestor is src/old/gauge_fixing.cpp:33-41.  Needs even dims. */
+int mesh_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, int prec, void *U, void *h, int parity,
+                             double omega);
+/* U_mu(x) <- h(x) U_mu(x) h(x+mu)^+ (out of place; Uout != U). */
+int mesh_su3_gauge_transform(mesh_ctx *ctx, const mesh_grid *g, int prec, void *Uout, const void *U,
+                             const void *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MESH_HIP_H */

@@ -1,0 +1,403 @@
+// Device-side SU(3) algebra for gfx950.
+//
+// All multiply-adds are explicit fma() and the library is compiled with
+// -ffp-contract=off, so each function performs one fixed sequence of IEEE
+// operations.  The sequences are chosen to be the natural minimal ones
+// (4 fma per complex multiply-add, k-loops in ascending order) and are the
+// same sequences the CPU restatement used by the tests performs, which makes
+// FP64 molecular dynamics reproducible bit for bit across CPU and GPU.
+//
+// Reference semantics: src/gauge/groups.h:198-268 (adj, real_trace,
+// project_on_algebra, project_on_group_fast, exp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mesh_dev {
+
+template <class R> struct vec2;
+template <> struct vec2<double>
+{
+	using type = double2;
+};
+template <> struct vec2<float>
+{
+	using type = float2;
+};
+template <class R> using vec2_t = typename vec2<R>::type;
+
+template <class R> struct cpx
+{
+	R re, im;
+};
+
+template <class R> struct m3
+{
+	cpx<R> e[9]; // row-major
+};
+
+#define MD __device__ __forceinline__
+
+template <class R> MD cpx<R> c0() { return {R(0), R(0)}; }
+
+// acc += a*b
+template <class R> MD void cmadd(cpx<R> &acc, cpx<R> a, cpx<R> b)
+{
+	acc.re = fma(a.re, b.re, acc.re);
+	acc.re = fma(-a.im, b.im, acc.re);
+	acc.im = fma(a.re, b.im, acc.im);
+	acc.im = fma(a.im, b.re, acc.im);
+}
+// acc -= a*b
+template <class R> MD void cmsub(cpx<R> &acc, cpx<R> a, cpx<R> b)
+{
+	acc.re = fma(-a.re, b.re, acc.re);
+	acc.re = fma(a.im, b.im, acc.re);
+	acc.im = fma(-a.re, b.im, acc.im);
+	acc.im = fma(-a.im, b.re, acc.im);
+}
+// acc += a*conj(b)
+template <class R> MD void cmadd_cb(cpx<R> &acc, cpx<R> a, cpx<R> b)
+{
+	acc.re = fma(a.re, b.re, acc.re);
+	acc.re = fma(a.im, b.im, acc.re);
+	acc.im = fma(a.im, b.re, acc.im);
+	acc.im = fma(-a.re, b.im, acc.im);
+}
+// acc += conj(a)*b
+template <class R> MD void cmadd_ca(cpx<R> &acc, cpx<R> a, cpx<R> b)
+{
+	acc.re = fma(a.re, b.re, acc.re);
+	acc.re = fma(a.im, b.im, acc.re);
+	acc.im = fma(a.re, b.im, acc.im);
+	acc.im = fma(-a.im, b.re, acc.im);
+}
+// acc += conj(a)*conj(b)
+template <class R> MD void cmadd_cc(cpx<R> &acc, cpx<R> a, cpx<R> b)
+{
+	acc.re = fma(a.re, b.re, acc.re);
+	acc.re = fma(-a.im, b.im, acc.re);
+	acc.im = fma(-a.re, b.im, acc.im);
+	acc.im = fma(-a.im, b.re, acc.im);
+}
+template <class R> MD cpx<R> cmul(cpx<R> a, cpx<R> b)
+{
+	cpx<R> r = c0<R>();
+	cmadd(r, a, b);
+	return r;
+}
+
+// C = A B
+template <class R> MD m3<R> mul_nn(m3<R> const &A, m3<R> const &B)
+{
+	m3<R> C;
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			cpx<R> acc = c0<R>();
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+				cmadd(acc, A.e[3 * i + k], B.e[3 * k + j]);
+			C.e[3 * i + j] = acc;
+		}
+	return C;
+}
+// C = A B^+
+template <class R> MD m3<R> mul_nd(m3<R> const &A, m3<R> const &B)
+{
+	m3<R> C;
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			cpx<R> acc = c0<R>();
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+				cmadd_cb(acc, A.e[3 * i + k], B.e[3 * j + k]);
+			C.e[3 * i + j] = acc;
+		}
+	return C;
+}
+// C = A^+ B^+
+template <class R> MD m3<R> mul_dd(m3<R> const &A, m3<R> const &B)
+{
+	m3<R> C;
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			cpx<R> acc = c0<R>();
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+				cmadd_cc(acc, A.e[3 * k + i], B.e[3 * j + k]);
+			C.e[3 * i + j] = acc;
+		}
+	return C;
+}
+// S += A B
+template <class R> MD void madd_nn(m3<R> &S, m3<R> const &A, m3<R> const &B)
+{
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			cpx<R> acc = S.e[3 * i + j];
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+				cmadd(acc, A.e[3 * i + k], B.e[3 * k + j]);
+			S.e[3 * i + j] = acc;
+		}
+}
+// S += A B^+
+template <class R> MD void madd_nd(m3<R> &S, m3<R> const &A, m3<R> const &B)
+{
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			cpx<R> acc = S.e[3 * i + j];
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+				cmadd_cb(acc, A.e[3 * i + k], B.e[3 * j + k]);
+			S.e[3 * i + j] = acc;
+		}
+}
+
+template <class R> MD m3<R> zero_m3()
+{
+	m3<R> r;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+		r.e[k] = c0<R>();
+	return r;
+}
+template <class R> MD m3<R> unit_m3()
+{
+	m3<R> r = zero_m3<R>();
+	r.e[0].re = R(1);
+	r.e[4].re = R(1);
+	r.e[8].re = R(1);
+	return r;
+}
+
+// Re tr(A B^+)
+template <class R> MD R retr_nd(m3<R> const &A, m3<R> const &B)
+{
+	R s = R(0);
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		s = fma(A.e[k].re, B.e[k].re, s);
+		s = fma(A.e[k].im, B.e[k].im, s);
+	}
+	return s;
+}
+template <class R> MD R norm2_m3(m3<R> const &A)
+{
+	R s = R(0);
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		s = fma(A.e[k].re, A.e[k].re, s);
+		s = fma(A.e[k].im, A.e[k].im, s);
+	}
+	return s;
+}
+
+// traceless antihermitian part (M - M^+)/2 - tr(M - M^+)/6
+template <class R> MD m3<R> ta(m3<R> const &M)
+{
+	m3<R> A;
+	R t = ((M.e[0].im + M.e[4].im) + M.e[8].im) * (R(1) / R(3));
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+	{
+		A.e[4 * i] = {R(0), M.e[4 * i].im - t};
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+			if (j != i)
+			{
+				A.e[3 * i + j].re = R(0.5) * (M.e[3 * i + j].re - M.e[3 * j + i].re);
+				A.e[3 * i + j].im = R(0.5) * (M.e[3 * i + j].im + M.e[3 * j + i].im);
+			}
+	}
+	return A;
+}
+
+template <class R> MD cpx<R> det(m3<R> const &X)
+{
+	cpx<R> m0 = c0<R>(), m1 = c0<R>(), m2 = c0<R>();
+	cmadd(m0, X.e[4], X.e[8]);
+	cmsub(m0, X.e[5], X.e[7]);
+	cmadd(m1, X.e[3], X.e[8]);
+	cmsub(m1, X.e[5], X.e[6]);
+	cmadd(m2, X.e[3], X.e[7]);
+	cmsub(m2, X.e[4], X.e[6]);
+	cpx<R> d = c0<R>();
+	cmadd(d, X.e[0], m0);
+	cmsub(d, X.e[1], m1);
+	cmadd(d, X.e[2], m2);
+	return d;
+}
+
+template <class R> constexpr int exp_terms() { return sizeof(R) == 8 ? 18 : 10; }
+
+// exp of a traceless antihermitian matrix: Cayley-Hamilton with
+// power-series coefficients (X^3 = a X + b with a = tr X^2 / 2, b = det X),
+// argument halved until ||X||_F <= 1 and the result squared back.
+template <class R> MD m3<R> expsu3(m3<R> X)
+{
+	m3<R> X2 = mul_nn(X, X);
+	R trx2 = (X2.e[0].re + X2.e[4].re) + X2.e[8].re;
+	R r2 = -trx2;
+	int s = 0;
+	R sc = R(1);
+	// bounded: an infinite or NaN argument must not loop forever
+	while (r2 > R(1) && s < 64)
+	{
+		r2 *= R(0.25);
+		sc *= R(0.5);
+		++s;
+	}
+	if (s > 0)
+	{
+		R sc2 = sc * sc;
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+		{
+			X.e[k].re *= sc;
+			X.e[k].im *= sc;
+			X2.e[k].re *= sc2;
+			X2.e[k].im *= sc2;
+		}
+		trx2 *= sc2;
+	}
+	R a = R(0.5) * trx2;
+	cpx<R> b = det(X);
+	cpx<R> al = {R(1), R(0)}, be = c0<R>(), ga = c0<R>();
+	cpx<R> f0 = {R(1), R(0)}, f1 = c0<R>(), f2 = c0<R>();
+#pragma unroll
+	for (int n = 0; n < exp_terms<R>(); ++n)
+	{
+		const R inv = R(1) / R(n + 1);
+		cpx<R> gb = cmul(ga, b);
+		cpx<R> nal = {gb.re * inv, gb.im * inv};
+		cpx<R> nbe = {fma(ga.re, a, al.re) * inv, fma(ga.im, a, al.im) * inv};
+		cpx<R> nga = {be.re * inv, be.im * inv};
+		al = nal;
+		be = nbe;
+		ga = nga;
+		f0.re += al.re;
+		f0.im += al.im;
+		f1.re += be.re;
+		f1.im += be.im;
+		f2.re += ga.re;
+		f2.im += ga.im;
+	}
+	m3<R> E;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		cpx<R> acc = (k % 4 == 0) ? f0 : c0<R>();
+		cmadd(acc, f1, X.e[k]);
+		cmadd(acc, f2, X2.e[k]);
+		E.e[k] = acc;
+	}
+	for (int k = 0; k < s; ++k)
+		E = mul_nn(E, E);
+	return E;
+}
+
+// project_on_group_fast: r = 1.5 a - 0.5 a a^+ a, r *= 4/3 - det(r)/3
+template <class R> MD m3<R> reunit(m3<R> const &A)
+{
+	m3<R> W = mul_nd(A, A);
+	m3<R> X = mul_nn(W, A);
+	m3<R> r;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		r.e[k].re = fma(R(-0.5), X.e[k].re, R(1.5) * A.e[k].re);
+		r.e[k].im = fma(R(-0.5), X.e[k].im, R(1.5) * A.e[k].im);
+	}
+	cpx<R> d = det(r);
+	cpx<R> c = {fma(-(R(1) / R(3)), d.re, R(4) / R(3)), -(R(1) / R(3)) * d.im};
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+		r.e[k] = cmul(r.e[k], c);
+	return r;
+}
+
+template <class R> MD m3<double> to_f64(m3<R> const &a)
+{
+	m3<double> r;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+		r.e[k] = {(double)a.e[k].re, (double)a.e[k].im};
+	return r;
+}
+template <class R> MD m3<R> from_f64(m3<double> const &a)
+{
+	m3<R> r;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+		r.e[k] = {(R)a.e[k].re, (R)a.e[k].im};
+	return r;
+}
+
+// ---- SoA field access ------------------------------------------------------
+
+// field base pointer, matrix f of the site, plane stride in complex elements
+template <class R> MD m3<R> load(const R *__restrict__ base, int f, long stride, long x)
+{
+	const vec2_t<R> *p = reinterpret_cast<const vec2_t<R> *>(base) + (long)(f * 9) * stride + x;
+	m3<R> m;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		vec2_t<R> v = p[k * stride];
+		m.e[k] = {v.x, v.y};
+	}
+	return m;
+}
+template <class R> MD void store(R *__restrict__ base, int f, long stride, long x, m3<R> const &m)
+{
+	vec2_t<R> *p = reinterpret_cast<vec2_t<R> *>(base) + (long)(f * 9) * stride + x;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		vec2_t<R> v;
+		v.x = m.e[k].re;
+		v.y = m.e[k].im;
+		p[k * stride] = v;
+	}
+}
+
+// ---- geometry ---------------------------------------------------------------
+
+struct Geo
+{
+	int nd;
+	int L[4];
+	int st[4]; // site strides, st[nd-1] = 1
+	long V;
+	long stride; // plane stride (V rounded up)
+};
+
+MD int coord(Geo const &g, long x, int mu) { return (int)((x / g.st[mu]) % g.L[mu]); }
+MD long shift_c(Geo const &g, long x, int mu, int c, int s)
+{
+	int n = c + s;
+	n = n >= g.L[mu] ? n - g.L[mu] : (n < 0 ? n + g.L[mu] : n);
+	return x + (long)(n - c) * g.st[mu];
+}
+
+#undef MD
+
+} // namespace mesh_dev

@@ -1,0 +1,683 @@
+// SU(3) pure-gauge Wilson-action HMC kernels for gfx950.
+//
+// One kernel per fused step of the trajectory, all over structure-of-arrays
+// fields (plane (f,k) = complex entry k of matrix f for every site):
+//   k_momenta       P ~ N(0,1) su(3) momenta, k(x) = sum_mu ||P_mu(x)||^2
+//   k_force_update  P_mu -= delta beta/6 TA(U_mu Sigma_mu), staples in-kernel
+//   k_link_update   U_mu <- exp(t TA(P_mu)) U_mu (+ reunitarisation)
+//   k_delta_h       per-site differenced Delta H and both plaquette sums
+//   k_plaq          per-site plaquette sum, block-reduced
+// Neighbours are addressed in-kernel (no shifted copies as in the
+// reference's cshift, src/lattice/lattice.h:366-389).
+#include "mesh_internal.h"
+#include "rng_dev.h"
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace mesh_dev;
+
+namespace {
+
+constexpr int SB = 64; // sites per workgroup in the stencil kernels (one wavefront)
+
+template <class R> __device__ __forceinline__ void coords(Geo const &g, long x, int *c)
+{
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
+		c[mu] = mu < g.nd ? coord(g, x, mu) : 0;
+}
+
+// staple sum of direction mu at site x with coordinates c
+// (src/gauge/utils.h:110-139)
+template <class R> __device__ __forceinline__ m3<R> staple_at(Geo const &g, const R *__restrict__ U, int mu,
+                                                            long x, int const *c)
+{
+	m3<R> S = zero_m3<R>();
+	long xpm = shift_c(g, x, mu, c[mu], 1);
+#pragma unroll
+	for (int nu = 0; nu < 4; ++nu)
+	{
+		if (nu == mu || nu >= g.nd)
+			continue;
+		long xpn = shift_c(g, x, nu, c[nu], 1);
+		long xmn = shift_c(g, x, nu, c[nu], -1);
+		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
+		{
+			m3<R> T1 = mul_nd(load(U, nu, g.stride, xpm), load(U, mu, g.stride, xpn));
+			madd_nd(S, T1, load(U, nu, g.stride, x));
+		}
+		{
+			m3<R> T2 = mul_dd(load(U, nu, g.stride, xpmmn), load(U, mu, g.stride, xmn));
+			madd_nn(S, T2, load(U, nu, g.stride, xmn));
+		}
+	}
+	return S;
+}
+
+// Re tr of all plaquettes at x, in double
+template <class R> __device__ __forceinline__ double plaq_site(Geo const &g, const R *__restrict__ U, long x,
+                                                             int const *c)
+{
+	double s = 0.0;
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+		for (int nu = mu + 1; nu < 4; ++nu)
+		{
+			if (nu >= g.nd)
+				continue;
+			long xpm = shift_c(g, x, mu, c[mu], 1), xpn = shift_c(g, x, nu, c[nu], 1);
+			m3<double> A = mul_nn(to_f64(load(U, mu, g.stride, x)), to_f64(load(U, nu, g.stride, xpm)));
+			m3<double> B = mul_nn(to_f64(load(U, nu, g.stride, x)), to_f64(load(U, mu, g.stride, xpn)));
+			s += retr_nd(A, B);
+		}
+	return s;
+}
+
+template <class R> __device__ __forceinline__ double kin_site(Geo const &g, const R *__restrict__ P, long x)
+{
+	double k = 0.0;
+	for (int mu = 0; mu < g.nd; ++mu)
+		k += norm2_m3(to_f64(load(P, mu, g.stride, x)));
+	return k;
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <class R> __global__ __launch_bounds__(256) void k_random_gauge(Geo g, int nf, R *U, uint64_t seed,
+                                                                        uint32_t tag)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= g.V * nf)
+		return;
+	int f = (int)(i / g.V);
+	long x = i - (long)f * g.V;
+	double z[18];
+	link_normals<18>(seed, tag, 0u, (uint32_t)x, (uint32_t)f, z);
+	store(U, f, g.stride, x, from_f64<R>(haar_from_normals(z)));
+}
+
+template <class R> __global__ __launch_bounds__(256) void k_momenta(Geo g, R *P, double *K, uint64_t seed,
+                                                                   uint32_t traj)
+{
+	long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= g.V)
+		return;
+	double k = 0.0;
+	for (int mu = 0; mu < g.nd; ++mu)
+	{
+		double z[8];
+		link_normals<8>(seed, TAG_MOM, traj, (uint32_t)x, (uint32_t)mu, z);
+		m3<R> p = from_f64<R>(algebra_from_normals(z));
+		store(P, mu, g.stride, x, p);
+		k += norm2_m3(to_f64(p));
+	}
+	if (K)
+		K[x] = k;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_plaq(Geo g, const R *__restrict__ U, double *partials, int nb, int remap)
+{
+	int lb = logical_block(nb, remap);
+	if (lb >= nb)
+		return;
+	long x = (long)lb * 256 + threadIdx.x;
+	double v[1] = {0.0};
+	if (x < g.V)
+	{
+		int c[4];
+		coords<R>(g, x, c);
+		v[0] = plaq_site(g, U, x, c);
+	}
+	block_sum_store<1>(v, partials, nb, lb);
+}
+
+// partial sums of (d(x), ps_old(x), ps_new(x), k_new(x), k_old(x))
+template <class R>
+__global__ __launch_bounds__(256) void k_delta_h(Geo g, const R *__restrict__ Uo, const R *__restrict__ Un,
+                                                 const R *__restrict__ P, const double *__restrict__ Ko,
+                                                 double c3, double *partials, int nb, int remap)
+{
+	int lb = logical_block(nb, remap);
+	if (lb >= nb)
+		return;
+	long x = (long)lb * 256 + threadIdx.x;
+	double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+	if (x < g.V)
+	{
+		int c[4];
+		coords<R>(g, x, c);
+		double pso = plaq_site(g, Uo, x, c);
+		double psn = plaq_site(g, Un, x, c);
+		double kn = kin_site(g, P, x);
+		double dps = psn - pso;
+		double dk = kn - Ko[x];
+		v[0] = fma(-c3, dps, dk);
+		v[1] = pso;
+		v[2] = psn;
+		v[3] = kn;
+		v[4] = Ko[x];
+	}
+	block_sum_store<5>(v, partials, nb, lb);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_staple(Geo g, const R *__restrict__ U, int mu, R *__restrict__ S)
+{
+	long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= g.V)
+		return;
+	int c[4];
+	coords<R>(g, x, c);
+	store(S, 0, g.stride, x, staple_at(g, U, mu, x, c));
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_force(Geo g, const R *__restrict__ U, int mu, R coef, R *__restrict__ F)
+{
+	long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (x >= g.V)
+		return;
+	int c[4];
+	coords<R>(g, x, c);
+	m3<R> A = ta(mul_nn(load(U, mu, g.stride, x), staple_at(g, U, mu, x, c)));
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		A.e[k].re *= coef;
+		A.e[k].im *= coef;
+	}
+	store(F, 0, g.stride, x, A);
+}
+
+// fused momentum step: workgroup = 64 sites x 4 directions, wavefront w
+// handles direction w of the workgroup's sites
+template <class R>
+__global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
+                                                      int nb, int remap)
+{
+	int lb = logical_block(nb, remap);
+	if (lb >= nb)
+		return;
+	int mu = threadIdx.x >> 6;
+	long x = (long)lb * SB + (threadIdx.x & 63);
+	if (mu >= g.nd || x >= g.V)
+		return;
+	int c[4];
+	coords<R>(g, x, c);
+	m3<R> S = staple_at(g, U, mu, x, c);
+	m3<R> A = ta(mul_nn(load(U, mu, g.stride, x), S));
+	m3<R> p = load(P, mu, g.stride, x);
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
+		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
+	}
+	store(P, mu, g.stride, x, p);
+}
+
+// link step over nf*V matrices: Uout = exp(t TA(P)) Uin (+ reunitarise)
+template <class R>
+__global__ __launch_bounds__(256) void k_link_update(Geo g, int nf, R *Uout, const R *Uin, const R *__restrict__ P,
+                                                     R t, int reunit_flag)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= g.V * nf)
+		return;
+	int f = (int)(i / g.V);
+	long x = i - (long)f * g.V;
+	m3<R> X = ta(load(P, f, g.stride, x));
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		X.e[k].re *= t;
+		X.e[k].im *= t;
+	}
+	m3<R> u = mul_nn(expsu3(X), load(Uin, f, g.stride, x));
+	if (reunit_flag)
+		u = reunit(u);
+	store(Uout, f, g.stride, x, u);
+}
+
+template <class R> __global__ __launch_bounds__(256) void k_reunit(Geo g, int nf, R *U)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= g.V * nf)
+		return;
+	int f = (int)(i / g.V);
+	long x = i - (long)f * g.V;
+	store(U, f, g.stride, x, reunit(load(U, f, g.stride, x)));
+}
+
+// sum over f,x of ||A||^2 (B == nullptr) or Re tr(A B), partials per block
+template <class R>
+__global__ __launch_bounds__(256) void k_trace_sum(Geo g, int nf, const R *__restrict__ A, const R *__restrict__ B,
+                                                   double *partials, int nb)
+{
+	long x = (long)blockIdx.x * 256 + threadIdx.x;
+	double v[1] = {0.0};
+	if (x < g.V)
+	{
+		double s = 0.0;
+		for (int f = 0; f < nf; ++f)
+		{
+			m3<double> a = to_f64(load(A, f, g.stride, x));
+			if (!B)
+				s += norm2_m3(a);
+			else
+			{
+				m3<double> p = mul_nn(a, to_f64(load(B, f, g.stride, x)));
+				s += (p.e[0].re + p.e[4].re) + p.e[8].re;
+			}
+		}
+		v[0] = s;
+	}
+	block_sum_store<1>(v, partials, nb, blockIdx.x);
+}
+
+// ---------------------------------------------------------------- host helpers
+
+struct Setup
+{
+	Geo geo;
+	int rc;
+};
+
+Setup setup(mesh_ctx *ctx, const mesh_grid *g, int prec)
+{
+	Setup s;
+	s.rc = mesh_make_geo(ctx, g, &s.geo);
+	if (!s.rc)
+		s.rc = mesh_check_prec(ctx, prec);
+	return s;
+}
+
+inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+int plaq_sum(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, double *out)
+{
+	int nb = nblk(geo.V, 256);
+	if (int rc = mesh_reserve_partials(ctx, nb))
+		return rc;
+	int gb = mesh_grid_blocks(ctx, nb);
+	int ph = mesh_prof_begin(ctx, PROF_PLAQ);
+	if (prec == MESH_F64)
+		k_plaq<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, ctx->partials, nb, ctx->xcd_remap);
+	else
+		k_plaq<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, ctx->partials, nb, ctx->xcd_remap);
+	mesh_prof_end(ctx, ph);
+	MESH_LAUNCHED(ctx);
+	return mesh_finish_reduction(ctx, nb, 1, out);
+}
+
+double n_plaquettes(Geo const &geo) { return (double)geo.V * geo.nd * (geo.nd - 1) / 2; }
+
+int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, void *P, double beta, double delta)
+{
+	int nb = nblk(geo.V, SB);
+	int gb = mesh_grid_blocks(ctx, nb);
+	double k = delta * (beta / 6.0);
+	int ph = mesh_prof_begin(ctx, PROF_FORCE_UPDATE);
+	if (prec == MESH_F64)
+		k_force_update<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, (double *)P, k, nb,
+		                                                     ctx->xcd_remap);
+	else
+		k_force_update<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, (float)k, nb,
+		                                                    ctx->xcd_remap);
+	mesh_prof_end(ctx, ph);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int launch_link_update(mesh_ctx *ctx, Geo const &geo, int nf, int prec, void *Uout, const void *Uin, const void *P,
+                       double t, int reunit_flag)
+{
+	int nb = nblk(geo.V * nf, 256);
+	int ph = mesh_prof_begin(ctx, PROF_LINK_UPDATE);
+	if (prec == MESH_F64)
+		k_link_update<double><<<nb, 256, 0, ctx->stream>>>(geo, nf, (double *)Uout, (const double *)Uin,
+		                                                   (const double *)P, t, reunit_flag);
+	else
+		k_link_update<float><<<nb, 256, 0, ctx->stream>>>(geo, nf, (float *)Uout, (const float *)Uin,
+		                                                  (const float *)P, (float)t, reunit_flag);
+	mesh_prof_end(ctx, ph);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int launch_momenta(mesh_ctx *ctx, Geo const &geo, int prec, void *P, double *K, uint64_t seed, uint32_t traj)
+{
+	int nb = nblk(geo.V, 256);
+	int ph = mesh_prof_begin(ctx, PROF_MOMENTA);
+	if (prec == MESH_F64)
+		k_momenta<double><<<nb, 256, 0, ctx->stream>>>(geo, (double *)P, K, seed, traj);
+	else
+		k_momenta<float><<<nb, 256, 0, ctx->stream>>>(geo, (float *)P, K, seed, traj);
+	mesh_prof_end(ctx, ph);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int delta_h(mesh_ctx *ctx, Geo const &geo, int prec, const void *Uo, const void *Un, const void *P, const double *Ko,
+            double beta, double *out)
+{
+	int nb = nblk(geo.V, 256);
+	if (int rc = mesh_reserve_partials(ctx, 5 * (size_t)nb))
+		return rc;
+	int gb = mesh_grid_blocks(ctx, nb);
+	double c3 = beta / 3.0;
+	int ph = mesh_prof_begin(ctx, PROF_DELTA_H);
+	if (prec == MESH_F64)
+		k_delta_h<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)Uo, (const double *)Un,
+		                                               (const double *)P, Ko, c3, ctx->partials, nb,
+		                                               ctx->xcd_remap);
+	else
+		k_delta_h<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)Uo, (const float *)Un,
+		                                              (const float *)P, Ko, c3, ctx->partials, nb,
+		                                              ctx->xcd_remap);
+	mesh_prof_end(ctx, ph);
+	MESH_LAUNCHED(ctx);
+	double r[5];
+	if (int rc = mesh_finish_reduction(ctx, nb, 5, r))
+		return rc;
+	out[0] = r[0];
+	out[1] = r[1];
+	out[2] = r[2];
+	out[3] = r[4];
+	out[4] = r[3];
+	return MESH_OK;
+}
+
+// Metropolis uniform of trajectory traj: Philox(seed; traj, 0, 0, METRO)
+double metropolis_uniform(uint64_t seed, uint32_t traj)
+{
+	u4 c = {traj, 0u, 0u, TAG_METRO};
+	u4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+	return u01_from_bits(r.x, r.y);
+}
+
+} // namespace
+
+extern "C" {
+
+int mesh_su3_random_gauge(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U, uint64_t seed,
+                          uint32_t stream)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || nf < 1)
+		return mesh_fail(ctx, MESH_E_ARG, "bad random_gauge arguments");
+	uint32_t tag = stream ? stream : TAG_HAAR;
+	int nb = nblk(s.geo.V * nf, 256);
+	if (prec == MESH_F64)
+		k_random_gauge<double><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (double *)U, seed, tag);
+	else
+		k_random_gauge<float><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (float *)U, seed, tag);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_su3_momenta(mesh_ctx *ctx, const mesh_grid *g, int prec, void *P, double *K, uint64_t seed, uint32_t traj)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!P)
+		return mesh_fail(ctx, MESH_E_ARG, "P is NULL");
+	return launch_momenta(ctx, s.geo, prec, P, K, seed, traj);
+}
+
+int mesh_su3_plaquette(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || !out)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to plaquette");
+	double sum;
+	if (int rc = plaq_sum(ctx, s.geo, prec, U, &sum))
+		return rc;
+	*out = sum / (n_plaquettes(s.geo) * 3.0);
+	return MESH_OK;
+}
+
+int mesh_su3_wilson_action(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double beta, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || !out)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to wilson_action");
+	double sum;
+	if (int rc = plaq_sum(ctx, s.geo, prec, U, &sum))
+		return rc;
+	*out = beta * (n_plaquettes(s.geo) - sum / 3.0);
+	return MESH_OK;
+}
+
+static int trace_sum(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *A, const void *B, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!A || !out || nf < 1)
+		return mesh_fail(ctx, MESH_E_ARG, "bad trace-sum arguments");
+	int nb = nblk(s.geo.V, 256);
+	if (int rc = mesh_reserve_partials(ctx, nb))
+		return rc;
+	if (prec == MESH_F64)
+		k_trace_sum<double><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (const double *)A, (const double *)B,
+		                                                 ctx->partials, nb);
+	else
+		k_trace_sum<float><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (const float *)A, (const float *)B,
+		                                                ctx->partials, nb);
+	MESH_LAUNCHED(ctx);
+	return mesh_finish_reduction(ctx, nb, 1, out);
+}
+
+int mesh_su3_norm2(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *P, double *out)
+{
+	return trace_sum(ctx, g, nf, prec, P, nullptr, out);
+}
+
+int mesh_su3_sum_real_trace_prod(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *A, const void *B,
+                                 double *out)
+{
+	if (!B)
+		return mesh_fail(ctx, MESH_E_ARG, "B is NULL");
+	return trace_sum(ctx, g, nf, prec, A, B, out);
+}
+
+int mesh_su3_staple(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, int mu, void *S)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || !S || mu < 0 || mu >= s.geo.nd)
+		return mesh_fail(ctx, MESH_E_ARG, "bad staple arguments");
+	int nb = nblk(s.geo.V, 256);
+	if (prec == MESH_F64)
+		k_staple<double><<<nb, 256, 0, ctx->stream>>>(s.geo, (const double *)U, mu, (double *)S);
+	else
+		k_staple<float><<<nb, 256, 0, ctx->stream>>>(s.geo, (const float *)U, mu, (float *)S);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_su3_force(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, int mu, double beta, void *F)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || !F || mu < 0 || mu >= s.geo.nd)
+		return mesh_fail(ctx, MESH_E_ARG, "bad force arguments");
+	int nb = nblk(s.geo.V, 256);
+	double c = beta / 6.0;
+	if (prec == MESH_F64)
+		k_force<double><<<nb, 256, 0, ctx->stream>>>(s.geo, (const double *)U, mu, c, (double *)F);
+	else
+		k_force<float><<<nb, 256, 0, ctx->stream>>>(s.geo, (const float *)U, mu, (float)c, (float *)F);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_su3_force_update(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, void *P, double beta,
+                          double delta)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || !P)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to force_update");
+	return launch_force_update(ctx, s.geo, prec, U, P, beta, delta);
+}
+
+int mesh_su3_link_update(mesh_ctx *ctx, const mesh_grid *g, int prec, void *Uout, const void *Uin, const void *P,
+                         double t, int reunit_flag)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!Uout || !Uin || !P)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to link_update");
+	return launch_link_update(ctx, s.geo, s.geo.nd, prec, Uout, Uin, P, t, reunit_flag);
+}
+
+int mesh_su3_exp_mul(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U, const void *F, double t)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || !F || nf < 1)
+		return mesh_fail(ctx, MESH_E_ARG, "bad exp_mul arguments");
+	return launch_link_update(ctx, s.geo, nf, prec, U, U, F, t, 0);
+}
+
+int mesh_su3_reunit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || nf < 1)
+		return mesh_fail(ctx, MESH_E_ARG, "bad reunit arguments");
+	int nb = nblk(s.geo.V * nf, 256);
+	if (prec == MESH_F64)
+		k_reunit<double><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (double *)U);
+	else
+		k_reunit<float><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (float *)U);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_su3_delta_h(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *Uo, const void *Un, const void *P,
+                     const double *Ko, double beta, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!Uo || !Un || !P || !Ko || !out)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to delta_h");
+	return delta_h(ctx, s.geo, prec, Uo, Un, P, Ko, beta, out);
+}
+
+int mesh_make_deltas(const char *scheme, double epsilon, int substeps, double *out, int cap)
+{
+	if (!scheme || !out || substeps < 1 || !(epsilon > 0))
+		return MESH_E_ARG;
+	std::vector<double> raw;
+	if (!std::strcmp(scheme, "2lf"))
+		raw = {0.5, 1.0, 0.5};
+	else if (!std::strcmp(scheme, "2mn"))
+	{
+		double l = 0.19318332750378361;
+		raw = {l, 0.5, 1.0 - 2.0 * l, 0.5, l};
+	}
+	else if (!std::strcmp(scheme, "4mn"))
+	{
+		double rho = 0.1786178958448091, theta = -0.06626458266981843, lam = 0.7123418310626056;
+		raw = {rho, lam, theta, 0.5 - lam, 1.0 - 2.0 * (rho + theta), 0.5 - lam, theta, lam, rho};
+	}
+	else
+		return MESH_E_ARG;
+	// Langevin-time epsilon to MD step: delta = sqrt(2 epsilon) / substeps;
+	// consecutive substeps share their boundary position step
+	double delta = std::sqrt(2 * epsilon) / substeps;
+	std::vector<double> d;
+	for (int i = 0; i < substeps; ++i)
+	{
+		if (i > 0)
+			d.back() *= 2;
+		d.insert(d.end(), raw.begin() + (i > 0 ? 1 : 0), raw.end());
+	}
+	if ((int)d.size() > cap)
+		return MESH_E_SIZE;
+	for (size_t i = 0; i < d.size(); ++i)
+		out[i] = d[i] * delta;
+	return (int)d.size();
+}
+
+int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **U, void **U_work, void *P,
+                            double *K, double beta, const double *deltas, int nd_, uint64_t seed, uint32_t traj,
+                            int metropolis, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!U || !U_work || !*U || !*U_work || !P || !K || !deltas || !out)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to hmc_trajectory");
+	if (nd_ < 1 || nd_ % 2 != 1)
+		return mesh_fail(ctx, MESH_E_ARG, "deltas must have odd length (src/gauge/hmc.h:44)");
+	if (*U == *U_work)
+		return mesh_fail(ctx, MESH_E_ARG, "U and U_work must differ");
+	int rc;
+	if ((rc = launch_momenta(ctx, s.geo, prec, P, K, seed, traj)))
+		return rc;
+	for (int i = 0; i < nd_; ++i)
+	{
+		if (i % 2 == 0)
+			rc = launch_link_update(ctx, s.geo, s.geo.nd, prec, *U_work, i == 0 ? *U : *U_work, P, deltas[i],
+			                        i == nd_ - 1);
+		else
+			rc = launch_force_update(ctx, s.geo, prec, *U_work, P, beta, deltas[i]);
+		if (rc)
+			return rc;
+	}
+	double dh[5];
+	if ((rc = delta_h(ctx, s.geo, prec, *U, *U_work, P, K, beta, dh)))
+		return rc;
+	double u = metropolis_uniform(seed, traj);
+	int acc = metropolis ? (u < std::exp(-dh[0])) : 1;
+	if (acc)
+	{
+		void *t = *U;
+		*U = *U_work;
+		*U_work = t;
+	}
+	double np = n_plaquettes(s.geo);
+	out[0] = dh[0];
+	out[1] = acc;
+	out[2] = (acc ? dh[2] : dh[1]) / (np * 3.0);
+	out[3] = beta * (np - dh[1] / 3.0) + dh[3];
+	out[4] = beta * (np - dh[2] / 3.0) + dh[4];
+	out[5] = u;
+	return MESH_OK;
+}
+
+} // extern "C"

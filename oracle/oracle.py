@@ -1,0 +1,307 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle_mesh.so).
+
+TEST INFRASTRUCTURE ONLY.  The oracle restates krox/mesh's lattice arithmetic
+on the CPU (see oracle_mesh.cpp for the reference file:line each function
+follows).  Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
+leg of ``bench.py`` import this module, and only as a checker; the product
+path (``mesh_amd``) never does.
+
+Arrays use the host layout U[Nd][V][3][3] complex (numpy complex128 or
+complex64 of shape (Nd, V, 3, 3)), sites row-major with dimension 0 slowest.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_mesh.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def _declare(L):
+    L.orc_make_deltas.argtypes = [C.c_char_p, C.c_double, C.c_int, _dp, C.c_int]
+    L.orc_make_deltas.restype = C.c_int
+    L.orc_philox4x32_10.argtypes = [_up, _up, _up]
+    L.orc_det_log.argtypes = [C.c_double]
+    L.orc_det_log.restype = C.c_double
+    L.orc_det_sincos2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_link_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _dp]
+    L.orc_metropolis_uniform.argtypes = [C.c_uint64, C.c_uint32]
+    L.orc_metropolis_uniform.restype = C.c_double
+    for n in ["orc_su3_mul_nn"]:
+        getattr(L, n).argtypes = [_dp, _dp, _dp]
+    for n in ["orc_su3_ta", "orc_su3_exp", "orc_su3_reunit", "orc_su3_det",
+              "orc_su3_haar_from_normals", "orc_su3_algebra_from_normals"]:
+        getattr(L, n).argtypes = [_dp, _dp]
+    L.orc_su3_exp_f32.argtypes = [_fp, _fp]
+    for sfx, rp in (("f64", _dp), ("f32", _fp)):
+        g = lambda n: getattr(L, n + "_" + sfx)
+        g("orc_su3_random_gauge").argtypes = [rp, _ip, C.c_int, C.c_uint64]
+        g("orc_su3_momenta").argtypes = [rp, _dp, _ip, C.c_int, C.c_uint64, C.c_uint32]
+        g("orc_su3_plaq_sites").argtypes = [rp, _ip, C.c_int, _dp]
+        g("orc_su3_plaquette").argtypes = [rp, _ip, C.c_int]
+        g("orc_su3_plaquette").restype = C.c_double
+        g("orc_su3_staple").argtypes = [rp, _ip, C.c_int, C.c_int, rp]
+        g("orc_su3_force_update").argtypes = [rp, rp, _ip, C.c_int, C.c_double, C.c_double]
+        g("orc_su3_link_update").argtypes = [rp, rp, rp, _ip, C.c_int, C.c_double, C.c_int]
+        g("orc_su3_reunit_field").argtypes = [rp, C.c_int64]
+        g("orc_su3_kinetic_sites").argtypes = [rp, _ip, C.c_int, _dp]
+        g("orc_su3_delta_h").argtypes = [rp, rp, rp, _dp, _ip, C.c_int, C.c_double, _dp]
+        g("orc_su3_hmc_trajectory").argtypes = [rp, _ip, C.c_int, C.c_double, _dp, C.c_int,
+                                                C.c_uint64, C.c_uint32, C.c_int, _dp]
+        g("orc_landau_functional").argtypes = [rp, C.c_void_p, _ip, C.c_int]
+        g("orc_landau_functional").restype = C.c_double
+        g("orc_landau_divsum").argtypes = [rp, C.c_void_p, _ip, C.c_int]
+        g("orc_landau_divsum").restype = C.c_double
+        g("orc_landau_or_halfsweep").argtypes = [rp, C.c_void_p, _ip, C.c_int, C.c_int, C.c_double]
+    L.orc_su3_force_f64.argtypes = [_dp, _ip, C.c_int, C.c_int, C.c_double, _dp]
+    L.orc_landau_deriv_f64.argtypes = [_dp, _dp, _ip, C.c_int, _dp]
+    L.orc_su3_random_transform_f64.argtypes = [_dp, _ip, C.c_int, C.c_uint64]
+    L.orc_su3_gauge_transform_f64.argtypes = [_dp, _dp, _ip, C.c_int]
+
+
+# ---------------------------------------------------------------- helpers
+
+def _real(a):
+    """complex (..., 3, 3) array -> contiguous real view (float64/float32)."""
+    a = np.ascontiguousarray(a)
+    return a.view(np.float64 if a.dtype == np.complex128 else np.float32)
+
+
+def _sfx(a):
+    return "f64" if a.dtype in (np.complex128, np.float64) else "f32"
+
+
+def _dims(dims):
+    return np.ascontiguousarray(dims, dtype=np.int32)
+
+
+def volume(dims):
+    return int(np.prod(dims))
+
+
+def make_deltas(scheme, epsilon, substeps):
+    out = np.zeros(4096)
+    n = lib().orc_make_deltas(scheme.encode(), float(epsilon), int(substeps), out, out.size)
+    if n < 0:
+        raise ValueError(f"unknown scheme '{scheme}'")
+    return out[:n].copy()
+
+
+def philox(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def det_log(x):
+    return lib().orc_det_log(float(x))
+
+
+def det_sincos2pi(u):
+    s, c = C.c_double(), C.c_double()
+    lib().orc_det_sincos2pi(float(u), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def link_normals(seed, tag, ctr2, site, mu, n):
+    z = np.zeros(n)
+    lib().orc_link_normals(seed, tag, ctr2, site, mu, n, z)
+    return z
+
+
+def metropolis_uniform(seed, traj):
+    return lib().orc_metropolis_uniform(seed, traj)
+
+
+def _unary(name, a):
+    a = np.ascontiguousarray(a, np.complex128)
+    out = np.zeros(a.shape[:-2] + (3, 3), np.complex128) if name != "orc_su3_det" else np.zeros(1, np.complex128)
+    getattr(lib(), name)(_real(a), _real(out))
+    return out
+
+
+def su3_mul(a, b):
+    out = np.zeros((3, 3), np.complex128)
+    lib().orc_su3_mul_nn(_real(np.asarray(a, np.complex128)), _real(np.asarray(b, np.complex128)), _real(out))
+    return out
+
+
+def su3_ta(m):
+    return _unary("orc_su3_ta", m)
+
+
+def su3_exp(x):
+    if np.asarray(x).dtype == np.complex64:
+        x = np.ascontiguousarray(x)
+        out = np.zeros((3, 3), np.complex64)
+        lib().orc_su3_exp_f32(_real(x), _real(out))
+        return out
+    return _unary("orc_su3_exp", x)
+
+
+def su3_reunit(a):
+    return _unary("orc_su3_reunit", a)
+
+
+def su3_det(a):
+    return _unary("orc_su3_det", a)[0]
+
+
+def haar_from_normals(z):
+    out = np.zeros((3, 3), np.complex128)
+    lib().orc_su3_haar_from_normals(np.ascontiguousarray(z, np.float64), _real(out))
+    return out
+
+
+def algebra_from_normals(p):
+    out = np.zeros((3, 3), np.complex128)
+    lib().orc_su3_algebra_from_normals(np.ascontiguousarray(p, np.float64), _real(out))
+    return out
+
+
+def _cdt(prec):
+    return np.complex128 if prec in ("f64", 64, "double") else np.complex64
+
+
+def random_gauge(dims, seed, prec="f64"):
+    nd, V = len(dims), volume(dims)
+    U = np.zeros((nd, V, 3, 3), _cdt(prec))
+    getattr(lib(), "orc_su3_random_gauge_" + _sfx(U))(_real(U), _dims(dims), nd, seed)
+    return U
+
+
+def momenta(dims, seed, traj, prec="f64"):
+    nd, V = len(dims), volume(dims)
+    P = np.zeros((nd, V, 3, 3), _cdt(prec))
+    K = np.zeros(V)
+    getattr(lib(), "orc_su3_momenta_" + _sfx(P))(_real(P), K, _dims(dims), nd, seed, traj)
+    return P, K
+
+
+def plaq_sites(U, dims):
+    out = np.zeros(volume(dims))
+    getattr(lib(), "orc_su3_plaq_sites_" + _sfx(U))(_real(U), _dims(dims), len(dims), out)
+    return out
+
+
+def plaquette(U, dims):
+    return getattr(lib(), "orc_su3_plaquette_" + _sfx(U))(_real(U), _dims(dims), len(dims))
+
+
+def staple(U, dims, mu):
+    S = np.zeros((volume(dims), 3, 3), U.dtype)
+    getattr(lib(), "orc_su3_staple_" + _sfx(U))(_real(U), _dims(dims), len(dims), mu, _real(S))
+    return S
+
+
+def force(U, dims, mu, beta):
+    F = np.zeros((volume(dims), 3, 3), np.complex128)
+    lib().orc_su3_force_f64(_real(U), _dims(dims), len(dims), mu, beta, _real(F))
+    return F
+
+
+def force_update(U, P, dims, beta, delta):
+    P = P.copy()
+    getattr(lib(), "orc_su3_force_update_" + _sfx(U))(_real(U), _real(P), _dims(dims), len(dims), beta, delta)
+    return P
+
+
+def link_update(U, P, dims, t, reunit=False):
+    out = np.zeros_like(U)
+    getattr(lib(), "orc_su3_link_update_" + _sfx(U))(_real(U), _real(out), _real(P), _dims(dims), len(dims),
+                                                     t, int(reunit))
+    return out
+
+
+def reunit_field(U):
+    U = np.ascontiguousarray(U).copy()
+    getattr(lib(), "orc_su3_reunit_field_" + _sfx(U))(_real(U), U.size // 9)
+    return U
+
+
+def kinetic_sites(P, dims):
+    out = np.zeros(volume(dims))
+    getattr(lib(), "orc_su3_kinetic_sites_" + _sfx(P))(_real(P), _dims(dims), len(dims), out)
+    return out
+
+
+def delta_h(Uo, Un, P, Ko, dims, beta):
+    out = np.zeros(5)
+    getattr(lib(), "orc_su3_delta_h_" + _sfx(Uo))(_real(Uo), _real(Un), _real(P), np.ascontiguousarray(Ko),
+                                                  _dims(dims), len(dims), beta, out)
+    return out
+
+
+def hmc_trajectory(U, dims, beta, deltas, seed, traj, metropolis=True):
+    """In place on U. Returns dict(dH, accepted, plaq, H_old, H_new, u)."""
+    out = np.zeros(6)
+    d = np.ascontiguousarray(deltas, np.float64)
+    getattr(lib(), "orc_su3_hmc_trajectory_" + _sfx(U))(_real(U), _dims(dims), len(dims), beta, d, d.size,
+                                                        seed, traj, int(metropolis), out)
+    return dict(dH=out[0], accepted=bool(out[1]), plaq=out[2], H_old=out[3], H_new=out[4], u=out[5])
+
+
+def _hptr(h):
+    return None if h is None else _real(h).ctypes.data
+
+
+def landau_functional(U, dims, h=None):
+    return getattr(lib(), "orc_landau_functional_" + _sfx(U))(_real(U), _hptr(h), _dims(dims), len(dims))
+
+
+def landau_divsum(U, dims, h=None):
+    return getattr(lib(), "orc_landau_divsum_" + _sfx(U))(_real(U), _hptr(h), _dims(dims), len(dims))
+
+
+def landau_deriv(U, h, dims):
+    F = np.zeros((volume(dims), 3, 3), np.complex128)
+    lib().orc_landau_deriv_f64(_real(U), _real(h), _dims(dims), len(dims), _real(F))
+    return F
+
+
+def landau_or_halfsweep(U, dims, parity, omega, h=None):
+    """In place on U (and h if given)."""
+    assert U.flags.c_contiguous and (h is None or h.flags.c_contiguous)
+    getattr(lib(), "orc_landau_or_halfsweep_" + _sfx(U))(_real(U), _hptr(h), _dims(dims), len(dims),
+                                                         parity, omega)
+
+
+def random_transform(dims, seed):
+    h = np.zeros((1, volume(dims), 3, 3), np.complex128)
+    lib().orc_su3_random_transform_f64(_real(h), _dims(dims), len(dims), seed)
+    return h
+
+
+def gauge_transform(U, h, dims):
+    U = np.ascontiguousarray(U, np.complex128).copy()
+    lib().orc_su3_gauge_transform_f64(_real(U), _real(h), _dims(dims), len(dims))
+    return U
+
+
+TAG_HAAR = 0x48414152
+TAG_MOM = 0x4D4F4D30
+TAG_METRO = 0x4D455452
+TAG_GAUGE = 0x47415547

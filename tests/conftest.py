@@ -1,0 +1,27 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """Session-wide GPU context; GPU tests fail (not skip) if it cannot be made."""
+    import mesh_amd
+
+    c = mesh_amd.Context(0)
+    yield c
+    c.close()
+
+
+def pytest_runtest_setup(item):
+    if item.get_closest_marker("gpu"):
+        print(f"\n[gpu] {item.nodeid}", flush=True)

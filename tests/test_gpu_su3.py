@@ -1,0 +1,219 @@
+"""GPU parity of the SU(3) HMC kernels against the CPU oracle.
+
+FP64: the kernels and the oracle perform the same sequence of IEEE
+operations, so link updates, momenta, staples and forces are compared bit for
+bit; global sums (different summation order) to 1e-12 relative.  FP32: same
+bit-for-bit comparison against the oracle's float instantiation.
+"""
+import numpy as np
+import pytest
+
+import mesh_amd
+from mesh_amd import F32, F64, gauge
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(4, 4, 4, 4), (6, 4, 2, 8), (8, 8, 8), (3, 4, 5, 2), (6, 10)]
+
+
+def _field(ctx, shape, prec, host):
+    g = mesh_amd.Grid(shape)
+    f = mesh_amd.Field(g, host.shape[0], prec, ctx)
+    f.upload(host)
+    return g, f
+
+
+def _cdt(prec):
+    return np.complex128 if prec == F64 else np.complex64
+
+
+def _pname(prec):
+    return "f64" if prec == F64 else "f32"
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_random_gauge_bitexact(ctx, shape, prec):
+    g = mesh_amd.Grid(shape)
+    U = mesh_amd.GaugeField(g, prec, ctx)
+    gauge.randomGaugeField(U, 1234)
+    ref = O.random_gauge(list(shape), 1234, _pname(prec))
+    got = U.download()
+    assert got.dtype == ref.dtype
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_upload_download_roundtrip(ctx, prec):
+    shape = (3, 4, 5, 2)
+    U = O.random_gauge(list(shape), 5, _pname(prec))
+    g, f = _field(ctx, shape, prec, U)
+    np.testing.assert_array_equal(f.download(), U)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES[:3])
+def test_momenta_bitexact(ctx, shape, prec):
+    g = mesh_amd.Grid(shape)
+    P = mesh_amd.GaugeField(g, prec, ctx)
+    K = mesh_amd.DeviceArray(g.size(), ctx)
+    gauge.randomMomenta(P, 77, 5, K)
+    Pr, Kr = O.momenta(list(shape), 77, 5, _pname(prec))
+    np.testing.assert_array_equal(P.download(), Pr)
+    np.testing.assert_array_equal(K.download(), Kr)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_plaquette(ctx, shape, prec):
+    U = O.random_gauge(list(shape), 3, _pname(prec))
+    g, f = _field(ctx, shape, prec, U)
+    ref = O.plaquette(U, list(shape))
+    got = gauge.plaquette(f)
+    assert abs(got - ref) <= 1e-12 * max(abs(ref), 1e-3)
+    ff = mesh_amd.GaugeField(g, prec, ctx).fill_unit()
+    assert gauge.plaquette(ff) == pytest.approx(1.0, abs=1e-14)
+    beta = 6.0
+    nP = g.size() * len(shape) * (len(shape) - 1) / 2
+    assert gauge.wilsonAction(f, beta) == pytest.approx(beta * nP * (1 - ref), rel=1e-12)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_staple_bitexact(ctx, shape, prec):
+    U = O.random_gauge(list(shape), 11, _pname(prec))
+    g, f = _field(ctx, shape, prec, U)
+    for mu in range(len(shape)):
+        S = gauge.stapleSum(f, mu).download()[0]
+        np.testing.assert_array_equal(S, O.staple(U, list(shape), mu))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_force_bitexact_f64(ctx, shape):
+    U = O.random_gauge(list(shape), 12, "f64")
+    g, f = _field(ctx, shape, F64, U)
+    for mu in range(len(shape)):
+        F = gauge.wilsonDeriv(f, mu, 5.7).download()[0]
+        np.testing.assert_array_equal(F, O.force(U, list(shape), mu, 5.7))
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_force_update_bitexact(ctx, shape, prec):
+    U = O.random_gauge(list(shape), 21, _pname(prec))
+    P, _ = O.momenta(list(shape), 21, 0, _pname(prec))
+    g, fu = _field(ctx, shape, prec, U)
+    _, fp = _field(ctx, shape, prec, P)
+    gauge.momentumStep(fu, fp, 6.0, 0.0833)
+    np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("reunit", [False, True])
+@pytest.mark.parametrize("t", [0.04, 0.7, 3.0])
+def test_link_update_bitexact(ctx, shape, prec, reunit, t):
+    U = O.random_gauge(list(shape), 31, _pname(prec))
+    P, _ = O.momenta(list(shape), 31, 2, _pname(prec))
+    g, fu = _field(ctx, shape, prec, U)
+    _, fp = _field(ctx, shape, prec, P)
+    out = mesh_amd.GaugeField(g, prec, ctx)
+    gauge.linkStep(out, fu, fp, t, reunit)
+    np.testing.assert_array_equal(out.download(), O.link_update(U, P, list(shape), t, reunit))
+    # in place gives the same
+    gauge.linkStep(fu, fu, fp, t, reunit)
+    np.testing.assert_array_equal(fu.download(), out.download())
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_reunit_bitexact(ctx, prec):
+    shape = (4, 4, 2, 2)
+    U = O.random_gauge(list(shape), 41, _pname(prec))
+    rng = np.random.default_rng(0)
+    U = (U + 1e-3 * (rng.normal(size=U.shape) + 1j * rng.normal(size=U.shape))).astype(U.dtype)
+    g, f = _field(ctx, shape, prec, U)
+    gauge.reunitize(f)
+    np.testing.assert_array_equal(f.download(), O.reunit_field(U))
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES[:3])
+def test_delta_h(ctx, shape, prec):
+    sh = list(shape)
+    U0 = O.random_gauge(sh, 51, _pname(prec))
+    P, K = O.momenta(sh, 51, 1, _pname(prec))
+    U1 = O.link_update(U0, P, sh, 0.05, True)
+    P1 = O.force_update(U1, P, sh, 6.0, 0.1)
+    ref = O.delta_h(U0, U1, P1, K, sh, 6.0)
+    g, fu0 = _field(ctx, shape, prec, U0)
+    _, fu1 = _field(ctx, shape, prec, U1)
+    _, fp = _field(ctx, shape, prec, P1)
+    Kd = mesh_amd.DeviceArray(g.size(), ctx).upload(K)
+    got = gauge.deltaH(fu0, fu1, fp, Kd, 6.0)
+    scale = np.abs(K).sum() + 6.0 * g.size()
+    assert abs(got["dH"] - ref[0]) <= 1e-13 * scale
+    for i, k in enumerate(["ps_old", "ps_new", "K_old", "K_new"]):
+        assert got[k] == pytest.approx(ref[i + 1], rel=1e-12, abs=1e-10)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_hmc_trajectories_match_oracle(ctx, prec):
+    """C1-style run: 4^4 (8^4 in the bench), seeded Haar start, 2lf eps=0.5 n=10;
+    three thermalisation trajectories without Metropolis, then five with."""
+    shape = [4, 4, 4, 4]
+    g = mesh_amd.Grid(shape)
+    hmc = gauge.Hmc(g, prec, seed=1, ctx=ctx)
+    hmc.randomizeGaugeField()
+    U = O.random_gauge(shape, 1, _pname(prec))
+    deltas = gauge.makeDeltas("2lf", 0.5, 10)
+    np.testing.assert_array_equal(deltas, O.make_deltas("2lf", 0.5, 10))
+    for traj in range(8):
+        metro = traj >= 3
+        r = hmc.runHmcUpdate(6.0, deltas, metropolis=metro)
+        o = O.hmc_trajectory(U, shape, 6.0, deltas, 1, traj, metropolis=metro)
+        # identical fields, different summation order: |dH| agrees to ~1e-12
+        assert abs(r["dH"] - o["dH"]) <= 1e-12 * max(1.0, abs(o["dH"])), (traj, r, o)
+        assert r["accepted"] == o["accepted"]
+        assert abs(r["plaq"] - o["plaq"]) <= 1e-13
+        np.testing.assert_array_equal(hmc.U.download(), U)
+
+
+def test_f32_vs_f64_same_config(ctx):
+    """Mixed precision against FP64 on one seeded trajectory."""
+    shape = [4, 4, 4, 4]
+    g = mesh_amd.Grid(shape)
+    U64 = mesh_amd.GaugeField(g, F64, ctx)
+    gauge.randomGaugeField(U64, 9)
+    res = {}
+    for prec in (F64, F32):
+        h = gauge.Hmc(g, prec, seed=9, ctx=ctx)
+        h.U = U64.convert(prec)
+        r = h.runHmcUpdate(6.0, gauge.makeDeltas("2mn", 0.5, 12), metropolis=False)
+        res[prec] = (r, gauge.plaquette(h.U))
+    assert abs(res[F32][1] - res[F64][1]) < 1e-5
+    assert abs(res[F32][0]["dH"] - res[F64][0]["dH"]) < 1e-2 * (1 + abs(res[F64][0]["dH"]))
+
+
+def test_errors(ctx):
+    g = mesh_amd.Grid((4, 4, 4, 4))
+    U = mesh_amd.GaugeField(g, F64, ctx)
+    with pytest.raises(mesh_amd.MeshError):
+        gauge.stapleSum(U, 7)
+    with pytest.raises(mesh_amd.MeshError):
+        gauge.makeDeltas("3xx", 1.0, 4)
+    with pytest.raises(ValueError):
+        gauge.runHmd(U, U, 6.0, [0.1, 0.2])
+
+
+def test_nan_momenta_rejects(ctx):
+    """A NaN Delta H is rejected (u < exp(NaN) is false, src/gauge/hmc.cpp:93)
+    and the field is left bit-unchanged."""
+    shape = [4, 4, 4, 4]
+    g = mesh_amd.Grid(shape)
+    hmc = gauge.Hmc(g, F64, seed=3, ctx=ctx)
+    hmc.randomizeGaugeField()
+    before = hmc.U.download()
+    r = hmc.runHmcUpdate(6.0, np.array([np.nan, 0.1, np.nan]), metropolis=True)
+    assert not r["accepted"]
+    np.testing.assert_array_equal(hmc.U.download(), before)
