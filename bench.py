@@ -20,6 +20,7 @@ cpu_baseline: the CPU oracle (FP64, OpenMP) running one trajectory of the
 same workload on the host cores.
 """
 import argparse
+from ctypes import c_double as C_double
 import json
 import math
 import os
@@ -102,8 +103,48 @@ def cpu_baseline(L, seed, scheme, eps, substeps):
                        f"{substeps} substeps) from the Haar start, CPU oracle with OpenMP; {dt:.2f} s")
 
 
+def kernel_only(name, prec, L, reps):
+    """Profiling mode: `reps` launches of one kernel on a seeded L^4 field,
+    preceded by `reps` plain copies of one gauge field (a transfer of known
+    size, used to calibrate rocprof byte counters).  Prints one JSON line."""
+    import mesh_amd
+    from mesh_amd import gauge
+
+    ctx = mesh_amd.default_context()
+    g = mesh_amd.Grid((L,) * 4)
+    U = mesh_amd.GaugeField(g, prec, ctx)
+    U2 = mesh_amd.GaugeField(g, prec, ctx)
+    P = mesh_amd.GaugeField(g, prec, ctx)
+    K = mesh_amd.DeviceArray(g.size(), ctx)
+    gauge.randomGaugeField(U, 2)
+    gauge.randomMomenta(P, 2, 0, K)
+    for _ in range(reps):
+        ctx.call("mesh_stream_copy", U2.ptr, U.ptr, U.bytes)
+    ctx.synchronize()
+    ctx.profile(True)
+    for _ in range(reps):
+        if name == "force_update":
+            gauge.momentumStep(U, P, 6.0, 1e-6)
+        elif name == "link_update":
+            gauge.linkStep(U2, U, P, 1e-6)
+        elif name == "delta_h":
+            gauge.deltaH(U, U2, P, K, 6.0)
+        elif name == "landau_or":
+            ctx.call("mesh_landau_or_halfsweep", g.ref, U.prec, U.ptr, None, 0, C_double(1.7))
+        else:
+            raise SystemExit(f"unknown kernel {name}")
+    ctx.synchronize()
+    ms, n = ctx.profile_read(name)
+    print(json.dumps({"kernel": name, "prec": "f64" if prec == 0 else "f32", "L": L, "avg_ms": ms / n,
+                      "launches": n, "copy_bytes_per_launch": 2 * U.bytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--kernel-only", default=None,
+                    help="profiling mode: run one kernel (force_update, link_update, delta_h, landau_or)")
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--prec", default="f64")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -122,6 +163,10 @@ def main():
         sys.exit(2)
 
     from mesh_amd import F32, F64
+
+    if a.kernel_only:
+        kernel_only(a.kernel_only, F64 if a.prec == "f64" else F32, a.L, a.reps)
+        return
 
     r64 = run_gpu(F64, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
     V = r64["V"]

@@ -13,14 +13,17 @@
  * gauge or momentum field, 1 for a single-direction lattice such as a staple
  * sum, a force or a gauge transformation):
  *
- *     plane (f, k), f < nf, k < 9 (row-major matrix entry k = 3 i + j),
- *     holds the complex entry k of matrix f for all sites:
- *         elem(f, k, x) = base[(f * 9 + k) * stride + x]    (complex: re, im)
- *     stride = mesh_site_stride(grid) = V rounded up to a multiple of 64.
+ *     matrix f < nf, complex entry k < 9 (row-major, k = 3 i + j), site x:
+ *         elem(f, k, x) = base[f * 9 * S + (x / B) * 9 * B + k * B + x % B]
+ *     (complex: re, im), S = mesh_site_stride(grid) = V rounded up to a
+ *     multiple of 256, B = mesh_site_block() (a power of two <= 256).
  *
- * i.e. consecutive lanes of a wavefront read consecutive sites of one plane
- * (16 B per lane in FP64, 8 B in FP32).  Sites are row-major with dimension 0
- * slowest (src/lattice/grid.h:39-49).
+ * i.e. each direction holds blocks of B consecutive sites; a block stores the
+ * 9 complex entries as 9 planes of B sites, so consecutive lanes of a
+ * wavefront read consecutive sites (16 B per lane in FP64, 8 B in FP32) and
+ * one matrix of a 64-site wavefront is one contiguous 9 B * 16 B range.  Sites
+ * are row-major with dimension 0 slowest (src/lattice/grid.h:39-49).  Matrix
+ * f of every site is the nf = 1 field at base + f * 9 * S complex elements.
  *
  * Host layout (mesh_*_upload / download): the reference's AoS order,
  * host[f][x][3][3] complex interleaved (src/lattice/lattice.h:159-192: one
@@ -108,11 +111,18 @@ int mesh_memcpy_2d(mesh_ctx *ctx, void *dst, size_t dpitch, const void *src, siz
 int mesh_memclear(mesh_ctx *ctx, void *ptr, size_t bytes);    /* device_memclear device.cpp:70-75 */
 int mesh_memcpy_to_host(mesh_ctx *ctx, void *dst, const void *src, size_t bytes);   /* device.cpp:56-61 */
 int mesh_memcpy_from_host(mesh_ctx *ctx, void *dst, const void *src, size_t bytes); /* device.cpp:63-68 */
+/* Device-to-device copy by a plain 16 B-per-lane kernel (bytes % 16 == 0):
+ * the streaming-bandwidth reference of the roofline figures (the reference
+ * measures memory bandwidth with host loops, src/apps/ompbench.cpp:52-110). */
+int mesh_stream_copy(mesh_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- geometry and SU(3) field transfer ---------------------------------- */
 
 long long mesh_volume(const mesh_grid *g);
 long long mesh_site_stride(const mesh_grid *g);
+/* site-block length B of the device layout, fixed when the library is
+ * built; 0 means B = S, i.e. plain structure of arrays */
+int mesh_site_block(void);
 /* bytes of an SU(3) field with nf matrices per site in precision prec */
 size_t mesh_su3_field_bytes(const mesh_grid *g, int nf, int prec);
 /* host AoS (double for MESH_F64, float for MESH_F32) <-> device SoA;

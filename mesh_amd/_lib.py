@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmesh_hip.so")
+# MESH_HIP_LIB selects another build of the same library (e.g. one built with
+# a different site-block length); the default is the in-tree build.
+LIB_PATH = os.environ.get("MESH_HIP_LIB") or os.path.join(HERE, "libmesh_hip.so")
 
 MESH_OK, MESH_E_ARG, MESH_E_SIZE, MESH_E_ALLOC, MESH_E_HIP, MESH_E_STATE = 0, -1, -2, -3, -4, -5
 F64, F32 = 0, 1
@@ -47,8 +49,10 @@ _SIGS = {
     "mesh_memclear": (i32, [vp, vp, C.c_size_t]),
     "mesh_memcpy_to_host": (i32, [vp, vp, vp, C.c_size_t]),
     "mesh_memcpy_from_host": (i32, [vp, vp, vp, C.c_size_t]),
+    "mesh_stream_copy": (i32, [vp, vp, vp, C.c_size_t]),
     "mesh_volume": (C.c_longlong, [gp]),
     "mesh_site_stride": (C.c_longlong, [gp]),
+    "mesh_site_block": (i32, []),
     "mesh_su3_field_bytes": (C.c_size_t, [gp, i32, i32]),
     "mesh_su3_upload": (i32, [vp, gp, i32, i32, vp, vp]),
     "mesh_su3_download": (i32, [vp, gp, i32, i32, vp, vp]),

@@ -26,20 +26,29 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, extra=()):
+def build(verbose=False, extra=(), out=LIB, tag="default"):
+    """Compile every source and link ``out``; ``extra`` are additional hipcc
+    flags (e.g. -DMESH_SITE_LB=5 for another site-block length)."""
     objs = []
-    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    bdir = os.path.join(HERE, "_build", tag)
+    os.makedirs(bdir, exist_ok=True)
     for src in SOURCES:
-        obj = os.path.join(HERE, "_build", src.replace(".hip", ".o"))
+        obj = os.path.join(bdir, src.replace(".hip", ".o"))
         cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        r = subprocess.run(cmd, stderr=subprocess.PIPE, text=True)
+        # the host pass does not know the GPU-only target feature: drop that note
+        err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+        if err.strip():
+            print(err, file=sys.stderr)
+        if r.returncode != 0:
+            raise subprocess.CalledProcessError(r.returncode, cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

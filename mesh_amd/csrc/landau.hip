@@ -21,11 +21,11 @@ template <class R>
 __device__ __forceinline__ m3<double> ug(Geo const &g, const R *__restrict__ U, const R *__restrict__ h, int mu,
                                          long x, long xp)
 {
-	m3<double> u = to_f64(load(U, mu, g.stride, x));
+	m3<double> u = to_f64(load(U, mu, g, x));
 	if (h)
 	{
-		u = mul_nn(to_f64(load(h, 0, g.stride, x)), u);
-		u = mul_nd(u, to_f64(load(h, 0, g.stride, xp)));
+		u = mul_nn(to_f64(load(h, 0, g, x)), u);
+		u = mul_nd(u, to_f64(load(h, 0, g, xp)));
 	}
 	return u;
 }
@@ -88,8 +88,8 @@ __global__ __launch_bounds__(256) void k_deriv(Geo g, const R *__restrict__ U, c
 	{
 		int cm = coord(g, x, mu);
 		long xp = shift_c(g, x, mu, cm, 1), xm = shift_c(g, x, mu, cm, -1);
-		m3<double> a = mul_nd(to_f64(load(U, mu, g.stride, x)), to_f64(load(h, 0, g.stride, xp)));
-		m3<double> b = mul_nn(to_f64(load(h, 0, g.stride, xm)), to_f64(load(U, mu, g.stride, xm)));
+		m3<double> a = mul_nd(to_f64(load(U, mu, g, x)), to_f64(load(h, 0, g, xp)));
+		m3<double> b = mul_nn(to_f64(load(h, 0, g, xm)), to_f64(load(U, mu, g, xm)));
 		// d += a + b^+
 #pragma unroll
 		for (int i = 0; i < 3; ++i)
@@ -100,14 +100,14 @@ __global__ __launch_bounds__(256) void k_deriv(Geo g, const R *__restrict__ U, c
 				d.e[3 * i + j].im += a.e[3 * i + j].im + -b.e[3 * j + i].im;
 			}
 	}
-	m3<double> f = ta(mul_nn(to_f64(load(h, 0, g.stride, x)), d));
+	m3<double> f = ta(mul_nn(to_f64(load(h, 0, g, x)), d));
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		f.e[k].re *= c;
 		f.e[k].im *= c;
 	}
-	store(F, 0, g.stride, x, from_f64<R>(f));
+	store(F, 0, g, x, from_f64<R>(f));
 }
 
 // G in SU(3) raising Re tr(G K), built from three SU(2) subgroup steps with
@@ -197,8 +197,8 @@ __global__ __launch_bounds__(256) void k_or_half(Geo g, R *U, R *h, int parity, 
 	for (int mu = 0; mu < g.nd; ++mu)
 	{
 		xm[mu] = shift_c(g, x, mu, coord(g, x, mu), -1);
-		m3<R> a = load(U, mu, g.stride, x);
-		m3<R> b = load(U, mu, g.stride, xm[mu]);
+		m3<R> a = load(U, mu, g, x);
+		m3<R> b = load(U, mu, g, xm[mu]);
 #pragma unroll
 		for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -213,11 +213,11 @@ __global__ __launch_bounds__(256) void k_or_half(Geo g, R *U, R *h, int parity, 
 	m3<R> G = or_transform(K, omega);
 	for (int mu = 0; mu < g.nd; ++mu)
 	{
-		store(U, mu, g.stride, x, mul_nn(G, load(U, mu, g.stride, x)));
-		store(U, mu, g.stride, xm[mu], mul_nd(load(U, mu, g.stride, xm[mu]), G));
+		store(U, mu, g, x, mul_nn(G, load(U, mu, g, x)));
+		store(U, mu, g, xm[mu], mul_nd(load(U, mu, g, xm[mu]), G));
 	}
 	if (h)
-		store(h, 0, g.stride, x, mul_nn(G, load(h, 0, g.stride, x)));
+		store(h, 0, g, x, mul_nn(G, load(h, 0, g, x)));
 }
 
 template <class R>
@@ -230,8 +230,8 @@ __global__ __launch_bounds__(256) void k_gauge_transform(Geo g, R *__restrict__ 
 	int mu = (int)(i / g.V);
 	long x = i - (long)mu * g.V;
 	long xp = shift_c(g, x, mu, coord(g, x, mu), 1);
-	m3<R> u = mul_nn(load(h, 0, g.stride, x), load(U, mu, g.stride, x));
-	store(Uo, mu, g.stride, x, mul_nd(u, load(h, 0, g.stride, xp)));
+	m3<R> u = mul_nn(load(h, 0, g, x), load(U, mu, g, x));
+	store(Uo, mu, g, x, mul_nd(u, load(h, 0, g, xp)));
 }
 
 int setup(mesh_ctx *ctx, const mesh_grid *g, int prec, Geo *geo)

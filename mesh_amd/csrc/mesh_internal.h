@@ -37,6 +37,7 @@ struct mesh_ctx
 	double *red_dev = nullptr;
 	double *red_host = nullptr;
 	int xcd_remap = 1;
+	int force_variant = 0;
 	// profiler: events around launches, resolved after a stream sync
 	int prof_on = 0;
 	std::vector<hipEvent_t> prof_pool;

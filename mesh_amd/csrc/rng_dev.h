@@ -152,17 +152,20 @@ MD m3<double> haar_from_normals(double const *z)
 #pragma unroll
 	for (int i = 0; i < 3; ++i)
 	{
+		// projection done twice (re-orthogonalisation) for full accuracy
 #pragma unroll
-		for (int k = 0; k < i; ++k)
-		{
-			cpx<double> c = c0<double>();
+		for (int pass = 0; pass < 2; ++pass)
 #pragma unroll
-			for (int j = 0; j < 3; ++j)
-				cmadd_ca(c, a.e[3 * k + j], a.e[3 * i + j]);
+			for (int k = 0; k < i; ++k)
+			{
+				cpx<double> c = c0<double>();
 #pragma unroll
-			for (int j = 0; j < 3; ++j)
-				cmsub(a.e[3 * i + j], c, a.e[3 * k + j]);
-		}
+				for (int j = 0; j < 3; ++j)
+					cmadd_ca(c, a.e[3 * k + j], a.e[3 * i + j]);
+#pragma unroll
+				for (int j = 0; j < 3; ++j)
+					cmsub(a.e[3 * i + j], c, a.e[3 * k + j]);
+			}
 		double n2 = 0.0;
 #pragma unroll
 		for (int j = 0; j < 3; ++j)

@@ -5,6 +5,7 @@
 // explicit context (stream + reduction scratch + error string) instead of
 // global CUDA state and exceptions.
 #include "mesh_internal.h"
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -61,7 +62,7 @@ int mesh_make_geo(mesh_ctx *ctx, const mesh_grid *g, Geo *geo)
 		s *= g->dims[i];
 	}
 	geo->V = V;
-	geo->stride = (V + 63) / 64 * 64;
+	geo->stride = (V + 255) / 256 * 256;
 	return MESH_OK;
 }
 
@@ -282,6 +283,8 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value)
 		return mesh_fail(ctx, MESH_E_ARG, "option name is NULL");
 	if (!std::strcmp(name, "xcd_remap"))
 		ctx->xcd_remap = value != 0;
+	else if (!std::strcmp(name, "force_variant") && value >= 0 && value <= 1)
+		ctx->force_variant = (int)value;
 	else
 		return mesh_fail(ctx, MESH_E_ARG, std::string("unknown option ") + name);
 	return MESH_OK;
@@ -380,6 +383,8 @@ long long mesh_volume(const mesh_grid *g)
 	return geo.V;
 }
 
+int mesh_site_block(void) { return MESH_SITE_LB ? (1 << MESH_SITE_LB) : 0; }
+
 long long mesh_site_stride(const mesh_grid *g)
 {
 	Geo geo;
@@ -400,34 +405,55 @@ size_t mesh_su3_field_bytes(const mesh_grid *g, int nf, int prec)
 
 // ---- AoS <-> SoA --------------------------------------------------------------
 
-template <class R> __global__ void k_aos_to_soa(const R *aos, R *soa, long V, long stride, int nf)
+template <class R> __global__ void k_aos_to_soa(const R *aos, R *soa, Geo g, int nf)
 {
 	long i = (long)blockIdx.x * blockDim.x + threadIdx.x; // matrix index f*V + x
-	if (i >= V * nf)
+	if (i >= g.V * nf)
 		return;
-	int f = (int)(i / V);
-	long x = i - (long)f * V;
+	int f = (int)(i / g.V);
+	long x = i - (long)f * g.V;
 	m3<R> m;
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 		m.e[k] = {aos[i * 18 + 2 * k], aos[i * 18 + 2 * k + 1]};
-	store(soa, f, stride, x, m);
+	store(soa, f, g, x, m);
 }
 
-template <class R> __global__ void k_soa_to_aos(const R *soa, R *aos, long V, long stride, int nf)
+template <class R> __global__ void k_soa_to_aos(const R *soa, R *aos, Geo g, int nf)
 {
 	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= V * nf)
+	if (i >= g.V * nf)
 		return;
-	int f = (int)(i / V);
-	long x = i - (long)f * V;
-	m3<R> m = load(soa, f, stride, x);
+	int f = (int)(i / g.V);
+	long x = i - (long)f * g.V;
+	m3<R> m = load(soa, f, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		aos[i * 18 + 2 * k] = m.e[k].re;
 		aos[i * 18 + 2 * k + 1] = m.e[k].im;
 	}
+}
+
+// 16 B per lane streaming copy (bandwidth reference)
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long n)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		dst[i] = src[i];
+}
+
+extern "C" int mesh_stream_copy(mesh_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+	MESH_CHECK_CTX(ctx);
+	if (!dst || !src || bytes % 16)
+		return mesh_fail(ctx, MESH_E_ARG, "stream_copy needs non-NULL pointers and a multiple of 16 bytes");
+	long n = (long)(bytes / 16);
+	if (n == 0)
+		return MESH_OK;
+	k_stream_copy<<<(int)((n + 255) / 256), 256, 0, ctx->stream>>>((const uint4 *)src, (uint4 *)dst, n);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
 }
 
 template <class D, class S> __global__ void k_convert(const S *src, D *dst, long n)
@@ -437,13 +463,13 @@ template <class D, class S> __global__ void k_convert(const S *src, D *dst, long
 		dst[i] = (D)src[i];
 }
 
-template <class R> __global__ void k_unit(R *dst, long V, long stride, int nf)
+template <class R> __global__ void k_unit(R *dst, Geo g, int nf)
 {
 	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= V * nf)
+	if (i >= g.V * nf)
 		return;
-	int f = (int)(i / V);
-	store(dst, f, stride, i - (long)f * V, unit_m3<R>());
+	int f = (int)(i / g.V);
+	store(dst, f, g, i - (long)f * g.V, unit_m3<R>());
 }
 
 template <class R>
@@ -464,12 +490,12 @@ static int transfer(mesh_ctx *ctx, const mesh_grid *g, int nf, const void *host,
 	if (up)
 	{
 		MESH_HIP(ctx, hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, ctx->stream));
-		k_aos_to_soa<R><<<nb, 256, 0, ctx->stream>>>(tmp, (R *)dev, geo.V, geo.stride, nf);
+		k_aos_to_soa<R><<<nb, 256, 0, ctx->stream>>>(tmp, (R *)dev, geo, nf);
 		MESH_LAUNCHED(ctx);
 	}
 	else
 	{
-		k_soa_to_aos<R><<<nb, 256, 0, ctx->stream>>>((const R *)dev, tmp, geo.V, geo.stride, nf);
+		k_soa_to_aos<R><<<nb, 256, 0, ctx->stream>>>((const R *)dev, tmp, geo, nf);
 		MESH_LAUNCHED(ctx);
 		MESH_HIP(ctx, hipMemcpyAsync((void *)host, tmp, bytes, hipMemcpyDeviceToHost, ctx->stream));
 	}
@@ -536,9 +562,9 @@ int mesh_su3_unit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *dst
 	long n = (long)nf * geo.V;
 	int nb = (int)((n + 255) / 256);
 	if (prec == MESH_F64)
-		k_unit<double><<<nb, 256, 0, ctx->stream>>>((double *)dst, geo.V, geo.stride, nf);
+		k_unit<double><<<nb, 256, 0, ctx->stream>>>((double *)dst, geo, nf);
 	else
-		k_unit<float><<<nb, 256, 0, ctx->stream>>>((float *)dst, geo.V, geo.stride, nf);
+		k_unit<float><<<nb, 256, 0, ctx->stream>>>((float *)dst, geo, nf);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }

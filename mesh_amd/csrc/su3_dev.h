@@ -351,34 +351,6 @@ template <class R> MD m3<R> from_f64(m3<double> const &a)
 	return r;
 }
 
-// ---- SoA field access ------------------------------------------------------
-
-// field base pointer, matrix f of the site, plane stride in complex elements
-template <class R> MD m3<R> load(const R *__restrict__ base, int f, long stride, long x)
-{
-	const vec2_t<R> *p = reinterpret_cast<const vec2_t<R> *>(base) + (long)(f * 9) * stride + x;
-	m3<R> m;
-#pragma unroll
-	for (int k = 0; k < 9; ++k)
-	{
-		vec2_t<R> v = p[k * stride];
-		m.e[k] = {v.x, v.y};
-	}
-	return m;
-}
-template <class R> MD void store(R *__restrict__ base, int f, long stride, long x, m3<R> const &m)
-{
-	vec2_t<R> *p = reinterpret_cast<vec2_t<R> *>(base) + (long)(f * 9) * stride + x;
-#pragma unroll
-	for (int k = 0; k < 9; ++k)
-	{
-		vec2_t<R> v;
-		v.x = m.e[k].re;
-		v.y = m.e[k].im;
-		p[k * stride] = v;
-	}
-}
-
 // ---- geometry ---------------------------------------------------------------
 
 struct Geo
@@ -387,8 +359,55 @@ struct Geo
 	int L[4];
 	int st[4]; // site strides, st[nd-1] = 1
 	long V;
-	long stride; // plane stride (V rounded up)
+	long stride; // sites per direction block, V rounded up to a multiple of 256
 };
+
+// ---- field access -------------------------------------------------------------
+//
+// Matrix f of site x, complex entry k:
+//   base[f * 9 * stride + (x / B) * 9 * B + k * B + (x % B)]
+// i.e. each direction is split into blocks of B = 2^MESH_SITE_LB consecutive
+// sites and a block stores its 9 complex planes back to back
+// (MESH_SITE_LB = 0: plain SoA, one block of `stride` sites).
+#ifndef MESH_SITE_LB
+#define MESH_SITE_LB 6
+#endif
+
+MD long site_off(Geo const &g, int f, long x)
+{
+	long o = (long)(f * 9) * g.stride;
+	if (MESH_SITE_LB)
+		return o + (((x >> MESH_SITE_LB) * 9) << MESH_SITE_LB) + (x & ((1L << MESH_SITE_LB) - 1));
+	return o + x;
+}
+MD long plane_step(Geo const &g) { return MESH_SITE_LB ? (1L << MESH_SITE_LB) : g.stride; }
+
+template <class R> MD m3<R> load(const R *__restrict__ base, int f, Geo const &g, long x)
+{
+	const vec2_t<R> *p = reinterpret_cast<const vec2_t<R> *>(base) + site_off(g, f, x);
+	const long ks = plane_step(g);
+	m3<R> m;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		vec2_t<R> v = p[k * ks];
+		m.e[k] = {v.x, v.y};
+	}
+	return m;
+}
+template <class R> MD void store(R *__restrict__ base, int f, Geo const &g, long x, m3<R> const &m)
+{
+	vec2_t<R> *p = reinterpret_cast<vec2_t<R> *>(base) + site_off(g, f, x);
+	const long ks = plane_step(g);
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		vec2_t<R> v;
+		v.x = m.e[k].re;
+		v.y = m.e[k].im;
+		p[k * ks] = v;
+	}
+}
 
 MD int coord(Geo const &g, long x, int mu) { return (int)((x / g.st[mu]) % g.L[mu]); }
 MD long shift_c(Geo const &g, long x, int mu, int c, int s)

@@ -44,12 +44,12 @@ template <class R> __device__ __forceinline__ m3<R> staple_at(Geo const &g, cons
 		long xmn = shift_c(g, x, nu, c[nu], -1);
 		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
 		{
-			m3<R> T1 = mul_nd(load(U, nu, g.stride, xpm), load(U, mu, g.stride, xpn));
-			madd_nd(S, T1, load(U, nu, g.stride, x));
+			m3<R> T1 = mul_nd(load(U, nu, g, xpm), load(U, mu, g, xpn));
+			madd_nd(S, T1, load(U, nu, g, x));
 		}
 		{
-			m3<R> T2 = mul_dd(load(U, nu, g.stride, xpmmn), load(U, mu, g.stride, xmn));
-			madd_nn(S, T2, load(U, nu, g.stride, xmn));
+			m3<R> T2 = mul_dd(load(U, nu, g, xpmmn), load(U, mu, g, xmn));
+			madd_nn(S, T2, load(U, nu, g, xmn));
 		}
 	}
 	return S;
@@ -68,8 +68,8 @@ template <class R> __device__ __forceinline__ double plaq_site(Geo const &g, con
 			if (nu >= g.nd)
 				continue;
 			long xpm = shift_c(g, x, mu, c[mu], 1), xpn = shift_c(g, x, nu, c[nu], 1);
-			m3<double> A = mul_nn(to_f64(load(U, mu, g.stride, x)), to_f64(load(U, nu, g.stride, xpm)));
-			m3<double> B = mul_nn(to_f64(load(U, nu, g.stride, x)), to_f64(load(U, mu, g.stride, xpn)));
+			m3<double> A = mul_nn(to_f64(load(U, mu, g, x)), to_f64(load(U, nu, g, xpm)));
+			m3<double> B = mul_nn(to_f64(load(U, nu, g, x)), to_f64(load(U, mu, g, xpn)));
 			s += retr_nd(A, B);
 		}
 	return s;
@@ -79,7 +79,7 @@ template <class R> __device__ __forceinline__ double kin_site(Geo const &g, cons
 {
 	double k = 0.0;
 	for (int mu = 0; mu < g.nd; ++mu)
-		k += norm2_m3(to_f64(load(P, mu, g.stride, x)));
+		k += norm2_m3(to_f64(load(P, mu, g, x)));
 	return k;
 }
 
@@ -95,7 +95,7 @@ template <class R> __global__ __launch_bounds__(256) void k_random_gauge(Geo g, 
 	long x = i - (long)f * g.V;
 	double z[18];
 	link_normals<18>(seed, tag, 0u, (uint32_t)x, (uint32_t)f, z);
-	store(U, f, g.stride, x, from_f64<R>(haar_from_normals(z)));
+	store(U, f, g, x, from_f64<R>(haar_from_normals(z)));
 }
 
 template <class R> __global__ __launch_bounds__(256) void k_momenta(Geo g, R *P, double *K, uint64_t seed,
@@ -110,7 +110,7 @@ template <class R> __global__ __launch_bounds__(256) void k_momenta(Geo g, R *P,
 		double z[8];
 		link_normals<8>(seed, TAG_MOM, traj, (uint32_t)x, (uint32_t)mu, z);
 		m3<R> p = from_f64<R>(algebra_from_normals(z));
-		store(P, mu, g.stride, x, p);
+		store(P, mu, g, x, p);
 		k += norm2_m3(to_f64(p));
 	}
 	if (K)
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_staple(Geo g, const R *__restrict__ U, 
 		return;
 	int c[4];
 	coords<R>(g, x, c);
-	store(S, 0, g.stride, x, staple_at(g, U, mu, x, c));
+	store(S, 0, g, x, staple_at(g, U, mu, x, c));
 }
 
 template <class R>
@@ -182,14 +182,14 @@ __global__ __launch_bounds__(256) void k_force(Geo g, const R *__restrict__ U, i
 		return;
 	int c[4];
 	coords<R>(g, x, c);
-	m3<R> A = ta(mul_nn(load(U, mu, g.stride, x), staple_at(g, U, mu, x, c)));
+	m3<R> A = ta(mul_nn(load(U, mu, g, x), staple_at(g, U, mu, x, c)));
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		A.e[k].re *= coef;
 		A.e[k].im *= coef;
 	}
-	store(F, 0, g.stride, x, A);
+	store(F, 0, g, x, A);
 }
 
 // fused momentum step: workgroup = 64 sites x 4 directions, wavefront w
@@ -208,15 +208,43 @@ __global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict
 	int c[4];
 	coords<R>(g, x, c);
 	m3<R> S = staple_at(g, U, mu, x, c);
-	m3<R> A = ta(mul_nn(load(U, mu, g.stride, x), S));
-	m3<R> p = load(P, mu, g.stride, x);
+	m3<R> A = ta(mul_nn(load(U, mu, g, x), S));
+	m3<R> p = load(P, mu, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
-	store(P, mu, g.stride, x, p);
+	store(P, mu, g, x, p);
+}
+
+// variant: one thread per site looping over the directions
+template <class R>
+__global__ __launch_bounds__(256) void k_force_update_site(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
+                                                           int nb, int remap)
+{
+	int lb = logical_block(nb, remap);
+	if (lb >= nb)
+		return;
+	long x = (long)lb * 256 + threadIdx.x;
+	if (x >= g.V)
+		return;
+	int c[4];
+	coords<R>(g, x, c);
+	for (int mu = 0; mu < g.nd; ++mu)
+	{
+		m3<R> S = staple_at(g, U, mu, x, c);
+		m3<R> A = ta(mul_nn(load(U, mu, g, x), S));
+		m3<R> p = load(P, mu, g, x);
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+		{
+			p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
+			p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
+		}
+		store(P, mu, g, x, p);
+	}
 }
 
 // link step over nf*V matrices: Uout = exp(t TA(P)) Uin (+ reunitarise)
@@ -229,17 +257,17 @@ __global__ __launch_bounds__(256) void k_link_update(Geo g, int nf, R *Uout, con
 		return;
 	int f = (int)(i / g.V);
 	long x = i - (long)f * g.V;
-	m3<R> X = ta(load(P, f, g.stride, x));
+	m3<R> X = ta(load(P, f, g, x));
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		X.e[k].re *= t;
 		X.e[k].im *= t;
 	}
-	m3<R> u = mul_nn(expsu3(X), load(Uin, f, g.stride, x));
+	m3<R> u = mul_nn(expsu3(X), load(Uin, f, g, x));
 	if (reunit_flag)
 		u = reunit(u);
-	store(Uout, f, g.stride, x, u);
+	store(Uout, f, g, x, u);
 }
 
 template <class R> __global__ __launch_bounds__(256) void k_reunit(Geo g, int nf, R *U)
@@ -249,7 +277,7 @@ template <class R> __global__ __launch_bounds__(256) void k_reunit(Geo g, int nf
 		return;
 	int f = (int)(i / g.V);
 	long x = i - (long)f * g.V;
-	store(U, f, g.stride, x, reunit(load(U, f, g.stride, x)));
+	store(U, f, g, x, reunit(load(U, f, g, x)));
 }
 
 // sum over f,x of ||A||^2 (B == nullptr) or Re tr(A B), partials per block
@@ -264,12 +292,12 @@ __global__ __launch_bounds__(256) void k_trace_sum(Geo g, int nf, const R *__res
 		double s = 0.0;
 		for (int f = 0; f < nf; ++f)
 		{
-			m3<double> a = to_f64(load(A, f, g.stride, x));
+			m3<double> a = to_f64(load(A, f, g, x));
 			if (!B)
 				s += norm2_m3(a);
 			else
 			{
-				m3<double> p = mul_nn(a, to_f64(load(B, f, g.stride, x)));
+				m3<double> p = mul_nn(a, to_f64(load(B, f, g, x)));
 				s += (p.e[0].re + p.e[4].re) + p.e[8].re;
 			}
 		}
@@ -317,16 +345,30 @@ double n_plaquettes(Geo const &geo) { return (double)geo.V * geo.nd * (geo.nd - 
 
 int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, void *P, double beta, double delta)
 {
-	int nb = nblk(geo.V, SB);
-	int gb = mesh_grid_blocks(ctx, nb);
 	double k = delta * (beta / 6.0);
 	int ph = mesh_prof_begin(ctx, PROF_FORCE_UPDATE);
-	if (prec == MESH_F64)
-		k_force_update<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, (double *)P, k, nb,
-		                                                     ctx->xcd_remap);
+	if (ctx->force_variant == 1)
+	{
+		int nb = nblk(geo.V, 256);
+		int gb = mesh_grid_blocks(ctx, nb);
+		if (prec == MESH_F64)
+			k_force_update_site<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, (double *)P, k, nb,
+			                                                          ctx->xcd_remap);
+		else
+			k_force_update_site<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, (float)k,
+			                                                         nb, ctx->xcd_remap);
+	}
 	else
-		k_force_update<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, (float)k, nb,
-		                                                    ctx->xcd_remap);
+	{
+		int nb = nblk(geo.V, SB);
+		int gb = mesh_grid_blocks(ctx, nb);
+		if (prec == MESH_F64)
+			k_force_update<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, (double *)P, k, nb,
+			                                                     ctx->xcd_remap);
+		else
+			k_force_update<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, (float)k, nb,
+			                                                    ctx->xcd_remap);
+	}
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;

@@ -113,7 +113,8 @@ class Grid:
 
     @property
     def stride(self):
-        return (self.size() + 63) // 64 * 64
+        """Sites per direction block of a device field (mesh_site_stride)."""
+        return int(_lib.lib().mesh_site_stride(self.ref))
 
     def __eq__(self, other):
         return isinstance(other, Grid) and self.shape == other.shape

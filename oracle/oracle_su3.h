@@ -549,15 +549,17 @@ inline mat3<double> haar_from_normals(double const *z)
 			a.e[i][j] = {z[6 * i + 2 * j], z[6 * i + 2 * j + 1]};
 	for (int i = 0; i < 3; ++i)
 	{
-		for (int k = 0; k < i; ++k)
-		{
-			// c = <row_k, row_i> = sum_j conj(a_kj) a_ij
-			cplx<double> c = {0.0, 0.0};
-			for (int j = 0; j < 3; ++j)
-				cmadd_ca(c, a.e[k][j], a.e[i][j]);
-			for (int j = 0; j < 3; ++j)
-				cmsub(a.e[i][j], c, a.e[k][j]);
-		}
+		// projection done twice (re-orthogonalisation) for full accuracy
+		for (int pass = 0; pass < 2; ++pass)
+			for (int k = 0; k < i; ++k)
+			{
+				// c = <row_k, row_i> = sum_j conj(a_kj) a_ij
+				cplx<double> c = {0.0, 0.0};
+				for (int j = 0; j < 3; ++j)
+					cmadd_ca(c, a.e[k][j], a.e[i][j]);
+				for (int j = 0; j < 3; ++j)
+					cmsub(a.e[i][j], c, a.e[k][j]);
+			}
 		double n2 = 0.0;
 		for (int j = 0; j < 3; ++j)
 		{
