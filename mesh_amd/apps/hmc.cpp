@@ -1,0 +1,157 @@
+// mesh_hmc: pure-gauge HMC driver on one GPU, the counterpart of the
+// reference's `hmc` app (src/apps/hmc.cpp:9-156): same options and defaults,
+// same summary; the trajectory runs in libmesh_hip.so.
+//
+// Options (reference defaults): --group su3 --geom 8 8 8 8 --beta 6
+//   --start random|cold --seed S --count 100 --scheme 4mn|2mn|2lf
+//   --epsilon 1 --substeps 8 --precision 2 (1 = FP32-mixed, 2 = FP64)
+// Additions: --therm K (first K trajectories without Metropolis),
+//   --landau (Landau gauge fixing of the final configuration).
+#include "mesh/gauge.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+using namespace mesh;
+
+namespace {
+
+[[noreturn]] void usage(char const *msg)
+{
+	std::fprintf(stderr,
+	             "error: %s\nusage: mesh_hmc [--group su3] [--geom L0 L1 [L2 [L3]]] [--beta B] [--start random|cold]\n"
+	             "                [--seed S] [--count N] [--scheme 2lf|2mn|4mn] [--epsilon E] [--substeps N]\n"
+	             "                [--precision 1|2] [--therm K] [--landau]\n",
+	             msg);
+	std::exit(2);
+}
+
+// seed string -> 64-bit key (FNV-1a; the reference hashes with sha3,
+// src/apps/hmc.cpp:93)
+uint64_t hash_seed(std::string const &s)
+{
+	uint64_t h = 1469598103934665603ull;
+	for (unsigned char c : s)
+	{
+		h ^= c;
+		h *= 1099511628211ull;
+	}
+	return h;
+}
+
+bool is_number(std::string const &s)
+{
+	return !s.empty() && s.find_first_not_of("0123456789") == std::string::npos;
+}
+
+} // namespace
+
+int main(int argc, char **argv)
+{
+	std::string group = "su3", scheme = "4mn", start = "random", seed = "";
+	std::vector<int32_t> geom = {8, 8, 8, 8};
+	double beta = 6.0, epsilon = 1.0;
+	int substeps = 8, count = 100, precision = 2, therm = 0;
+	bool landau = false;
+
+	for (int i = 1; i < argc; ++i)
+	{
+		std::string a = argv[i];
+		auto next = [&]() -> std::string {
+			if (i + 1 >= argc)
+				usage(("missing value for " + a).c_str());
+			return argv[++i];
+		};
+		if (a == "--group")
+			group = next();
+		else if (a == "--geom")
+		{
+			geom.clear();
+			while (i + 1 < argc && is_number(argv[i + 1]))
+				geom.push_back(std::atoi(argv[++i]));
+		}
+		else if (a == "--beta")
+			beta = std::atof(next().c_str());
+		else if (a == "--start")
+			start = next();
+		else if (a == "--seed")
+			seed = next();
+		else if (a == "--count")
+			count = std::atoi(next().c_str());
+		else if (a == "--scheme")
+			scheme = next();
+		else if (a == "--epsilon")
+			epsilon = std::atof(next().c_str());
+		else if (a == "--substeps")
+			substeps = std::atoi(next().c_str());
+		else if (a == "--precision")
+			precision = std::atoi(next().c_str());
+		else if (a == "--therm")
+			therm = std::atoi(next().c_str());
+		else if (a == "--landau")
+			landau = true;
+		else if (a == "--help" || a == "-h")
+			usage("help requested");
+		else
+			usage(("unknown option " + a).c_str());
+	}
+	if (seed.empty())
+		seed = std::to_string(std::random_device{}());
+
+	try
+	{
+		auto deltas = makeDeltas(scheme, epsilon, substeps);
+		auto t_total = std::chrono::steady_clock::now();
+		mesh_profile_enable(ctx().get(), 1);
+		dispatchByGroup(
+		    [&]<typename G>() {
+			    Grid g(geom);
+			    Hmc<G> hmc(g);
+			    hmc.seed = is_number(seed) ? std::strtoull(seed.c_str(), nullptr, 10) : hash_seed(seed);
+			    if (start == "random")
+				    hmc.randomizeGaugeField();
+			    else if (start == "cold")
+				    hmc.U.fill_unit();
+			    else
+				    throw std::runtime_error("unknown start '" + start + "' (random or cold)");
+			    for (int iter = 0; iter < count; ++iter)
+			    {
+				    hmc.runHmcUpdate(beta, deltas, iter >= therm);
+				    if (iter == therm - 1)
+					    hmc.reset_observables();
+			    }
+			    hmc.print_summary();
+			    if (landau)
+			    {
+				    Landau<G> lan(hmc.U);
+				    lan.verbose = true;
+				    int it = lan.run(100000);
+				    std::printf("landau: %d sweeps, theta = %.3e, functional = %.12f\n", it,
+				                lan.condition() * g.ndim(), lan());
+			    }
+		    },
+		    group, precision);
+		ctx().synchronize();
+		double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_total).count();
+		std::printf("========== performance summary ==========\n");
+		for (char const *k : {"force_update", "link_update", "delta_h", "momenta", "landau_or"})
+		{
+			double ms;
+			long long n;
+			ctx().check(mesh_profile_read(ctx().get(), k, &ms, &n));
+			std::printf("time in %-13s %8.3f s (%5.1f %%, %lld launches)\n", k, ms * 1e-3, 100 * ms * 1e-3 / total,
+			            n);
+		}
+		std::printf("total:                %8.3f s\n", total);
+	}
+	catch (std::exception const &e)
+	{
+		std::fprintf(stderr, "error: %s\n", e.what());
+		return 1;
+	}
+	return 0;
+}

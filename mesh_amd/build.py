@@ -48,7 +48,27 @@ def build(verbose=False, extra=(), out=LIB, tag="default"):
     tmp = out + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
     os.replace(tmp, out)
+    if out == LIB:
+        build_apps(verbose)
     return out
+
+
+APPS = {"mesh_hmc": "hmc.cpp", "test_lattice_gauge": "test_lattice_gauge.cpp"}
+
+
+def build_apps(verbose=False):
+    """Host programs over the C ABI (g++, no HIP headers): the C++ call
+    surface of mesh/gauge.h exercised by a driver and a test program."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    root = os.path.dirname(HERE)
+    for exe, src in APPS.items():
+        cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", f"-I{root}/include", f"-I{HERE}/host",
+               os.path.join(HERE, "apps", src), f"-L{HERE}", "-lmesh_hip", "-Wl,-rpath,$ORIGIN/..",
+               "-Wl,-rpath-link,/opt/rocm/lib", "-o", os.path.join(bindir, exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

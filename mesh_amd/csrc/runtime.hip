@@ -283,7 +283,7 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value)
 		return mesh_fail(ctx, MESH_E_ARG, "option name is NULL");
 	if (!std::strcmp(name, "xcd_remap"))
 		ctx->xcd_remap = value != 0;
-	else if (!std::strcmp(name, "force_variant") && value >= 0 && value <= 1)
+	else if (!std::strcmp(name, "force_variant") && value >= 0 && value <= 2)
 		ctx->force_variant = (int)value;
 	else
 		return mesh_fail(ctx, MESH_E_ARG, std::string("unknown option ") + name);

@@ -13,6 +13,7 @@
 #include "rng_dev.h"
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 using namespace mesh_dev;
@@ -219,6 +220,99 @@ __global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict
 	store(P, mu, g, x, p);
 }
 
+// Fused momentum step with an LDS tile (4-d lattices whose fastest extent
+// L3 divides 64 with R = 64 / L3 dividing L2).  A workgroup owns the 64
+// sites of R consecutive x2-rows (one site block of the layout) and stages
+// all four link directions of those rows plus one halo row on each side in
+// LDS; every staple link displaced only along x2/x3 is then read from LDS and
+// only the x0/x1-displaced ones from global memory.  Wavefront w handles
+// direction w; the arithmetic is exactly that of k_force_update.
+template <class R, int L3>
+__global__ __launch_bounds__(256) void k_force_update_tile(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
+                                                           int nb, int remap)
+{
+	constexpr int RW = 64 / L3; // rows per tile
+	constexpr int NR = RW + 2;  // with halo rows
+	__shared__ vec2_t<R> lds[NR][4][9][L3];
+	int lb = logical_block(nb, remap);
+	if (lb >= nb)
+		return;
+	const long base = (long)lb * 64; // first site of the tile (x3 = 0 of its first row)
+	const int x2_0 = (int)((base / L3) % g.L[2]);
+	const long row0 = base - (long)x2_0 * L3; // site index of (x0, x1, 0, 0)
+	// stage rows x2_0 - 1 ... x2_0 + RW (periodic in x2)
+	for (int e = threadIdx.x; e < NR * 4 * 9 * L3; e += 256)
+	{
+		int c = e % L3;
+		int k = (e / L3) % 9;
+		int d = (e / (9 * L3)) % 4;
+		int lr = e / (36 * L3);
+		int x2 = x2_0 - 1 + lr;
+		x2 = x2 < 0 ? x2 + g.L[2] : (x2 >= g.L[2] ? x2 - g.L[2] : x2);
+		long xs = row0 + (long)x2 * L3 + c;
+		lds[lr][d][k][c] = reinterpret_cast<const vec2_t<R> *>(U)[site_off(g, d, xs) + k * plane_step(g)];
+	}
+	__syncthreads();
+
+	const int mu = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int r = lane / L3, c3 = lane % L3;
+	const long x = base + lane;
+	int c[4];
+	coords<R>(g, x, c);
+	// matrix of direction d at the site displaced by (dx2, dx3) from x, from LDS
+	auto from_lds = [&](int d, int dx2, int dx3) {
+		int cc = c3 + dx3;
+		cc = cc < 0 ? cc + L3 : (cc >= L3 ? cc - L3 : cc);
+		m3<R> m;
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+		{
+			vec2_t<R> v = lds[r + 1 + dx2][d][k][cc];
+			m.e[k] = {v.x, v.y};
+		}
+		return m;
+	};
+	// displacement of +/-1 along direction a as (dx2, dx3); a < 2 is non-local
+	auto d2 = [](int a, int s) { return a == 2 ? s : 0; };
+	auto d3 = [](int a, int s) { return a == 3 ? s : 0; };
+
+	m3<R> S = zero_m3<R>();
+	const bool mloc = mu >= 2;
+	long xpm = shift_c(g, x, mu, c[mu], 1);
+#pragma unroll
+	for (int nu = 0; nu < 4; ++nu)
+	{
+		if (nu == mu)
+			continue;
+		const bool nloc = nu >= 2;
+		long xpn = shift_c(g, x, nu, c[nu], 1);
+		long xmn = shift_c(g, x, nu, c[nu], -1);
+		{
+			m3<R> a = mloc ? from_lds(nu, d2(mu, 1), d3(mu, 1)) : load(U, nu, g, xpm);
+			m3<R> b = nloc ? from_lds(mu, d2(nu, 1), d3(nu, 1)) : load(U, mu, g, xpn);
+			m3<R> T1 = mul_nd(a, b);
+			madd_nd(S, T1, from_lds(nu, 0, 0));
+		}
+		{
+			m3<R> a = (mloc && nloc) ? from_lds(nu, d2(mu, 1) + d2(nu, -1), d3(mu, 1) + d3(nu, -1))
+			                         : load(U, nu, g, shift_c(g, xpm, nu, c[nu], -1));
+			m3<R> b = nloc ? from_lds(mu, d2(nu, -1), d3(nu, -1)) : load(U, mu, g, xmn);
+			m3<R> T2 = mul_dd(a, b);
+			madd_nn(S, T2, nloc ? from_lds(nu, d2(nu, -1), d3(nu, -1)) : load(U, nu, g, xmn));
+		}
+	}
+	m3<R> A = ta(mul_nn(from_lds(mu, 0, 0), S));
+	m3<R> p = load(P, mu, g, x);
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
+		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
+	}
+	store(P, mu, g, x, p);
+}
+
 // variant: one thread per site looping over the directions
 template <class R>
 __global__ __launch_bounds__(256) void k_force_update_site(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
@@ -347,7 +441,26 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 {
 	double k = delta * (beta / 6.0);
 	int ph = mesh_prof_begin(ctx, PROF_FORCE_UPDATE);
-	if (ctx->force_variant == 1)
+	const int L3 = geo.L[3], RW = L3 > 0 ? 64 / L3 : 0;
+	const bool tile_ok = geo.nd == 4 && (L3 == 16 || L3 == 32 || L3 == 64) && geo.L[2] % RW == 0;
+	if (ctx->force_variant == 2 && tile_ok)
+	{
+		int nb = (int)(geo.V / 64);
+		int gb = mesh_grid_blocks(ctx, nb);
+		auto launch = [&](auto tag, auto l3) {
+			using Rt = decltype(tag);
+			k_force_update_tile<Rt, decltype(l3)::value><<<gb, 256, 0, ctx->stream>>>(
+			    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, ctx->xcd_remap);
+		};
+		using I16 = std::integral_constant<int, 16>;
+		using I32 = std::integral_constant<int, 32>;
+		using I64 = std::integral_constant<int, 64>;
+		if (prec == MESH_F64)
+			L3 == 16 ? launch(0.0, I16{}) : L3 == 32 ? launch(0.0, I32{}) : launch(0.0, I64{});
+		else
+			L3 == 16 ? launch(0.0f, I16{}) : L3 == 32 ? launch(0.0f, I32{}) : launch(0.0f, I64{});
+	}
+	else if (ctx->force_variant == 1)
 	{
 		int nb = nblk(geo.V, 256);
 		int gb = mesh_grid_blocks(ctx, nb);

@@ -1,0 +1,287 @@
+// C++ host layer: the gauge call surface of krox/mesh (src/gauge/utils.h,
+// hmc.h, hmc.cpp, landau.h) executed by libmesh_hip.so on the GPU.
+// Same names and argument meaning as the reference; each function cites the
+// reference function it stands for.
+#pragma once
+
+#include "mesh/lattice.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <string_view>
+
+namespace mesh {
+
+inline Context &ctx() { return Context::global(); }
+
+// ---- observables ------------------------------------------------------------
+
+// normalised to [0,1] (src/gauge/utils.h:93-108)
+template <class G> double plaquette(GaugeField<G> const &U)
+{
+	double r;
+	ctx().check(mesh_su3_plaquette(ctx().get(), U.grid().c(), G::prec, U.data(), &r));
+	return r;
+}
+
+// (1 - plaquette) * beta * Nd(Nd-1)/2 * V (src/gauge/utils.h:141-146)
+template <class G> double wilsonAction(GaugeField<G> const &U, double beta)
+{
+	double r;
+	ctx().check(mesh_su3_wilson_action(ctx().get(), U.grid().c(), G::prec, U.data(), beta, &r));
+	return r;
+}
+
+// sum ||U||^2 (src/gauge/utils.h:28, 35-41)
+template <class G> double norm2(Field<G> const &P)
+{
+	double r;
+	ctx().check(mesh_su3_norm2(ctx().get(), P.grid().c(), P.nfields(), G::prec, P.data(), &r));
+	return r;
+}
+
+// sum_real_trace(A * B) (src/gauge/utils.h:29)
+template <class G> double sum_real_trace_prod(Field<G> const &A, Field<G> const &B)
+{
+	double r;
+	ctx().check(mesh_su3_sum_real_trace_prod(ctx().get(), A.grid().c(), A.nfields(), G::prec, A.data(), B.data(),
+	                                         &r));
+	return r;
+}
+
+// staple sum of direction mu (src/gauge/utils.h:110-139)
+template <class G> Lattice<G> stapleSum(GaugeField<G> const &U, int mu)
+{
+	Lattice<G> S(U.grid());
+	ctx().check(mesh_su3_staple(ctx().get(), U.grid().c(), G::prec, U.data(), mu, S.data()));
+	return S;
+}
+
+// (beta / 2 Nc) TA(U_mu stapleSum(U, mu)) (src/gauge/utils.h:148-153)
+template <class G> Lattice<G> wilsonDeriv(GaugeField<G> const &U, int mu, double beta)
+{
+	Lattice<G> F(U.grid());
+	ctx().check(mesh_su3_force(ctx().get(), U.grid().c(), G::prec, U.data(), mu, beta, F.data()));
+	return F;
+}
+
+// project_on_group_fast on every link (src/gauge/utils.h:80-91)
+template <class G> void reunitize(Field<G> &U)
+{
+	ctx().check(mesh_su3_reunit(ctx().get(), U.grid().c(), U.nfields(), G::prec, U.data()));
+}
+
+// ---- random fields (Philox-keyed; the reference draws from a host xoshiro256)
+
+template <class G> void random_gauge_field(Field<G> &U, uint64_t seed, uint32_t stream = 0)
+{
+	ctx().check(mesh_su3_random_gauge(ctx().get(), U.grid().c(), U.nfields(), G::prec, U.data(), seed, stream));
+}
+
+template <class G> void random_algebra_field(GaugeField<G> &P, uint64_t seed, uint32_t traj)
+{
+	ctx().check(mesh_su3_momenta(ctx().get(), P.grid().c(), G::prec, P.data(), nullptr, seed, traj));
+}
+
+// ---- molecular dynamics -------------------------------------------------------
+
+// Integrator step table (src/gauge/hmc.cpp:7-51)
+inline std::vector<double> makeDeltas(std::string_view scheme, double epsilon, int substeps)
+{
+	std::vector<double> d(9 * (size_t)std::max(substeps, 1) + 1);
+	std::string s(scheme);
+	int n = mesh_make_deltas(s.c_str(), epsilon, substeps, d.data(), (int)d.size());
+	if (n < 0)
+		throw std::runtime_error("unknown scheme '" + s + "'");
+	d.resize(n);
+	return d;
+}
+
+// U = exp(TA(P) t) U (src/gauge/hmc.h:26-36), per direction or whole stack
+template <class G> void evolve(Field<G> &U, Field<G> const &P, double t)
+{
+	if (U.nfields() != P.nfields())
+		throw std::runtime_error("evolve: U and P differ in the number of directions");
+	ctx().check(mesh_su3_exp_mul(ctx().get(), U.grid().c(), U.nfields(), G::prec, U.data(), P.data(), t));
+}
+
+// Hamiltonian dynamics over deltas (src/gauge/hmc.h:38-59); even index: all
+// directions' link step, odd index: fused momentum step
+template <class G>
+void runHmd(GaugeField<G> &U, GaugeField<G> &P, double beta, std::vector<double> const &deltas)
+{
+	if (deltas.size() % 2 != 1)
+		throw std::runtime_error("runHmd: deltas must have odd length");
+	for (size_t i = 0; i < deltas.size(); ++i)
+		if (i % 2 == 0)
+			ctx().check(mesh_su3_link_update(ctx().get(), U.grid().c(), G::prec, U.data(), U.data(), P.data(),
+			                                 deltas[i], 0));
+		else
+			ctx().check(mesh_su3_force_update(ctx().get(), U.grid().c(), G::prec, U.data(), P.data(), beta,
+			                                  deltas[i]));
+}
+
+// HMC state and observables (src/gauge/hmc.h:61-115)
+template <class G> class Hmc
+{
+  public:
+	Grid const &g;
+	GaugeField<G> U, P, U_new;
+	uint64_t seed = 0; // keys the Philox streams (the reference seeds xoshiro256)
+	uint32_t traj = 0;
+	std::vector<double> plaq_history, deltaH_history, accept_history, time_history;
+
+  private:
+	void *K_ = nullptr; // per-site kinetic energy, V doubles
+
+  public:
+	explicit Hmc(Grid const &g_) : g(g_), U(g_), P(g_), U_new(g_)
+	{
+		ctx().check(mesh_malloc(ctx().get(), &K_, g_.size() * sizeof(double)));
+	}
+	~Hmc()
+	{
+		if (K_)
+			mesh_free(ctx().get(), K_);
+	}
+	Hmc(Hmc const &) = delete;
+	Hmc &operator=(Hmc const &) = delete;
+
+	void reset_observables()
+	{
+		plaq_history.clear();
+		deltaH_history.clear();
+		accept_history.clear();
+		time_history.clear();
+	}
+
+	void randomizeGaugeField() { random_gauge_field(U, seed); }
+	void randomizeMomenta()
+	{
+		ctx().check(mesh_su3_momenta(ctx().get(), g.c(), G::prec, P.data(), (double *)K_, seed, traj));
+	}
+
+	// momenta -> trajectory -> accept/reject -> measure (src/gauge/hmc.cpp:75-107);
+	// metropolis == false accepts unconditionally (thermalisation)
+	void runHmcUpdate(double beta, std::vector<double> const &deltas, bool metropolis = true)
+	{
+		auto t0 = std::chrono::steady_clock::now();
+		void *pu = U.data(), *pw = U_new.data();
+		double out[6];
+		ctx().check(mesh_su3_hmc_trajectory(ctx().get(), g.c(), G::prec, &pu, &pw, P.data(), (double *)K_, beta,
+		                                    deltas.data(), (int)deltas.size(), seed, traj, metropolis ? 1 : 0,
+		                                    out));
+		if (pu != U.data())
+			swap(U, U_new);
+		++traj;
+		double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		accept_history.push_back(out[1]);
+		plaq_history.push_back(out[2]);
+		deltaH_history.push_back(out[0]);
+		time_history.push_back(dt);
+	}
+
+	// same quantities as Hmc::print_summary (src/gauge/hmc.h:88-100)
+	void print_summary() const
+	{
+		auto mean = [](std::vector<double> const &v) {
+			return v.empty() ? 0.0 : std::accumulate(v.begin(), v.end(), 0.0) / v.size();
+		};
+		double mp = mean(plaq_history), var = 0;
+		for (double p : plaq_history)
+			var += (p - mp) * (p - mp);
+		size_t n = plaq_history.size();
+		var = n > 1 ? var / (n - 1) : 0;
+		double emdh = 0;
+		for (double d : deltaH_history)
+			emdh += std::exp(-d);
+		std::printf("========== HMC summary ==========\n");
+		std::printf("time per step = %.3f s (min = %.3f, max = %.3f)\n", mean(time_history),
+		            n ? *std::min_element(time_history.begin(), time_history.end()) : 0.0,
+		            n ? *std::max_element(time_history.begin(), time_history.end()) : 0.0);
+		std::printf("plaquette = %.4f +- %.4f\n", mp, n ? std::sqrt(var / n) : 0.0);
+		std::printf("acceptance = %.2f\n", mean(accept_history));
+		std::printf("<exp(-dH)> = %.4f\n", n ? emdh / n : 0.0);
+	}
+};
+
+// ---- Landau gauge (src/gauge/landau.h) ----------------------------------------
+
+template <class G> struct Landau
+{
+	GaugeField<G> const &U;
+	Lattice<G> g;
+	bool verbose = false;
+
+	void reset() { g.fill_unit(); }
+
+	explicit Landau(GaugeField<G> const &U_) : U(U_), g(U_.grid()) { reset(); }
+
+	// F[h] = sum Re tr h U h(x+mu)^+ / (Nc Nd V) (landau.h:23-34)
+	double operator()(Lattice<G> const &h) const
+	{
+		double r;
+		ctx().check(mesh_landau_functional(ctx().get(), U.grid().c(), G::prec, U.data(), h.data(), &r));
+		return r;
+	}
+	double operator()() const { return (*this)(g); }
+
+	// (1/(Nd V)) sum ||TA(g d)||^2 (landau.h:37-49)
+	double condition() const
+	{
+		double r;
+		ctx().check(mesh_landau_divsum(ctx().get(), U.grid().c(), G::prec, U.data(), g.data(), &r));
+		return r / (U.grid().ndim() * (double)U.grid().size());
+	}
+
+	// -0.5/(Nc Nd V) TA(g d) (landau.h:52-65)
+	Lattice<G> deriv() const
+	{
+		Lattice<G> F(U.grid());
+		ctx().check(mesh_landau_deriv(ctx().get(), U.grid().c(), G::prec, U.data(), g.data(), F.data()));
+		return F;
+	}
+
+	// Even/odd overrelaxation until theta = Nd * condition() < tol
+	// (replaces the line-search ascent of landau.h:100-136); returns sweeps.
+	int run(int max_iter, double omega = 1.7, double tol = 1e-12, int check_every = 10)
+	{
+		Context &c = ctx();
+		GaugeField<G> Ug(U.grid());
+		c.check(mesh_su3_gauge_transform(c.get(), U.grid().c(), G::prec, Ug.data(), U.data(), g.data()));
+		int it = 0;
+		while (it < max_iter)
+		{
+			for (int k = 0; k < check_every && it < max_iter; ++k, ++it)
+				for (int parity = 0; parity < 2; ++parity)
+					c.check(mesh_landau_or_halfsweep(c.get(), U.grid().c(), G::prec, Ug.data(), g.data(), parity,
+					                                 omega));
+			double d;
+			c.check(mesh_landau_divsum(c.get(), U.grid().c(), G::prec, Ug.data(), nullptr, &d));
+			double theta = d / (double)U.grid().size();
+			if (verbose)
+				std::printf("iter = %d, theta = %.3e\n", it, theta);
+			if (theta < tol)
+				break;
+		}
+		return it;
+	}
+};
+
+// ---- runtime dispatch (src/gauge/utils.h:155-186) ------------------------------
+
+template <typename F> void dispatchByGroup(F f, std::string const &group, int precision)
+{
+	if (group != "su3")
+		throw std::runtime_error("unknown gauge group '" + group + "'");
+	if (precision == 1)
+		f.template operator()<SU3<float>>();
+	else if (precision == 2)
+		f.template operator()<SU3<double>>();
+	else
+		throw std::runtime_error("invlid precision level '" + std::to_string(precision) + "'");
+}
+
+} // namespace mesh

@@ -82,3 +82,12 @@ def test_context_creation_without_gpu_fails_cleanly():
     h = C.c_void_p()
     assert _lib.lib().mesh_ctx_create(C.byref(h), 0) != 0
     assert not h.value
+
+
+def test_cpp_programs_built_and_reject_bad_options():
+    import subprocess
+
+    exe = os.path.join(ROOT, "mesh_amd", "bin", "mesh_hmc")
+    assert os.path.exists(exe) and os.path.exists(os.path.join(ROOT, "mesh_amd", "bin", "test_lattice_gauge"))
+    r = subprocess.run([exe, "--bogus"], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 2 and "unknown option --bogus" in r.stderr

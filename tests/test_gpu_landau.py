@@ -1,0 +1,131 @@
+"""GPU parity of the Landau gauge kernels against the CPU oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mesh_amd
+from mesh_amd import F32, F64, gauge
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(4, 4, 4, 4), (6, 4, 2, 8), (8, 8, 8), (6, 10)]
+
+
+def _pname(prec):
+    return "f64" if prec == F64 else "f32"
+
+
+def _up(ctx, shape, prec, host):
+    g = mesh_amd.Grid(shape)
+    f = mesh_amd.Field(g, host.shape[0], prec, ctx)
+    f.upload(host)
+    return g, f
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_functional_and_divsum(ctx, shape, prec):
+    sh = list(shape)
+    U = O.random_gauge(sh, 21, _pname(prec))
+    h = O.random_transform(sh, 22)
+    hp = h.astype(U.dtype)
+    g, fu = _up(ctx, shape, prec, U)
+    _, fh = _up(ctx, shape, prec, hp)
+    lan = gauge.Landau(fu)
+    assert lan() == pytest.approx(O.landau_functional(U, sh), rel=1e-12, abs=1e-14)  # g = 1
+    lan.g.upload(hp)
+    assert lan() == pytest.approx(O.landau_functional(U, sh, hp), rel=1e-12, abs=1e-14)
+    assert lan.divsum() == pytest.approx(O.landau_divsum(U, sh, hp), rel=1e-12)
+    assert lan.condition() == pytest.approx(O.landau_divsum(U, sh, hp) / (len(sh) * g.size()), rel=1e-12)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_deriv_f64(ctx, shape):
+    sh = list(shape)
+    U = O.random_gauge(sh, 31)
+    h = O.random_transform(sh, 32)
+    g, fu = _up(ctx, shape, F64, U)
+    lan = gauge.Landau(fu)
+    lan.g.upload(h)
+    np.testing.assert_allclose(lan.deriv().download()[0], O.landau_deriv(U, h, sh), rtol=0, atol=1e-17)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("track", [False, True])
+def test_or_halfsweep_bitexact(ctx, shape, prec, track):
+    sh = list(shape)
+    U = O.random_gauge(sh, 41, _pname(prec))
+    h = np.zeros((1,) + U.shape[1:], U.dtype)
+    h[:, :] = np.eye(3)
+    g, fu = _up(ctx, shape, prec, U)
+    _, fh = _up(ctx, shape, prec, h)
+    for sweep in range(3):
+        for parity in (0, 1):
+            ctx.call("mesh_landau_or_halfsweep", g.ref, prec, fu.ptr, fh.ptr if track else None, parity,
+                     C.c_double(1.7))
+            O.landau_or_halfsweep(U, sh, parity, 1.7, h if track else None)
+    np.testing.assert_array_equal(fu.download(), U)
+    if track:
+        np.testing.assert_array_equal(fh.download(), h)
+
+
+def test_gauge_transform_bitexact(ctx):
+    sh = [4, 6, 2, 4]
+    U = O.random_gauge(sh, 51)
+    h = O.random_transform(sh, 52)
+    g, fu = _up(ctx, sh, F64, U)
+    _, fh = _up(ctx, sh, F64, h)
+    out = mesh_amd.GaugeField(g, F64, ctx)
+    ctx.call("mesh_su3_gauge_transform", g.ref, F64, out.ptr, fu.ptr, fh.ptr)
+    np.testing.assert_array_equal(out.download(), O.gauge_transform(U, h, sh))
+    assert gauge.plaquette(out) == pytest.approx(gauge.plaquette(fu), abs=1e-14)
+
+
+def test_or_odd_extent_rejected(ctx):
+    g = mesh_amd.Grid((4, 3, 4, 4))
+    U = mesh_amd.GaugeField(g, F64, ctx)
+    with pytest.raises(mesh_amd.MeshError):
+        ctx.call("mesh_landau_or_halfsweep", g.ref, F64, U.ptr, None, 0, C.c_double(1.7))
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_run_pure_gauge_converges(ctx, prec):
+    """U = h(x) h(x+mu)^+ is a gauge copy of the unit field: the solver must
+    reach F = 1 and theta ~ 0, and U^g must equal the transformed field."""
+    sh = [8, 8, 8, 8]
+    g = mesh_amd.Grid(sh)
+    one = mesh_amd.GaugeField(g, prec, ctx).fill_unit()
+    h = mesh_amd.Lattice(g, prec, ctx)
+    gauge.randomGaugeField(h, 5, stream=O.TAG_GAUGE)
+    U = mesh_amd.GaugeField(g, prec, ctx)
+    ctx.call("mesh_su3_gauge_transform", g.ref, prec, U.ptr, one.ptr, h.ptr)
+    lan = gauge.Landau(U)
+    tol = 1e-24 if prec == F64 else 1e-10
+    it, theta = lan.run(2000, omega=1.7, tol=tol)
+    assert theta < tol, (it, theta)
+    assert lan() > 1 - (1e-12 if prec == F64 else 1e-6)
+
+
+def test_landau_finite_difference_gpu(ctx):
+    """The reference test's derivative check (src/tests/lattice_gauge.cpp:31-40)
+    through the GPU entry points: functional, deriv, exp_mul."""
+    sh = [4, 4, 4, 4]
+    g = mesh_amd.Grid(sh)
+    U = mesh_amd.GaugeField(g, F64, ctx)
+    gauge.randomGaugeField(U, 61)
+    lan = gauge.Landau(U)
+    gauge.randomGaugeField(lan.g, 62, stream=O.TAG_GAUGE)
+    f = mesh_amd.Lattice(g, F64, ctx)
+    P = mesh_amd.GaugeField(g, F64, ctx)
+    gauge.randomMomenta(P, 63, 0)
+    f.copy_from(P[0])
+    D = lan.deriv()
+    eps = 1e-6
+    expected = -2 * eps * gauge.sum_real_trace_prod(f, D)
+    old = lan()
+    gauge.evolve(lan.g, f, eps)
+    new = lan()
+    assert abs((new - old) / expected - 1) < 1e-4

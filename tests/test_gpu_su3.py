@@ -109,6 +109,27 @@ def test_force_update_bitexact(ctx, shape, prec):
     np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", [(4, 4, 2, 32), (2, 3, 4, 16), (2, 2, 2, 64), (3, 2, 6, 32), (4, 4, 4, 4)])
+@pytest.mark.parametrize("remap", [0, 1])
+def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
+    """Every force-kernel variant (wavefront per direction, thread per site,
+    LDS tile; tile-eligible shapes included) gives the oracle's bits."""
+    U = O.random_gauge(list(shape), 22, _pname(prec))
+    P, _ = O.momenta(list(shape), 22, 0, _pname(prec))
+    g, fu = _field(ctx, shape, prec, U)
+    _, fp = _field(ctx, shape, prec, P)
+    ctx.set_option("force_variant", variant)
+    ctx.set_option("xcd_remap", remap)
+    try:
+        gauge.momentumStep(fu, fp, 6.0, 0.0833)
+    finally:
+        ctx.set_option("force_variant", 0)
+        ctx.set_option("xcd_remap", 1)
+    np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
+
+
 @pytest.mark.parametrize("prec", [F64, F32])
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("reunit", [False, True])
