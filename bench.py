@@ -11,10 +11,13 @@ The headline value is the FP64 path; the FP32-mixed path (FP32 links and
 momenta, FP64 sums and Metropolis) runs the same workload and is reported
 beside it.  All inputs are resident in HBM when the timed region starts.
 
-Roofline: the dominant kernel is the fused force update (staples + TA +
-momentum step).  Its algorithmic traffic is 216 reals per site (read U and P,
-write P: 1728 B FP64, 864 B FP32); its average duration comes from HIP events
+Roofline: the dominant kernel is the fused molecular-dynamics step
+"force_link" (staples + TA + momentum step, then exp(t P) U into the second
+field).  Its algorithmic traffic is 288 reals per site (read U and P, write P
+and U': 2304 B FP64, 1152 B FP32); its average duration comes from HIP events
 recorded around every launch on the library's stream during the timed region.
+Per-site algorithmic bytes of every kernel are in kernel_bytes(); the
+trajectory's total is the sum over the launches actually made.
 
 cpu_baseline: the CPU oracle (FP64, OpenMP) running one trajectory of the
 same workload on the host cores.
@@ -33,8 +36,27 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def bytes_per_site_update(R, n_force, n_link):
-    return R * (216 * (n_force + n_link) + 288) + 16
+# algorithmic bytes per site of each timed kernel, R = bytes per real:
+#   force_update: read U, P; write P            216 R
+#   force_link:   read U, P; write P, U'        288 R
+#   link_update:  read U, P; write U'           216 R
+#   delta_h:      read U, U', P, K(8 B)         216 R + 8
+#   momenta:      write P, K                     72 R + 8
+def kernel_bytes(name, R):
+    return {"force_update": 216 * R, "force_link": 288 * R, "link_update": 216 * R,
+            "delta_h": 216 * R + 8, "momenta": 72 * R + 8}[name]
+
+
+KERNELS = ("force_link", "force_update", "link_update", "delta_h", "momenta")
+
+
+def trajectory_bytes_per_site(kern, R, steps):
+    """algorithmic bytes per site of one trajectory, from the launches made"""
+    return sum(kernel_bytes(k, R) * kern[k]["launches"] for k in KERNELS) / steps
+
+
+def dominant(kern):
+    return max(KERNELS, key=lambda k: (kern[k]["avg_ms"] or 0.0) * kern[k]["launches"])
 
 
 def run_gpu(prec, L, steps, warmup, therm, seed, scheme, eps, substeps):
@@ -63,13 +85,12 @@ def run_gpu(prec, L, steps, warmup, therm, seed, scheme, eps, substeps):
     dt = time.perf_counter() - t0
     ctx.profile(False)
     kern = {}
-    for k in ("force_update", "link_update", "delta_h", "momenta"):
+    for k in KERNELS:
         ms, n = ctx.profile_read(k)
         kern[k] = dict(avg_ms=ms / n if n else None, launches=n)
     s = hmc.summary()
     return dict(hmc=hmc, seconds=dt, V=g.size(), kernels=kern, acceptance=s["acceptance"],
-                plaquette=s["plaquette"], exp_mdH=s["exp_mdH"], dH=list(hmc.deltaH_history),
-                n_force=(len(deltas) - 1) // 2, n_link=(len(deltas) + 1) // 2)
+                plaquette=s["plaquette"], exp_mdH=s["exp_mdH"], dH=list(hmc.deltaH_history))
 
 
 def same_config_compare(U64, seed, scheme, eps, substeps):
@@ -122,7 +143,14 @@ def kernel_only(name, prec, L, reps):
         ctx.call("mesh_stream_copy", U2.ptr, U.ptr, U.bytes)
     ctx.synchronize()
     ctx.profile(True)
-    for _ in range(reps):
+    if name == "force_link":
+        # the fused step only runs inside trajectories: time `reps` of them
+        h = gauge.Hmc(g, prec, seed=2, ctx=ctx)
+        h.U = U
+        d = gauge.makeDeltas("2mn", 0.5, 12)
+        for _ in range(reps):
+            h.runHmcUpdate(6.0, d, metropolis=False)
+    for _ in range(reps if name != "force_link" else 0):
         if name == "force_update":
             gauge.momentumStep(U, P, 6.0, 1e-6)
         elif name == "link_update":
@@ -142,7 +170,8 @@ def kernel_only(name, prec, L, reps):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--kernel-only", default=None,
-                    help="profiling mode: run one kernel (force_update, link_update, delta_h, landau_or)")
+                    help="profiling mode: run one kernel (force_link, force_update, link_update, delta_h, "
+                         "landau_or)")
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--prec", default="f64")
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,9 +200,10 @@ def main():
     r64 = run_gpu(F64, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
     V = r64["V"]
     value = V * a.steps / r64["seconds"]
-    fk = r64["kernels"]["force_update"]
+    dom = dominant(r64["kernels"])
+    fk = r64["kernels"][dom]
     R = 8
-    alg = 216 * R * V
+    alg = kernel_bytes(dom, R) * V
     achieved = alg / (fk["avg_ms"] * 1e-3) / 1e9
     line = {
         "metric": "SU(3) HMC site-updates/s at 32^4 (FP64)",
@@ -193,9 +223,9 @@ def main():
                    "L": a.L, "beta": 6.0, "thermalisation": a.therm},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_force_update<double>", "avg_ms": fk["avg_ms"],
+                     "kernel": dom, "avg_ms": fk["avg_ms"],
                      "algorithmic_bytes_per_launch": alg},
-        "trajectory_roofline_frac": value * bytes_per_site_update(8, r64["n_force"], r64["n_link"]) / 8e12,
+        "trajectory_roofline_frac": value * trajectory_bytes_per_site(r64["kernels"], 8, a.steps) / 8e12,
         "acceptance": r64["acceptance"],
         "plaquette": r64["plaquette"],
         "exp_minus_dH": r64["exp_mdH"],
@@ -204,14 +234,15 @@ def main():
     if not a.no_f32:
         r32 = run_gpu(F32, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
         v32 = V * a.steps / r32["seconds"]
-        fk32 = r32["kernels"]["force_update"]
-        ach32 = 216 * 4 * V / (fk32["avg_ms"] * 1e-3) / 1e9
+        dom32 = dominant(r32["kernels"])
+        fk32 = r32["kernels"][dom32]
+        ach32 = kernel_bytes(dom32, 4) * V / (fk32["avg_ms"] * 1e-3) / 1e9
         cmp = same_config_compare(r64["hmc"].U, a.seed, a.scheme, a.epsilon, a.substeps)
         line["f32_mixed"] = {
             "value": v32, "ms_per_step": 1e3 * r32["seconds"] / a.steps, "ratio_vs_f64": v32 / value,
             "acceptance": r32["acceptance"], "plaquette": r32["plaquette"],
-            "force_update_frac_of_hbm": ach32 / HBM_PEAK_GBS,
-            "trajectory_roofline_frac": v32 * bytes_per_site_update(4, r32["n_force"], r32["n_link"]) / 8e12,
+            "dominant_kernel": dom32, "dominant_frac_of_hbm": ach32 / HBM_PEAK_GBS,
+            "trajectory_roofline_frac": v32 * trajectory_bytes_per_site(r32["kernels"], 4, a.steps) / 8e12,
             "same_config_vs_f64": cmp, "kernels": r32["kernels"],
         }
     if not a.no_cpu:

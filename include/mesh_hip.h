@@ -88,15 +88,24 @@ const char *mesh_last_error(mesh_ctx *ctx);
 int mesh_synchronize(mesh_ctx *ctx);
 /* The context's hipStream_t, for callers that order their own work. */
 void *mesh_stream(mesh_ctx *ctx);
-/* Kernel launch tuning: xcd_remap != 0 maps consecutive workgroups of the
- * stencil kernels onto one XCD's contiguous site range (default 1). */
+/* Kernel launch tuning (no effect on results):
+ *   "xcd_remap" = workgroup order of the stencil kernels: 0 linear, 1 each
+ *       XCD takes a contiguous site range, 2 (default) each XCD takes a range
+ *       of x2/x3 row-blocks and walks x0 fastest, then x1 (4-d lattices);
+ *   "force_variant" = 0 (default) wavefront per direction, 1 thread per
+ *       site, 2 LDS-staged x2/x3 tile;
+ *   "fuse_md" = 1 (default): in mesh_su3_hmc_trajectory each momentum step
+ *       and the link step after it run as one kernel (the thread that
+ *       updates P_mu(x) writes exp(t P_mu(x)) U_mu(x) into a second field);
+ *       same arithmetic, so the same bits. */
 int mesh_set_option(mesh_ctx *ctx, const char *name, long value);
 /* Per-kernel timing (replaces the util::Stopwatch globals swStaples, swExp,
  * swPlaquette ... of src/gauge/utils.h:23, src/gauge/hmc.h:16, printed at
  * src/apps/hmc.cpp:138-155): when enabled, launches of the HMC and Landau
  * kernels are bracketed by HIP events on the context stream and resolved at
  * the next synchronisation.  Kernel names: "force_update", "link_update",
- * "delta_h", "momenta", "plaquette", "landau_or". */
+ * "delta_h", "momenta", "plaquette", "landau_or", "force_link" (fused
+ * momentum + link step). */
 int mesh_profile_enable(mesh_ctx *ctx, int on);
 int mesh_profile_reset(mesh_ctx *ctx);
 int mesh_profile_read(mesh_ctx *ctx, const char *kernel, double *total_ms, long long *count);
@@ -236,6 +245,59 @@ int mesh_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, int prec, void *
 /* U_mu(x) <- h(x) U_mu(x) h(x+mu)^+ (out of place; Uout != U). */
 int mesh_su3_gauge_transform(mesh_ctx *ctx, const mesh_grid *g, int prec, void *Uout, const void *U,
                              const void *h);
+
+/* ---- U(1) on 2-d lattices (src/gauge/groups.h:29-108, hmc.cpp:109-110) -----
+ *
+ * A batch of `batch` independent configurations ("chains") of an L0 x L1
+ * periodic lattice (grid nd = 2), FP32 angles theta[c][mu][x] in [-pi, pi),
+ * x = x0 * L1 + x1.  theta_P(x) = theta_0(x) + theta_1(x+0) - theta_0(x+1)
+ * - theta_1(x); S = beta sum_x (1 - cos theta_P); the topological charge is
+ * Q = sum_x wrap(theta_P(x)) / 2 pi.  Device buffers: theta 2 V floats per
+ * configuration; per-configuration outputs are device doubles. */
+
+/* uniform (Haar) random angles: random_gauge_field for U1 (utils.h:43-52) */
+int mesh_u1_random(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, uint64_t seed);
+/* out_dev[4c + {0,1,2,3}] = {sum_x (1 - cos theta_P), Q, sum_x cos theta_P, sum_x wrap(theta_P)} */
+int mesh_u1_observables(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *theta, double *out_dev);
+/* One HMC trajectory per configuration (runHmcUpdate, hmc.cpp:75-107): momenta
+ * q ~ N(0, 1/2) per link, K = sum q^2, link step theta += t q, momentum step
+ * q -= t (beta/2) Im(U S), accept on u < exp(-Delta H) (metropolis != 0).
+ * deltas: host step table (mesh_make_deltas), odd length; V <= 4096.
+ * out_dev[4c + {0,1,2,3}] = {Delta H, accepted, u, S_new / beta}. */
+int mesh_u1_hmc(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, const double *deltas, int ndeltas,
+                double beta, uint64_t seed, uint32_t traj, int metropolis, double *out_dev);
+
+/* ---- U(1) learned proposal (no counterpart in the reference) ---------------
+ *
+ * Gauge-equivariant coupling-layer flow: 8 layers, each moving one direction's
+ * links on every fourth row/column by h(theta_P) - theta_P, with h a mixture of
+ * non-compact projections whose parameters an MLP (12 -> 64 -> 64 -> 10, tanh)
+ * computes from the frozen plaquettes around the active one.  Lattice extents
+ * divisible by 4, V <= 4096.  params: mesh_flow_num_params() floats (device). */
+int mesh_flow_num_params(void);
+/* deterministic initial weights (identity transform) */
+int mesh_flow_init(mesh_ctx *ctx, float *params, uint64_t seed);
+/* batch samples U = f(z), z uniform from Philox(seed, step):
+ * out_dev[2c + {0,1}] = {log q(U), S(U)/beta}; theta_layers may be NULL */
+int mesh_flow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, uint64_t seed, uint32_t step,
+                     float *theta_out, float *theta_layers, double *out_dev);
+/* log q of given configurations by the inverse flow (z_out may be NULL):
+ * out_dev[2c + {0,1}] = {log q(U), S(U)/beta} */
+int mesh_flow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, const float *theta, float *z_out,
+                   double *out_dev);
+/* bytes of the workspace mesh_flow_gradient needs */
+size_t mesh_flow_workspace_bytes(const mesh_grid *g, int batch);
+/* reverse-KL gradient: grad = (1/batch) sum_c d/dparams [beta S(f(z_c)) - log det f'(z_c)]
+ * over a fresh batch (Philox(seed, step)); out_dev as mesh_flow_sample */
+int mesh_flow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, uint64_t seed, uint32_t step,
+                       double beta, void *workspace, float *grad, double *out_dev);
+/* Adam step t >= 1 (moments m, v zero-initialised by the caller) */
+int mesh_flow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, float *v, int t, double lr);
+/* independence Metropolis-Hastings: per chain accept the proposal with
+ * probability min(1, exp[(-beta S' - log q') - (-beta S - log q)]); state and
+ * prop hold {log q, S/beta} per chain; acc_dev[c] = 1 on acceptance. */
+int mesh_flow_mh(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, double *state, const float *prop_theta,
+                 const double *prop, double beta, uint64_t seed, uint32_t step, double *acc_dev);
 
 #ifdef __cplusplus
 }

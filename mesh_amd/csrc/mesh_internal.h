@@ -15,6 +15,7 @@ enum mesh_prof_kind
 	PROF_MOMENTA,
 	PROF_PLAQ,
 	PROF_LANDAU_OR,
+	PROF_FORCE_LINK,
 	PROF_NKINDS
 };
 
@@ -36,8 +37,13 @@ struct mesh_ctx
 	size_t partials_cap = 0; // doubles
 	double *red_dev = nullptr;
 	double *red_host = nullptr;
-	int xcd_remap = 1;
+	double *aux = nullptr; // small device buffer (integrator step tables)
+	size_t aux_cap = 0;    // doubles
+	int xcd_remap = 2;     // block order of the stencil kernels (ordered_block)
 	int force_variant = 0;
+	int fuse_md = 1;        // fuse each momentum step with the next link step
+	void *md_spare = nullptr; // third gauge field for the fused trajectory
+	size_t md_spare_bytes = 0;
 	// profiler: events around launches, resolved after a stream sync
 	int prof_on = 0;
 	std::vector<hipEvent_t> prof_pool;
@@ -87,6 +93,8 @@ int mesh_make_geo(mesh_ctx *ctx, const mesh_grid *g, mesh_dev::Geo *geo);
 int mesh_check_prec(mesh_ctx *ctx, int prec);
 // ensures ctx->partials holds at least n doubles
 int mesh_reserve_partials(mesh_ctx *ctx, size_t n);
+// copies n host doubles into ctx->aux (grown as needed), on the ctx stream
+int mesh_stage_doubles(mesh_ctx *ctx, const double *host, size_t n);
 // sums nq quantities of nb partials each into host out[nq] (synchronises)
 int mesh_finish_reduction(mesh_ctx *ctx, int nb, int nq, double *out);
 
@@ -107,6 +115,33 @@ __device__ __forceinline__ int logical_block(int nb, int remap)
 		return b;
 	int chunk = (nb + 7) >> 3;
 	return (b & 7) * chunk + (b >> 3);
+}
+
+// Block order of the stencil kernels (sbk consecutive sites per workgroup):
+//   0: linear;  1: XCD-contiguous linear;
+//   2: cache order for 4-d lattices with sbk | L2*L3 -- each XCD walks its own
+//      range of x2/x3 row-blocks and, within it, x0 fastest then x1, so the
+//      x0-neighbours of a workgroup were staged by the previous one and the
+//      x1-neighbours L0 workgroups earlier, all in the same L2.
+// Returns the memory-order block index (-1: padding workgroup).  The order
+// only permutes workgroups; every site's arithmetic is unchanged.
+__device__ __forceinline__ int ordered_block(Geo const &g, int nb, int sbk, int order)
+{
+	int b = blockIdx.x;
+	if (order == 0)
+		return b < nb ? b : -1;
+	int chunk = (nb + 7) >> 3;
+	int o = (b & 7) * chunk + (b >> 3);
+	if (o >= nb)
+		return -1;
+	if (order == 1 || g.nd != 4 || (g.L[2] * g.L[3]) % sbk != 0)
+		return o;
+	int nb2 = (g.L[2] * g.L[3]) / sbk;
+	int x0 = o % g.L[0];
+	int t = o / g.L[0];
+	int x1 = t % g.L[1];
+	int rb = t / g.L[1];
+	return (x0 * g.L[1] + x1) * nb2 + rb;
 }
 
 // deterministic block sum of NQ doubles; thread 0 writes partials[q*nb + lb]

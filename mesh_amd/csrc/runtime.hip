@@ -83,6 +83,27 @@ int mesh_reserve_partials(mesh_ctx *ctx, size_t n)
 	return MESH_OK;
 }
 
+int mesh_stage_doubles(mesh_ctx *ctx, const double *host, size_t n)
+{
+	if (n > ctx->aux_cap)
+	{
+		if (ctx->aux)
+		{
+			MESH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+			MESH_HIP(ctx, hipFree(ctx->aux));
+			ctx->aux = nullptr;
+			ctx->aux_cap = 0;
+		}
+		if (hipMalloc(&ctx->aux, n * sizeof(double)) != hipSuccess)
+			return mesh_fail(ctx, MESH_E_ALLOC, "cannot allocate the step-table buffer");
+		ctx->aux_cap = n;
+	}
+	// the previous user of aux may still be in flight on the stream: the copy
+	// is ordered after it on the same stream
+	MESH_HIP(ctx, hipMemcpyAsync(ctx->aux, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+	return MESH_OK;
+}
+
 // second reduction pass: one workgroup, fixed summation order
 template <int NQ> __global__ __launch_bounds__(1024) void k_reduce_partials(const double *p, int nb, double *out)
 {
@@ -98,8 +119,8 @@ template <int NQ> __global__ __launch_bounds__(1024) void k_reduce_partials(cons
 	block_sum_store<NQ>(v, out, 1, 0);
 }
 
-static const char *const PROF_NAMES[PROF_NKINDS] = {"force_update", "link_update", "delta_h",
-                                                     "momenta",      "plaquette",   "landau_or"};
+static const char *const PROF_NAMES[PROF_NKINDS] = {"force_update", "link_update", "delta_h",   "momenta",
+                                                     "plaquette",    "landau_or",   "force_link"};
 
 int mesh_prof_begin(mesh_ctx *ctx, int kind)
 {
@@ -216,6 +237,10 @@ int mesh_ctx_destroy(mesh_ctx *ctx)
 		(void)hipEventDestroy(e);
 	if (ctx->partials)
 		(void)hipFree(ctx->partials);
+	if (ctx->aux)
+		(void)hipFree(ctx->aux);
+	if (ctx->md_spare)
+		(void)hipFree(ctx->md_spare);
 	(void)hipFree(ctx->red_dev);
 	(void)hipHostFree(ctx->red_host);
 	(void)hipStreamDestroy(ctx->stream);
@@ -281,10 +306,12 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value)
 	MESH_CHECK_CTX(ctx);
 	if (!name)
 		return mesh_fail(ctx, MESH_E_ARG, "option name is NULL");
-	if (!std::strcmp(name, "xcd_remap"))
-		ctx->xcd_remap = value != 0;
+	if (!std::strcmp(name, "xcd_remap") && value >= 0 && value <= 2)
+		ctx->xcd_remap = (int)value;
 	else if (!std::strcmp(name, "force_variant") && value >= 0 && value <= 2)
 		ctx->force_variant = (int)value;
+	else if (!std::strcmp(name, "fuse_md"))
+		ctx->fuse_md = value != 0;
 	else
 		return mesh_fail(ctx, MESH_E_ARG, std::string("unknown option ") + name);
 	return MESH_OK;

@@ -121,8 +121,8 @@ template <class R> __global__ __launch_bounds__(256) void k_momenta(Geo g, R *P,
 template <class R>
 __global__ __launch_bounds__(256) void k_plaq(Geo g, const R *__restrict__ U, double *partials, int nb, int remap)
 {
-	int lb = logical_block(nb, remap);
-	if (lb >= nb)
+	int lb = ordered_block(g, nb, 256, remap);
+	if (lb < 0)
 		return;
 	long x = (long)lb * 256 + threadIdx.x;
 	double v[1] = {0.0};
@@ -141,8 +141,8 @@ __global__ __launch_bounds__(256) void k_delta_h(Geo g, const R *__restrict__ Uo
                                                  const R *__restrict__ P, const double *__restrict__ Ko,
                                                  double c3, double *partials, int nb, int remap)
 {
-	int lb = logical_block(nb, remap);
-	if (lb >= nb)
+	int lb = ordered_block(g, nb, 256, remap);
+	if (lb < 0)
 		return;
 	long x = (long)lb * 256 + threadIdx.x;
 	double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -193,14 +193,35 @@ __global__ __launch_bounds__(256) void k_force(Geo g, const R *__restrict__ U, i
 	store(F, 0, g, x, A);
 }
 
+// the link step that may follow a momentum step in the same thread:
+// Unext_mu(x) = exp(tl TA(p)) u (+ reunitarisation), u = U_mu(x), p = new P_mu(x)
+template <class R>
+__device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Unext, int mu, long x, m3<R> const &p,
+                                                m3<R> const &u, R tl, int reunit_flag)
+{
+	m3<R> X = ta(p);
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		X.e[k].re *= tl;
+		X.e[k].im *= tl;
+	}
+	m3<R> un = mul_nn(expsu3(X), u);
+	if (reunit_flag)
+		un = reunit(un);
+	store(Unext, mu, g, x, un);
+}
+
 // fused momentum step: workgroup = 64 sites x 4 directions, wavefront w
-// handles direction w of the workgroup's sites
+// handles direction w of the workgroup's sites.  With Unext != nullptr the
+// following link step is fused in: Unext = exp(tl TA(P_new)) U (Unext != U,
+// since neighbouring threads still read U).
 template <class R>
 __global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
-                                                      int nb, int remap)
+                                                      int nb, int remap, R *__restrict__ Unext, R tl, int reunit_flag)
 {
-	int lb = logical_block(nb, remap);
-	if (lb >= nb)
+	int lb = ordered_block(g, nb, SB, remap);
+	if (lb < 0)
 		return;
 	int mu = threadIdx.x >> 6;
 	long x = (long)lb * SB + (threadIdx.x & 63);
@@ -209,7 +230,8 @@ __global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict
 	int c[4];
 	coords<R>(g, x, c);
 	m3<R> S = staple_at(g, U, mu, x, c);
-	m3<R> A = ta(mul_nn(load(U, mu, g, x), S));
+	m3<R> u = load(U, mu, g, x);
+	m3<R> A = ta(mul_nn(u, S));
 	m3<R> p = load(P, mu, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
@@ -218,6 +240,8 @@ __global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
 	store(P, mu, g, x, p);
+	if (Unext)
+		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 }
 
 // Fused momentum step with an LDS tile (4-d lattices whose fastest extent
@@ -229,13 +253,14 @@ __global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict
 // direction w; the arithmetic is exactly that of k_force_update.
 template <class R, int L3>
 __global__ __launch_bounds__(256) void k_force_update_tile(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
-                                                           int nb, int remap)
+                                                           int nb, int remap, R *__restrict__ Unext, R tl,
+                                                           int reunit_flag)
 {
 	constexpr int RW = 64 / L3; // rows per tile
 	constexpr int NR = RW + 2;  // with halo rows
 	__shared__ vec2_t<R> lds[NR][4][9][L3];
-	int lb = logical_block(nb, remap);
-	if (lb >= nb)
+	int lb = ordered_block(g, nb, 64, remap);
+	if (lb < 0)
 		return;
 	const long base = (long)lb * 64; // first site of the tile (x3 = 0 of its first row)
 	const int x2_0 = (int)((base / L3) % g.L[2]);
@@ -302,7 +327,8 @@ __global__ __launch_bounds__(256) void k_force_update_tile(Geo g, const R *__res
 			madd_nn(S, T2, nloc ? from_lds(nu, d2(nu, -1), d3(nu, -1)) : load(U, nu, g, xmn));
 		}
 	}
-	m3<R> A = ta(mul_nn(from_lds(mu, 0, 0), S));
+	m3<R> u = from_lds(mu, 0, 0);
+	m3<R> A = ta(mul_nn(u, S));
 	m3<R> p = load(P, mu, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
@@ -311,6 +337,8 @@ __global__ __launch_bounds__(256) void k_force_update_tile(Geo g, const R *__res
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
 	store(P, mu, g, x, p);
+	if (Unext)
+		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 }
 
 // variant: one thread per site looping over the directions
@@ -318,8 +346,8 @@ template <class R>
 __global__ __launch_bounds__(256) void k_force_update_site(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
                                                            int nb, int remap)
 {
-	int lb = logical_block(nb, remap);
-	if (lb >= nb)
+	int lb = ordered_block(g, nb, 256, remap);
+	if (lb < 0)
 		return;
 	long x = (long)lb * 256 + threadIdx.x;
 	if (x >= g.V)
@@ -437,12 +465,20 @@ int plaq_sum(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, double *out
 
 double n_plaquettes(Geo const &geo) { return (double)geo.V * geo.nd * (geo.nd - 1) / 2; }
 
-int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, void *P, double beta, double delta)
+int launch_link_update(mesh_ctx *ctx, Geo const &geo, int nf, int prec, void *Uout, const void *Uin, const void *P,
+                       double t, int reunit_flag);
+
+// momentum step P -= delta beta/6 TA(U Sigma); with Unext != nullptr the
+// link step Unext = exp(tl TA(P)) U (+ reunit) follows, fused into the same
+// kernel where the variant supports it (Unext must differ from U)
+int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, void *P, double beta, double delta,
+                        void *Unext = nullptr, double tl = 0.0, int reunit_flag = 0)
 {
 	double k = delta * (beta / 6.0);
-	int ph = mesh_prof_begin(ctx, PROF_FORCE_UPDATE);
+	int ph = mesh_prof_begin(ctx, Unext ? PROF_FORCE_LINK : PROF_FORCE_UPDATE);
 	const int L3 = geo.L[3], RW = L3 > 0 ? 64 / L3 : 0;
 	const bool tile_ok = geo.nd == 4 && (L3 == 16 || L3 == 32 || L3 == 64) && geo.L[2] % RW == 0;
+	bool fused = Unext != nullptr;
 	if (ctx->force_variant == 2 && tile_ok)
 	{
 		int nb = (int)(geo.V / 64);
@@ -450,7 +486,7 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 		auto launch = [&](auto tag, auto l3) {
 			using Rt = decltype(tag);
 			k_force_update_tile<Rt, decltype(l3)::value><<<gb, 256, 0, ctx->stream>>>(
-			    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, ctx->xcd_remap);
+			    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, ctx->xcd_remap, (Rt *)Unext, (Rt)tl, reunit_flag);
 		};
 		using I16 = std::integral_constant<int, 16>;
 		using I32 = std::integral_constant<int, 32>;
@@ -470,6 +506,7 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 		else
 			k_force_update_site<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, (float)k,
 			                                                         nb, ctx->xcd_remap);
+		fused = false;
 	}
 	else
 	{
@@ -477,13 +514,16 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 		int gb = mesh_grid_blocks(ctx, nb);
 		if (prec == MESH_F64)
 			k_force_update<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, (double *)P, k, nb,
-			                                                     ctx->xcd_remap);
+			                                                     ctx->xcd_remap, (double *)Unext, tl, reunit_flag);
 		else
 			k_force_update<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, (float)k, nb,
-			                                                    ctx->xcd_remap);
+			                                                    ctx->xcd_remap, (float *)Unext, (float)tl,
+			                                                    reunit_flag);
 	}
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
+	if (Unext && !fused)
+		return launch_link_update(ctx, geo, geo.nd, prec, Unext, U, P, tl, reunit_flag);
 	return MESH_OK;
 }
 
@@ -804,16 +844,51 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 	int rc;
 	if ((rc = launch_momenta(ctx, s.geo, prec, P, K, seed, traj)))
 		return rc;
-	for (int i = 0; i < nd_; ++i)
+	const int nfused = (nd_ - 1) / 2;
+	if (ctx->fuse_md && nfused > 0)
 	{
-		if (i % 2 == 0)
-			rc = launch_link_update(ctx, s.geo, s.geo.nd, prec, *U_work, i == 0 ? *U : *U_work, P, deltas[i],
-			                        i == nd_ - 1);
-		else
-			rc = launch_force_update(ctx, s.geo, prec, *U_work, P, beta, deltas[i]);
-		if (rc)
+		// momentum step i fused with link step i+1, ping-pong between the
+		// caller's work field and a spare; the first link step writes the
+		// field that makes the last one land in *U_work
+		size_t fb = mesh_su3_field_bytes(g, s.geo.nd, prec);
+		if (fb > ctx->md_spare_bytes)
+		{
+			if (ctx->md_spare)
+			{
+				MESH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+				MESH_HIP(ctx, hipFree(ctx->md_spare));
+				ctx->md_spare = nullptr;
+				ctx->md_spare_bytes = 0;
+			}
+			if (hipMalloc(&ctx->md_spare, fb) != hipSuccess)
+				return mesh_fail(ctx, MESH_E_ALLOC, "cannot allocate the spare field of the fused trajectory");
+			ctx->md_spare_bytes = fb;
+		}
+		void *cur = (nfused % 2) ? ctx->md_spare : *U_work;
+		void *other = (nfused % 2) ? *U_work : ctx->md_spare;
+		if ((rc = launch_link_update(ctx, s.geo, s.geo.nd, prec, cur, *U, P, deltas[0], 0)))
 			return rc;
+		for (int i = 1; i + 1 < nd_; i += 2)
+		{
+			if ((rc = launch_force_update(ctx, s.geo, prec, cur, P, beta, deltas[i], other, deltas[i + 1],
+			                              i + 1 == nd_ - 1)))
+				return rc;
+			void *t = cur;
+			cur = other;
+			other = t;
+		}
 	}
+	else
+		for (int i = 0; i < nd_; ++i)
+		{
+			if (i % 2 == 0)
+				rc = launch_link_update(ctx, s.geo, s.geo.nd, prec, *U_work, i == 0 ? *U : *U_work, P, deltas[i],
+				                        i == nd_ - 1);
+			else
+				rc = launch_force_update(ctx, s.geo, prec, *U_work, P, beta, deltas[i]);
+			if (rc)
+				return rc;
+		}
 	double dh[5];
 	if ((rc = delta_h(ctx, s.geo, prec, *U, *U_work, P, K, beta, dh)))
 		return rc;

@@ -1,0 +1,242 @@
+"""U(1) 2-d ensembles on the GPU: batched HMC, the coupling-layer flow
+(training by reverse KL, sampling, inverse), independence Metropolis-Hastings
+interleaved with HMC, and the autocorrelation analysis that judges them.
+
+Everything numerical runs in libmesh_hip.so (u1.hip, flow.hip); this module
+only sequences launches and reads back per-chain scalars.
+"""
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from ._lib import MeshError, lib
+from .lattice import Grid, default_context
+
+_d = C.c_double
+
+
+class Buffer:
+    """Device buffer of `n` elements of a numpy dtype."""
+
+    def __init__(self, n, dtype, ctx):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx.call("mesh_malloc", C.byref(p), self.nbytes)
+        self.ptr = p
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        self.ctx.call("mesh_memcpy_to_host", out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes)
+        return out
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype).reshape(-1)
+        if a.size != self.n:
+            raise ValueError("size mismatch")
+        self.ctx.call("mesh_memcpy_from_host", self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes)
+        return self
+
+    def copy_from(self, other):
+        self.ctx.call("mesh_memcpy", self.ptr, other.ptr, self.nbytes)
+        return self
+
+    def zero(self):
+        self.ctx.call("mesh_memclear", self.ptr, self.nbytes)
+        return self
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                self.ctx._L.mesh_free(self.ctx.handle, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class U1Ensemble:
+    """`batch` independent U(1) configurations of an L0 x L1 lattice."""
+
+    def __init__(self, L0, L1, batch, ctx=None):
+        self.ctx = ctx or default_context()
+        self.grid = Grid((L0, L1))
+        self.L0, self.L1, self.batch, self.V = L0, L1, batch, L0 * L1
+        self.theta = Buffer(batch * 2 * self.V, np.float32, self.ctx)
+        self._obs = Buffer(4 * batch, np.float64, self.ctx)
+        self._hmc = Buffer(4 * batch, np.float64, self.ctx)
+
+    def random_start(self, seed):
+        self.ctx.call("mesh_u1_random", self.grid.ref, self.batch, self.theta.ptr, C.c_uint64(seed))
+
+    def cold_start(self):
+        self.theta.zero()
+
+    def angles(self):
+        return self.theta.download().reshape(self.batch, 2, self.L0, self.L1)
+
+    def set_angles(self, a):
+        self.theta.upload(np.asarray(a, np.float32))
+
+    def observables(self):
+        """dict of per-chain arrays: action/beta, Q, sum cos theta_P."""
+        self.ctx.call("mesh_u1_observables", self.grid.ref, self.batch, self.theta.ptr, self._obs.ptr)
+        o = self._obs.download().reshape(self.batch, 4)
+        return dict(S=o[:, 0], Q=o[:, 1], cos=o[:, 2] / self.V)
+
+    def hmc(self, beta, deltas, seed, traj, metropolis=True):
+        """One trajectory of every chain; returns dict(dH, acc, S) arrays."""
+        d = np.ascontiguousarray(deltas, np.float64)
+        self.ctx.call("mesh_u1_hmc", self.grid.ref, self.batch, self.theta.ptr,
+                      d.ctypes.data_as(C.POINTER(C.c_double)), len(d), _d(beta), C.c_uint64(seed),
+                      C.c_uint32(traj), int(bool(metropolis)), self._hmc.ptr)
+        o = self._hmc.download().reshape(self.batch, 4)
+        return dict(dH=o[:, 0], acc=o[:, 1], S=o[:, 3])
+
+
+class Flow:
+    """Gauge-equivariant coupling-layer flow on an L0 x L1 lattice."""
+
+    def __init__(self, L0, L1, ctx=None, seed=0):
+        self.ctx = ctx or default_context()
+        self.grid = Grid((L0, L1))
+        self.L0, self.L1, self.V = L0, L1, L0 * L1
+        self.n = lib().mesh_flow_num_params()
+        self.params = Buffer(self.n, np.float32, self.ctx)
+        self.m = Buffer(self.n, np.float32, self.ctx).zero()
+        self.v = Buffer(self.n, np.float32, self.ctx).zero()
+        self.grad = Buffer(self.n, np.float32, self.ctx)
+        self.t = 0
+        self._ws = None
+        self._ws_batch = 0
+        self.ctx.call("mesh_flow_init", self.params.ptr, C.c_uint64(seed))
+
+    def get_params(self):
+        return self.params.download()
+
+    def set_params(self, p):
+        self.params.upload(p)
+
+    def _workspace(self, batch):
+        if self._ws is None or self._ws_batch != batch:
+            nb = lib().mesh_flow_workspace_bytes(self.grid.ref, batch)
+            if nb == 0:
+                raise MeshError(-2, "unsupported flow geometry")
+            self._ws = Buffer(nb, np.uint8, self.ctx)
+            self._out = Buffer(2 * batch, np.float64, self.ctx)
+            self._ws_batch = batch
+        return self._ws
+
+    def gradient(self, batch, beta, seed, step):
+        """Reverse-KL gradient over a fresh batch; returns (loss, log q, S/beta)."""
+        ws = self._workspace(batch)
+        self.ctx.call("mesh_flow_gradient", self.grid.ref, batch, self.params.ptr, C.c_uint64(seed),
+                      C.c_uint32(step), _d(beta), ws.ptr, self.grad.ptr, self._out.ptr)
+        o = self._out.download().reshape(batch, 2)
+        logq, S = o[:, 0], o[:, 1]
+        return float(np.mean(logq + beta * S)), logq, S
+
+    def train(self, beta, steps, batch=256, lr=1e-3, seed=1, log_every=0, lr_final=None):
+        """Adam on the reverse KL; returns the history of the mean loss
+        E[log q + beta S] (= KL(q||p) - log Z)."""
+        hist = []
+        for i in range(steps):
+            loss, _, _ = self.gradient(batch, beta, seed, self.t)
+            self.t += 1
+            lr_i = lr if lr_final is None else lr * (lr_final / lr) ** (i / max(1, steps - 1))
+            self.ctx.call("mesh_flow_adam", self.params.ptr, self.grad.ptr, self.m.ptr, self.v.ptr, self.t,
+                          _d(lr_i))
+            hist.append(loss)
+            if log_every and (i % log_every == 0 or i == steps - 1):
+                print(f"flow step {i}: loss {loss:.4f}", flush=True)
+        return np.array(hist)
+
+    def sample(self, ens, seed, step, theta_out=None, out=None):
+        """Fill `theta_out` (or the ensemble) with proposals; returns
+        per-chain (log q, S/beta) as a device buffer of 2*batch doubles."""
+        tgt = theta_out if theta_out is not None else ens.theta
+        out = out or Buffer(2 * ens.batch, np.float64, self.ctx)
+        self.ctx.call("mesh_flow_sample", self.grid.ref, ens.batch, self.params.ptr, C.c_uint64(seed),
+                      C.c_uint32(step), tgt.ptr, None, out.ptr)
+        return out
+
+    def logq(self, ens, out=None, z_out=None):
+        out = out or Buffer(2 * ens.batch, np.float64, self.ctx)
+        self.ctx.call("mesh_flow_logq", self.grid.ref, ens.batch, self.params.ptr, ens.theta.ptr,
+                      None if z_out is None else z_out.ptr, out.ptr)
+        return out
+
+
+def tau_int(series, S=1.5):
+    """Integrated autocorrelation time by the Gamma method with Wolff's
+    automatic window (U. Wolff, Comput. Phys. Commun. 156 (2004) 143).
+    series: (n,) or (replicas, n).  Returns (tau_int, error)."""
+    x = np.atleast_2d(np.asarray(series, np.float64))
+    R, n = x.shape
+    xm = x - x.mean()
+    var = np.mean(xm * xm)
+    if var == 0:
+        return 0.5, 0.0
+    csum = 0.0
+    W = 0
+    tau = 0.5
+    for W in range(1, n // 2):
+        g = np.mean(xm[:, :-W] * xm[:, W:]) / var
+        csum += g
+        tau = 0.5 + csum
+        if tau <= 0.5:
+            tau = 0.5
+            break
+        tauW = S / math.log((2 * tau + 1) / (2 * tau - 1))
+        if math.exp(-W / tauW) - tauW / math.sqrt(W * n * R) < 0:
+            break
+    err = tau * math.sqrt(4 * (W + 0.5 - tau) / (n * R))
+    return tau, err
+
+
+def run_chains(ens, beta, deltas, n_steps, seed, flow=None, n_therm=0, record_every=1):
+    """Markov chains over the ensemble.  Without `flow`: plain HMC.  With it:
+    each step is one HMC trajectory followed by one independence MH step
+    with a flow proposal (1:1), exact accept/reject against log q computed
+    by the inverse flow for the state HMC produced.
+
+    Returns dict(Q (chains x steps), plaq, acc_hmc, acc_flow, seconds)."""
+    ctx = ens.ctx
+    Qs, Ps = [], []
+    acc_h, acc_f = [], []
+    if flow is not None:
+        prop = Buffer(ens.batch * 2 * ens.V, np.float32, ctx)
+        state = Buffer(2 * ens.batch, np.float64, ctx)
+        pout = Buffer(2 * ens.batch, np.float64, ctx)
+        accb = Buffer(ens.batch, np.float64, ctx)
+    ctx.synchronize()
+    t0 = None
+    for it in range(n_therm + n_steps):
+        if it == n_therm:
+            ctx.synchronize()
+            t0 = time.perf_counter()
+        r = ens.hmc(beta, deltas, seed, it)
+        acc_h.append(r["acc"].mean())
+        if flow is not None:
+            flow.logq(ens, out=state)
+            flow.sample(ens, seed + 7919, it, theta_out=prop, out=pout)
+            ctx.call("mesh_flow_mh", ens.grid.ref, ens.batch, ens.theta.ptr, state.ptr, prop.ptr, pout.ptr,
+                     _d(beta), C.c_uint64(seed), C.c_uint32(it), accb.ptr)
+            acc_f.append(accb.download().mean())
+        if it >= n_therm and (it - n_therm) % record_every == 0:
+            o = ens.observables()
+            Qs.append(o["Q"])
+            Ps.append(o["cos"])
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    return dict(Q=np.array(Qs).T, plaq=np.array(Ps).T, acc_hmc=float(np.mean(acc_h[n_therm:])),
+                acc_flow=float(np.mean(acc_f[n_therm:])) if acc_f else None, seconds=dt)
+
+
+def ess_per_second(res, obs="Q"):
+    """effective samples per second of an observable over all chains."""
+    x = res[obs]
+    tau, err = tau_int(x)
+    n = x.shape[0] * x.shape[1]
+    return n / (2 * tau) / res["seconds"], tau, err

@@ -50,6 +50,7 @@ def _declare(L):
     L.orc_link_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _dp]
     L.orc_metropolis_uniform.argtypes = [C.c_uint64, C.c_uint32]
     L.orc_metropolis_uniform.restype = C.c_double
+    L.orc_philox_draw.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _dp]
     for n in ["orc_su3_mul_nn"]:
         getattr(L, n).argtypes = [_dp, _dp, _dp]
     for n in ["orc_su3_ta", "orc_su3_exp", "orc_su3_reunit", "orc_su3_det",
@@ -134,6 +135,13 @@ def link_normals(seed, tag, ctr2, site, mu, n):
 
 def metropolis_uniform(seed, traj):
     return lib().orc_metropolis_uniform(seed, traj)
+
+
+def philox_draw(seed, c0, c1, c2, c3):
+    """(u01 in (0,1], normal z0, normal z1) from one Philox4x32-10 block."""
+    out = np.zeros(3)
+    lib().orc_philox_draw(seed, c0, c1, c2, c3, out)
+    return out
 
 
 def _unary(name, a):

@@ -520,6 +520,18 @@ void orc_link_normals(uint64_t seed, uint32_t tag, uint32_t ctr2, uint32_t site,
 }
 double orc_metropolis_uniform(uint64_t seed, uint32_t traj) { return metropolis_uniform(seed, traj); }
 
+// Philox block of an arbitrary counter -> (u01 in (0,1], z0, z1) as used by
+// the U(1) and flow streams
+void orc_philox_draw(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, double *out3)
+{
+	uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+	uint32_t ctr[4] = {c0, c1, c2, c3};
+	uint32_t r[4];
+	philox4x32_10(ctr, key, r);
+	out3[0] = u01_from_bits(r[0], r[1]);
+	normal_pair(r, &out3[1], &out3[2]);
+}
+
 // element ops on 18-real matrices (double)
 void orc_su3_mul_nn(double const *a, double const *b, double *c)
 {

@@ -1,0 +1,243 @@
+"""CPU oracle of the U(1) 2-d path: observables, HMC, exact finite-volume
+results, and the coupling-layer flow.
+
+TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's cpu_baseline;
+never by mesh_amd).  Conventions follow mesh_amd/csrc/u1.hip and flow.hip,
+which follow the reference's U(1) algebra (src/gauge/groups.h:29-108):
+angles theta[mu][x0][x1], theta_P = theta_0(x) + theta_1(x+0) - theta_0(x+1)
+- theta_1(x), S = beta sum (1 - cos theta_P), momenta q ~ N(0, 1/2)
+(groups.h:86-91), force (beta/2) Im(U S).
+
+Numerics: numpy float64 for the lattice parts; the flow is restated in torch
+(CPU, float64) so that its gradients come from autograd.
+"""
+import math
+
+import numpy as np
+
+from . import oracle as O
+
+TAG_U1MOM = 0x55314D4F
+TAG_U1METRO = 0x55314D45
+TAG_U1RAND = 0x55315244
+TAG_FLOWZ = 0x464C575A
+
+
+def wrap(a):
+    return np.remainder(np.asarray(a) + np.pi, 2 * np.pi) - np.pi
+
+
+def plaq(th):
+    """th: (..., 2, L0, L1) -> theta_P (..., L0, L1) (unwrapped)."""
+    t0, t1 = th[..., 0, :, :], th[..., 1, :, :]
+    return t0 + np.roll(t1, -1, axis=-2) - np.roll(t0, -1, axis=-1) - t1
+
+
+def action_over_beta(th):
+    return np.sum(1 - np.cos(plaq(th)), axis=(-2, -1))
+
+
+def charge(th):
+    return np.rint(np.sum(wrap(plaq(th)), axis=(-2, -1)) / (2 * np.pi))
+
+
+def dS(th, beta):
+    """dS/dtheta, shape of th."""
+    s = np.sin(plaq(th))
+    g0 = beta * (s - np.roll(s, 1, axis=-1))
+    g1 = beta * (np.roll(s, 1, axis=-2) - s)
+    return np.stack([g0, g1], axis=-3)
+
+
+def _draw(seed, c0, c1, c2, c3):
+    return O.philox_draw(seed, c0, c1, c2, c3)
+
+
+def random_angles(seed, chain, L0, L1):
+    V = L0 * L1
+    out = np.zeros((2, V), np.float32)
+    for mu in range(2):
+        for x in range(V):
+            u = _draw(seed, x, (chain << 1) | mu, 0, TAG_U1RAND)[0] - 2.0 ** -53
+            out[mu, x] = np.float32(6.283185307179586477 * u - 3.141592653589793238)
+    return out.reshape(2, L0, L1)
+
+
+def momenta(seed, traj, chain, L0, L1):
+    V = L0 * L1
+    q = np.zeros((2, V), np.float32)
+    for mu in range(2):
+        for x in range(V):
+            q[mu, x] = np.float32(_draw(seed, x, (chain << 1) | mu, traj << 5, TAG_U1MOM)[1] * 0.70710678118654752440)
+    return q.reshape(2, L0, L1)
+
+
+def metropolis_uniform(seed, traj, chain):
+    return _draw(seed, traj, chain, 0, TAG_U1METRO)[0]
+
+
+def hmc_trajectory(th, q, beta, deltas):
+    """float64 leapfrog/Omelyan over a step table (hmc.h:38-59); returns
+    (theta', q', Delta H)."""
+    th = np.asarray(th, np.float64).copy()
+    q = np.asarray(q, np.float64).copy()
+    e0 = beta * action_over_beta(th) + np.sum(q * q)
+    for i, d in enumerate(deltas):
+        if i % 2 == 0:
+            th = wrap(th + d * q)
+        else:
+            q = q - d * 0.5 * dS(th, beta)
+    e1 = beta * action_over_beta(th) + np.sum(q * q)
+    return th, q, e1 - e0
+
+
+# ---------------------------------------------------------------- exact results
+
+def _c_nu(beta, nu):
+    from scipy.integrate import quad
+
+    return quad(lambda p: math.exp(beta * math.cos(p)) * math.cos(nu * p), 0, math.pi, limit=200)[0] / math.pi
+
+
+def exact_mean_cos(beta, V, nmax=30):
+    """<cos theta_P> on a periodic torus of V plaquettes: Z = sum_n I_n(beta)^V."""
+    from scipy.special import ive
+
+    n = np.arange(-nmax, nmax + 1)
+    # scaled Bessel functions: I_n = ive * e^beta; ratios to I_0
+    r = ive(n, beta) / ive(0, beta)
+    dr = 0.5 * (ive(n - 1, beta) + ive(n + 1, beta)) / ive(0, beta)
+    return float(np.sum(r ** (V - 1) * dr) / np.sum(r ** V))
+
+
+def exact_chi_top(beta, V, nmax=12):
+    """<Q^2>/V for the geometric charge on a periodic torus:
+    Z(theta) = sum_n c(n - theta/2pi)^V, c(nu) = (1/pi) int_0^pi e^{beta cos p} cos(nu p) dp."""
+    from scipy.integrate import quad
+
+    num = den = 0.0
+    scale = _c_nu(beta, 0)
+    for n in range(-nmax, nmax + 1):
+        c = _c_nu(beta, n) / scale
+        c1 = -quad(lambda p: p * math.exp(beta * math.cos(p)) * math.sin(n * p), 0, math.pi, limit=200)[0] / math.pi
+        c2 = -quad(lambda p: p * p * math.exp(beta * math.cos(p)) * math.cos(n * p), 0, math.pi, limit=200)[0] / math.pi
+        c1 /= scale
+        c2 /= scale
+        # d/dtheta = -(1/2pi) d/dnu
+        ct1, ct2 = -c1 / (2 * math.pi), c2 / (2 * math.pi) ** 2
+        den += c ** V
+        num += V * (V - 1) * c ** (V - 2) * ct1 ** 2 + V * c ** (V - 1) * ct2
+    return -num / den / V
+
+
+# ---------------------------------------------------------------- flow (torch)
+
+NL, NK, NI, NH = 8, 4, 12, 64
+NO = 2 * NK + 2
+NP_LAYER = NI * NH + NH + NH * NH + NH + NH * NO + NO
+
+
+def flow_z(seed, step, chain, L0, L1):
+    """prior sample the GPU draws for (seed, step, chain)."""
+    V = L0 * L1
+    z = np.zeros((2, V), np.float32)
+    for mu in range(2):
+        for x in range(V):
+            u = _draw(seed, x, (chain << 1) | mu, step << 5, TAG_FLOWZ)[0] - 2.0 ** -53
+            z[mu, x] = np.float32(6.283185307179586477 * u - 3.141592653589793238)
+    return z.reshape(2, L0, L1)
+
+
+def _layer_params(p, layer):
+    o = layer * NP_LAYER
+    W1 = p[o:o + NI * NH].reshape(NI, NH)
+    o += NI * NH
+    b1 = p[o:o + NH]
+    o += NH
+    W2 = p[o:o + NH * NH].reshape(NH, NH)
+    o += NH * NH
+    b2 = p[o:o + NH]
+    o += NH
+    W3 = p[o:o + NH * NO].reshape(NH, NO)
+    o += NH * NO
+    b3 = p[o:o + NO]
+    return W1, b1, W2, b2, W3, b3
+
+
+def _tplaq(th):
+    import torch
+
+    t0, t1 = th[:, 0], th[:, 1]
+    return t0 + torch.roll(t1, -1, dims=1) - torch.roll(t0, -1, dims=2) - t1
+
+
+def _twrap(a):
+    import torch
+
+    return torch.remainder(a + math.pi, 2 * math.pi) - math.pi
+
+
+def flow_forward(params, z, inverse=False):
+    """torch float64 flow on a batch z (B, 2, L0, L1).  Returns (theta,
+    logdet) with logdet = sum over layers and active plaquettes of log g'(u).
+    The same definitions as mesh_amd/csrc/flow.hip (layer masks, features,
+    NCP mixture, shift by atan2)."""
+    import torch
+
+    th = z.clone()
+    B, _, L0, L1 = th.shape
+    x0 = torch.arange(L0).view(L0, 1).expand(L0, L1)
+    x1 = torch.arange(L1).view(1, L1).expand(L0, L1)
+    logdet = torch.zeros(B, dtype=th.dtype)
+    for layer in range(NL):
+        mu, a = layer & 1, (layer >> 1) & 3
+        W1, b1, W2, b2, W3, b3 = _layer_params(params, layer)
+        P = _tplaq(th)
+        feats = []
+        for jj in range(2):
+            for i in (-1, 0, 1):
+                if mu == 0:
+                    sh = (-i, -(1 + jj))       # feature(y) = P(y + (i, 1 + jj))
+                else:
+                    sh = (1 + jj, -i)          # feature(y) = P(y + (-1 - jj, i))
+                F = torch.roll(P, shifts=sh, dims=(1, 2))
+                feats += [torch.cos(F), torch.sin(F)]
+        X = torch.stack(feats, dim=-1)                          # (B, L0, L1, 12)
+        mask = ((x1 if mu == 0 else x0) % 4) == a
+        Xa = X[:, mask]                                         # (B, NA, 12)
+        H1 = torch.tanh(Xa @ W1 + b1)
+        H2 = torch.tanh(H1 @ W2 + b2)
+        Y = H2 @ W3 + b3
+        s = torch.exp(torch.clamp(Y[..., :NK], -8.0, 8.0))
+        w = torch.softmax(Y[..., NK:2 * NK], dim=-1)
+        t = torch.atan2(Y[..., 2 * NK + 1], Y[..., 2 * NK])
+        tp = P[:, mask]
+        u = _twrap(tp - t)
+        c, sn = torch.cos(0.5 * u), torch.sin(0.5 * u)
+        G = torch.sum(w * 2 * torch.atan2(s * sn.unsqueeze(-1), c.unsqueeze(-1)), dim=-1)
+        Gp = torch.sum(w * s / (c.unsqueeze(-1) ** 2 + s ** 2 * sn.unsqueeze(-1) ** 2), dim=-1)
+        delta = G + t - tp
+        logdet = logdet + torch.sum(torch.log(Gp), dim=-1)
+        D = torch.zeros(B, L0, L1, dtype=th.dtype)
+        D[:, mask] = delta
+        if mu == 0:
+            new0 = _twrap(th[:, 0] + D)
+            th = torch.stack([new0, th[:, 1]], dim=1)
+        else:
+            new1 = _twrap(th[:, 1] + torch.roll(D, 1, dims=1))
+            th = torch.stack([th[:, 0], new1], dim=1)
+    return th, logdet
+
+
+def flow_loss(params, z, beta):
+    """mean over the batch of beta S(f(z)) - log det (reverse KL up to constants)."""
+    import torch
+
+    th, ld = flow_forward(params, z)
+    S = torch.sum(1 - torch.cos(_tplaq(th)), dim=(1, 2))
+    return torch.mean(beta * S - ld), th, ld, S
+
+
+def log_prior(L0, L1):
+    return -2 * L0 * L1 * math.log(2 * math.pi)
+

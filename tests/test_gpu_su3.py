@@ -112,7 +112,7 @@ def test_force_update_bitexact(ctx, shape, prec):
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("prec", [F64, F32])
 @pytest.mark.parametrize("shape", [(4, 4, 2, 32), (2, 3, 4, 16), (2, 2, 2, 64), (3, 2, 6, 32), (4, 4, 4, 4)])
-@pytest.mark.parametrize("remap", [0, 1])
+@pytest.mark.parametrize("remap", [0, 1, 2])
 def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
     """Every force-kernel variant (wavefront per direction, thread per site,
     LDS tile; tile-eligible shapes included) gives the oracle's bits."""
@@ -126,7 +126,7 @@ def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
         gauge.momentumStep(fu, fp, 6.0, 0.0833)
     finally:
         ctx.set_option("force_variant", 0)
-        ctx.set_option("xcd_remap", 1)
+        ctx.set_option("xcd_remap", 2)
     np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
 
 
@@ -198,6 +198,30 @@ def test_hmc_trajectories_match_oracle(ctx, prec):
         assert r["accepted"] == o["accepted"]
         assert abs(r["plaq"] - o["plaq"]) <= 1e-13
         np.testing.assert_array_equal(hmc.U.download(), U)
+
+
+@pytest.mark.parametrize("fuse", [0, 1])
+@pytest.mark.parametrize("scheme,n", [("2lf", 3), ("2mn", 2), ("4mn", 1), ("2lf", 4)])
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_hmc_fused_md_bitexact(ctx, prec, scheme, n, fuse):
+    """The fused momentum+link step and the separate kernels give the oracle's
+    bits for every integrator; odd and even numbers of fused pairs exercise
+    both parities of the field ping-pong."""
+    shape = [4, 4, 6, 4]
+    g = mesh_amd.Grid(shape)
+    ctx.set_option("fuse_md", fuse)
+    try:
+        hmc = gauge.Hmc(g, prec, seed=5, ctx=ctx)
+        hmc.randomizeGaugeField()
+        U = O.random_gauge(shape, 5, _pname(prec))
+        deltas = gauge.makeDeltas(scheme, 0.4, n)
+        for traj in range(2):
+            r = hmc.runHmcUpdate(5.5, deltas, metropolis=True)
+            o = O.hmc_trajectory(U, shape, 5.5, deltas, 5, traj, metropolis=True)
+            assert r["accepted"] == o["accepted"]
+            np.testing.assert_array_equal(hmc.U.download(), U)
+    finally:
+        ctx.set_option("fuse_md", 1)
 
 
 def test_f32_vs_f64_same_config(ctx):

@@ -1,0 +1,248 @@
+"""U(1) 2-d HMC and the learned flow proposal on the GPU against the oracle:
+bit-exact random starts and Metropolis draws, FP32 HMC against the FP64
+oracle trajectory, the HMC ensemble against the exact torus <cos theta_P>,
+the flow's forward map / log-det / inverse / reverse-KL gradient against the
+torch float64 oracle, training against the exact free-energy bound, and the
+independence Metropolis-Hastings rule."""
+import math
+
+import numpy as np
+import pytest
+
+import mesh_amd
+from mesh_amd import gauge
+from mesh_amd import u1 as U1
+from oracle import oracle as O
+from oracle import u1_oracle as U
+
+pytestmark = pytest.mark.gpu
+
+TAG_FLOWMH = 0x464C4D48
+
+
+def _wrapdiff(a, b):
+    return np.abs(U.wrap(np.asarray(a, np.float64) - np.asarray(b, np.float64)))
+
+
+def _params(seed, scale):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(size=U.NL * U.NP_LAYER) * scale
+    for layer in range(U.NL):
+        p[layer * U.NP_LAYER + U.NP_LAYER - U.NO + 2 * U.NK] += 1.0
+    return p.astype(np.float32)
+
+
+def _log_z(beta, V):
+    """log of int prod dtheta exp(-beta S) on a periodic 2-d torus:
+    (2 pi)^{2V} e^{-beta V} sum_n I_n(beta)^V (character expansion)."""
+    import scipy.special as sp
+
+    terms = [V * math.log(sp.ive(n, beta)) for n in range(-30, 31)]
+    m = max(terms)
+    return 2 * V * math.log(2 * math.pi) + m + math.log(sum(math.exp(t - m) for t in terms))
+
+
+@pytest.mark.parametrize("L0,L1", [(8, 8), (4, 12), (6, 10)])
+def test_u1_random_bitexact(ctx, L0, L1):
+    ens = U1.U1Ensemble(L0, L1, 3, ctx)
+    ens.random_start(11)
+    got = ens.angles()
+    for c in range(3):
+        np.testing.assert_array_equal(got[c], U.random_angles(11, c, L0, L1))
+
+
+def test_u1_observables(ctx):
+    L0, L1, B = 8, 12, 6
+    ens = U1.U1Ensemble(L0, L1, B, ctx)
+    ens.random_start(12)
+    th = ens.angles().astype(np.float64)
+    o = ens.observables()
+    P = U.plaq(th)
+    np.testing.assert_allclose(o["S"], np.sum(1 - np.cos(P), axis=(-2, -1)), rtol=1e-5)
+    np.testing.assert_allclose(o["cos"], np.mean(np.cos(P), axis=(-2, -1)), atol=1e-5)
+    np.testing.assert_array_equal(o["Q"], U.charge(th))
+
+
+@pytest.mark.parametrize("scheme,n", [("2lf", 6), ("2mn", 3)])
+def test_u1_hmc_against_oracle(ctx, scheme, n):
+    """One FP32 trajectory per chain against the FP64 oracle from the same
+    angles and the same (bit-identical) momenta; Metropolis uniforms exact."""
+    L0, L1, B, beta, seed, traj = 8, 8, 4, 2.0, 21, 3
+    ens = U1.U1Ensemble(L0, L1, B, ctx)
+    ens.random_start(seed)
+    th0 = ens.angles().astype(np.float64)
+    d = gauge.makeDeltas(scheme, 0.5, n)
+    ens.hmc(beta, d, seed, traj, metropolis=False)
+    raw = ens._hmc.download().reshape(B, 4)
+    th1 = ens.angles()
+    for c in range(B):
+        q = U.momenta(seed, traj, c, L0, L1)
+        t_ref, _, dh_ref = U.hmc_trajectory(th0[c], q, beta, d)
+        assert _wrapdiff(th1[c], t_ref).max() < 2e-4
+        assert abs(raw[c, 0] - dh_ref) < 2e-3 * (1 + abs(dh_ref)), (c, raw[c, 0], dh_ref)
+        assert raw[c, 2] == U.metropolis_uniform(seed, traj, c)
+        assert raw[c, 3] == pytest.approx(U.action_over_beta(t_ref), rel=1e-4)
+
+
+def test_u1_hmc_rejection_keeps_field(ctx):
+    """A huge step size makes Delta H large: every chain rejects and the
+    angles stay bit-unchanged; acceptance follows u < exp(-dH) exactly."""
+    ens = U1.U1Ensemble(8, 8, 8, ctx)
+    ens.random_start(5)
+    before = ens.angles().copy()
+    r = ens.hmc(4.0, gauge.makeDeltas("2lf", 3.0, 2), 5, 0)
+    raw = ens._hmc.download().reshape(8, 4)
+    after = ens.angles()
+    for c in range(8):
+        a = raw[c, 2] < math.exp(-raw[c, 0])
+        assert bool(r["acc"][c]) == a
+        if not a:
+            np.testing.assert_array_equal(after[c], before[c])
+
+
+def test_u1_hmc_ensemble_matches_exact(ctx):
+    """<cos theta_P> of 512 independent chains (8x8, beta 2) after
+    thermalisation against the exact torus value, within 5 sigma."""
+    L, B, beta = 8, 512, 2.0
+    ens = U1.U1Ensemble(L, L, B, ctx)
+    ens.random_start(31)
+    d = gauge.makeDeltas("2mn", 1.0, 5)
+    acc = []
+    vals = []
+    for t in range(160):
+        r = ens.hmc(beta, d, 31, t)
+        if t >= 40:
+            acc.append(r["acc"].mean())
+            if t % 4 == 0:
+                vals.append(ens.observables()["cos"])
+    v = np.array(vals)          # (measurements, chains)
+    chain_means = v.mean(axis=0)
+    est = chain_means.mean()
+    err = chain_means.std(ddof=1) / math.sqrt(B)
+    exact = U.exact_mean_cos(beta, L * L)
+    assert np.mean(acc) > 0.6
+    assert abs(est - exact) < 5 * err + 1e-4, (est, exact, err)
+
+
+def test_flow_identity_initialisation(ctx):
+    L, B = 8, 4
+    f = U1.Flow(L, L, ctx)
+    ens = U1.U1Ensemble(L, L, B, ctx)
+    out = f.sample(ens, 3, 0).download().reshape(B, 2)
+    th = ens.angles()
+    for c in range(B):
+        z = U.flow_z(3, 0, c, L, L)
+        assert _wrapdiff(th[c], z).max() < 1e-5
+    assert np.abs(out[:, 0] - U.log_prior(L, L)).max() < 1e-3
+
+
+@pytest.mark.parametrize("L0,L1", [(8, 8), (4, 8)])
+def test_flow_forward_against_oracle(ctx, L0, L1):
+    torch = pytest.importorskip("torch")
+    B, seed, step = 4, 41, 7
+    p = _params(42, 0.2)
+    f = U1.Flow(L0, L1, ctx)
+    f.set_params(p)
+    ens = U1.U1Ensemble(L0, L1, B, ctx)
+    out = f.sample(ens, seed, step).download().reshape(B, 2)
+    th = ens.angles()
+    z = torch.tensor(np.stack([U.flow_z(seed, step, c, L0, L1) for c in range(B)]).astype(np.float64))
+    t_ref, ld = U.flow_forward(torch.tensor(p.astype(np.float64)), z)
+    t_ref, ld = t_ref.numpy(), ld.numpy()
+    assert np.abs(ld).max() > 0.5  # a non-trivial map
+    for c in range(B):
+        assert _wrapdiff(th[c], t_ref[c]).max() < 2e-3, c
+    np.testing.assert_allclose(out[:, 0], U.log_prior(L0, L1) - ld, atol=5e-3)
+    np.testing.assert_allclose(out[:, 1], U.action_over_beta(th.astype(np.float64)), rtol=1e-4)
+
+
+def test_flow_inverse_roundtrip(ctx):
+    """log q by the inverse flow of a sample equals the sampler's log q and
+    recovers its prior draw z."""
+    L, B, seed, step = 8, 4, 43, 2
+    f = U1.Flow(L, L, ctx)
+    f.set_params(_params(44, 0.2))
+    ens = U1.U1Ensemble(L, L, B, ctx)
+    fwd = f.sample(ens, seed, step).download().reshape(B, 2)
+    z_out = U1.Buffer(B * 2 * L * L, np.float32, ctx)
+    inv = f.logq(ens, z_out=z_out).download().reshape(B, 2)
+    z = z_out.download().reshape(B, 2, L, L)
+    for c in range(B):
+        assert _wrapdiff(z[c], U.flow_z(seed, step, c, L, L)).max() < 1e-3
+    np.testing.assert_allclose(inv[:, 0], fwd[:, 0], atol=2e-3)
+    np.testing.assert_allclose(inv[:, 1], fwd[:, 1], rtol=1e-6)
+
+
+def test_flow_gradient_against_autograd(ctx):
+    """mesh_flow_gradient against torch float64 autograd of
+    mean(beta S(f(z)) - log det f'(z)) on the same prior draws."""
+    torch = pytest.importorskip("torch")
+    L, B, beta, seed, step = 8, 8, 2.0, 45, 1
+    p = _params(46, 0.2)
+    f = U1.Flow(L, L, ctx)
+    f.set_params(p)
+    loss, logq, S = f.gradient(B, beta, seed, step)
+    g = f.grad.download().astype(np.float64)
+    pt = torch.tensor(p.astype(np.float64), requires_grad=True)
+    z = torch.tensor(np.stack([U.flow_z(seed, step, c, L, L) for c in range(B)]).astype(np.float64))
+    ref_loss, _, ld, Sref = U.flow_loss(pt, z, beta)
+    ref_loss.backward()
+    gr = pt.grad.numpy()
+    assert loss == pytest.approx(U.log_prior(L, L) + ref_loss.item(), abs=5e-3)
+    np.testing.assert_allclose(S, Sref.detach().numpy(), rtol=1e-4)
+    rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    assert rel < 1e-3, rel
+    # per layer as well, so that no layer's gradient hides under another's
+    for layer in range(U.NL):
+        sl = slice(layer * U.NP_LAYER, (layer + 1) * U.NP_LAYER)
+        assert np.linalg.norm(g[sl] - gr[sl]) <= 2e-3 * np.linalg.norm(gr[sl]) + 1e-6, layer
+
+
+def test_flow_training_respects_free_energy_bound(ctx):
+    """Adam on the reverse KL lowers E[log q + beta S] from its identity value
+    and never below -log Z (the KL is non-negative)."""
+    L, beta = 8, 2.0
+    f = U1.Flow(L, L, ctx)
+    hist = f.train(beta, 300, batch=128, lr=2e-3, seed=47)
+    bound = -_log_z(beta, L * L)
+    start = U.log_prior(L, L) + beta * L * L  # identity map: E[S] = V under the uniform prior
+    assert abs(hist[:3].mean() - start) < 3.0
+    assert hist[-30:].mean() < start - 10.0, (hist[:3].mean(), hist[-30:].mean())
+    assert hist[-30:].mean() > bound - 0.5, (hist[-30:].mean(), bound)
+
+
+def test_flow_mh_rule(ctx):
+    """Independence MH: acceptance exactly u < exp(lw' - lw) with the Philox
+    uniform of (step, chain); accepted chains take the proposal bits, the
+    others keep theirs, and the state's (log q, S/beta) follows."""
+    L, B, beta, seed, step = 8, 16, 2.0, 48, 9
+    f = U1.Flow(L, L, ctx)
+    f.set_params(_params(49, 0.1))
+    ens = U1.U1Ensemble(L, L, B, ctx)
+    ens.random_start(50)
+    state = U1.Buffer(2 * B, np.float64, ctx)
+    f.logq(ens, out=state)
+    prop = U1.Buffer(B * 2 * L * L, np.float32, ctx)
+    pout = f.sample(ens, seed + 1, step, theta_out=prop)
+    st0, pr = state.download().reshape(B, 2), pout.download().reshape(B, 2)
+    th0, pth = ens.angles().copy(), prop.download().reshape(B, 2, L, L)
+    acc = U1.Buffer(B, np.float64, ctx)
+    ctx.call("mesh_flow_mh", ens.grid.ref, B, ens.theta.ptr, state.ptr, prop.ptr, pout.ptr, U1._d(beta),
+             U1.C.c_uint64(seed), U1.C.c_uint32(step), acc.ptr)
+    a = acc.download()
+    st1, th1 = state.download().reshape(B, 2), ens.angles()
+    for c in range(B):
+        u = O.philox_draw(seed, step, c, 1, TAG_FLOWMH)[0]
+        lw_new = -beta * pr[c, 1] - pr[c, 0]
+        lw_old = -beta * st0[c, 1] - st0[c, 0]
+        expect = u < math.exp(lw_new - lw_old)
+        assert bool(a[c]) == expect, c
+        np.testing.assert_array_equal(th1[c], pth[c] if expect else th0[c])
+        np.testing.assert_array_equal(st1[c], pr[c] if expect else st0[c])
+
+
+def test_flow_errors(ctx):
+    with pytest.raises(mesh_amd.MeshError):
+        U1.Flow(6, 8, ctx).sample(U1.U1Ensemble(6, 8, 2, ctx), 0, 0)
+    with pytest.raises(mesh_amd.MeshError):
+        U1.U1Ensemble(128, 128, 1, ctx).hmc(1.0, [0.1], 0, 0)
