@@ -167,6 +167,50 @@ def kernel_only(name, prec, L, reps):
                       "launches": n, "copy_bytes_per_launch": 2 * U.bytes}), flush=True)
 
 
+def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed):
+    """U(1) 2-d at L x L: ESS/s of the topological charge Q for plain HMC
+    (2lf, tau = 1, 20 steps) and for HMC interleaved 1:1 with independence
+    MH steps proposed by the trained flow.  Training time is reported apart
+    from the sampling time the ESS/s figures use."""
+    import numpy as np
+
+    from mesh_amd import gauge
+    from mesh_amd import u1 as U1
+
+    deltas = gauge.makeDeltas("2lf", 0.5, 20)
+    out = {"metric": f"U(1) 2-d ESS/s of Q at L={L} beta={beta}, flow MH + HMC vs HMC",
+           "config": {"workload": f"U(1) {L}x{L} beta={beta} batch={batch}, HMC 2lf tau=1 20 steps, "
+                                  f"flow 8 coupling layers, MH 1:1", "L": L, "beta": beta, "batch": batch},
+           "dtype": "f32 angles, f64 energies and log-weights", "data": "synthetic: Philox-seeded starts"}
+    ens = U1.U1Ensemble(L, L, batch)
+    ens.random_start(seed)
+    r = U1.run_chains(ens, beta, deltas, hmc_steps, seed, n_therm=500)
+    ess, tau, err = U1.ess_per_second(r)
+    taup, _ = U1.tau_int(r["plaq"])
+    out["hmc"] = {"ess_per_s": ess, "tau_int_Q": tau, "tau_err": err, "tau_int_plaq": taup,
+                  "acceptance": r["acc_hmc"], "seconds": r["seconds"], "steps": hmc_steps,
+                  "chi_t": float(np.mean(r["Q"] ** 2) / (L * L)), "mean_cos": float(r["plaq"].mean())}
+    flow = U1.Flow(L, L, seed=seed)
+    t0 = time.perf_counter()
+    hist = flow.train(beta, train_steps, batch=batch, lr=1e-3, lr_final=1e-4, seed=seed + 1)
+    flow.ctx.synchronize()
+    t_train = time.perf_counter() - t0
+    ens2 = U1.U1Ensemble(L, L, batch)
+    ens2.random_start(seed + 2)
+    r2 = U1.run_chains(ens2, beta, deltas, flow_steps, seed + 3, flow=flow, n_therm=100)
+    ess2, tau2, err2 = U1.ess_per_second(r2)
+    taup2, _ = U1.tau_int(r2["plaq"])
+    out["flow"] = {"ess_per_s": ess2, "tau_int_Q": tau2, "tau_err": err2, "tau_int_plaq": taup2,
+                   "acceptance_hmc": r2["acc_hmc"], "acceptance_flow": r2["acc_flow"], "seconds": r2["seconds"],
+                   "steps": flow_steps, "train_steps": train_steps, "train_seconds": t_train,
+                   "loss_first": float(np.mean(hist[:10])), "loss_last": float(np.mean(hist[-50:])),
+                   "chi_t": float(np.mean(r2["Q"] ** 2) / (L * L)), "mean_cos": float(r2["plaq"].mean())}
+    out["value"] = ess2 / ess
+    out["unit"] = "ratio of ESS/s (flow / HMC)"
+    out["higher_is_better"] = True
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--kernel-only", default=None,
@@ -185,6 +229,14 @@ def main():
     ap.add_argument("--substeps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32-mixed run")
+    ap.add_argument("--u1", action="store_true",
+                    help="instead: U(1) 2-d ESS/s of Q, flow-assisted vs plain HMC (one JSON line)")
+    ap.add_argument("--u1-L", type=int, default=32)
+    ap.add_argument("--u1-beta", type=float, default=6.0)
+    ap.add_argument("--u1-batch", type=int, default=256)
+    ap.add_argument("--u1-train", type=int, default=2000)
+    ap.add_argument("--u1-hmc-steps", type=int, default=4000)
+    ap.add_argument("--u1-flow-steps", type=int, default=1000)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -193,6 +245,10 @@ def main():
 
     from mesh_amd import F32, F64
 
+    if a.u1:
+        print(json.dumps(u1_ess(a.u1_L, a.u1_beta, a.u1_batch, a.u1_train, a.u1_hmc_steps, a.u1_flow_steps,
+                                a.seed)), flush=True)
+        return
     if a.kernel_only:
         kernel_only(a.kernel_only, F64 if a.prec == "f64" else F32, a.L, a.reps)
         return

@@ -23,7 +23,7 @@ namespace {
 [[noreturn]] void usage(char const *msg)
 {
 	std::fprintf(stderr,
-	             "error: %s\nusage: mesh_hmc [--group su3] [--geom L0 L1 [L2 [L3]]] [--beta B] [--start random|cold]\n"
+	             "error: %s\nusage: mesh_hmc [--group su3|u1] [--geom L0 L1 [L2 [L3]]] [--beta B] [--start random|cold]\n"
 	             "                [--seed S] [--count N] [--scheme 2lf|2mn|4mn] [--epsilon E] [--substeps N]\n"
 	             "                [--precision 1|2] [--therm K] [--landau]\n",
 	             msg);
@@ -125,7 +125,12 @@ int main(int argc, char **argv)
 					    hmc.reset_observables();
 			    }
 			    hmc.print_summary();
-			    if (landau)
+			    if constexpr (G::Nc() == 1)
+			    {
+				    if (landau)
+					    throw std::runtime_error("--landau needs an SU(3) field");
+			    }
+			    else if (landau)
 			    {
 				    Landau<G> lan(hmc.U);
 				    lan.verbose = true;

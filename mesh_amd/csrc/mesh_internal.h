@@ -39,6 +39,7 @@ struct mesh_ctx
 	double *red_host = nullptr;
 	double *aux = nullptr; // small device buffer (integrator step tables)
 	size_t aux_cap = 0;    // doubles
+	std::vector<double> aux_host; // host copy of what aux holds
 	int xcd_remap = 2;     // block order of the stencil kernels (ordered_block)
 	int force_variant = 0;
 	int fuse_md = 1;        // fuse each momentum step with the next link step

@@ -97,10 +97,16 @@ int mesh_stage_doubles(mesh_ctx *ctx, const double *host, size_t n)
 		if (hipMalloc(&ctx->aux, n * sizeof(double)) != hipSuccess)
 			return mesh_fail(ctx, MESH_E_ALLOC, "cannot allocate the step-table buffer");
 		ctx->aux_cap = n;
+		ctx->aux_host.clear();
 	}
+	// a table equal to the one already staged is not copied again (a pageable
+	// host-to-device copy can stall the host until the stream drains)
+	if (ctx->aux_host.size() == n && std::memcmp(ctx->aux_host.data(), host, n * sizeof(double)) == 0)
+		return MESH_OK;
 	// the previous user of aux may still be in flight on the stream: the copy
 	// is ordered after it on the same stream
 	MESH_HIP(ctx, hipMemcpyAsync(ctx->aux, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+	ctx->aux_host.assign(host, host + n);
 	return MESH_OK;
 }
 

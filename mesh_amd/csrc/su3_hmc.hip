@@ -18,6 +18,16 @@
 
 using namespace mesh_dev;
 
+// waves per SIMD the force kernels are compiled for (0: the compiler's choice)
+#ifndef MESH_FORCE_WAVES
+#define MESH_FORCE_WAVES 0
+#endif
+#if MESH_FORCE_WAVES > 0
+#define FORCE_OCC __attribute__((amdgpu_waves_per_eu(MESH_FORCE_WAVES)))
+#else
+#define FORCE_OCC
+#endif
+
 namespace {
 
 constexpr int SB = 64; // sites per workgroup in the stencil kernels (one wavefront)
@@ -216,9 +226,10 @@ __device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Un
 // handles direction w of the workgroup's sites.  With Unext != nullptr the
 // following link step is fused in: Unext = exp(tl TA(P_new)) U (Unext != U,
 // since neighbouring threads still read U).
-template <class R>
-__global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
-                                                      int nb, int remap, R *__restrict__ Unext, R tl, int reunit_flag)
+template <class R, bool FUSE>
+__global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
+                                                                int nb, int remap, R *__restrict__ Unext, R tl,
+                                                                int reunit_flag)
 {
 	int lb = ordered_block(g, nb, SB, remap);
 	if (lb < 0)
@@ -240,7 +251,7 @@ __global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
 	store(P, mu, g, x, p);
-	if (Unext)
+	if constexpr (FUSE)
 		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 }
 
@@ -251,10 +262,10 @@ __global__ __launch_bounds__(256) void k_force_update(Geo g, const R *__restrict
 // LDS; every staple link displaced only along x2/x3 is then read from LDS and
 // only the x0/x1-displaced ones from global memory.  Wavefront w handles
 // direction w; the arithmetic is exactly that of k_force_update.
-template <class R, int L3>
-__global__ __launch_bounds__(256) void k_force_update_tile(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
-                                                           int nb, int remap, R *__restrict__ Unext, R tl,
-                                                           int reunit_flag)
+template <class R, int L3, bool FUSE>
+__global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, const R *__restrict__ U, R *__restrict__ P,
+                                                                     R kc, int nb, int remap, R *__restrict__ Unext,
+                                                                     R tl, int reunit_flag)
 {
 	constexpr int RW = 64 / L3; // rows per tile
 	constexpr int NR = RW + 2;  // with halo rows
@@ -337,7 +348,7 @@ __global__ __launch_bounds__(256) void k_force_update_tile(Geo g, const R *__res
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
 	store(P, mu, g, x, p);
-	if (Unext)
+	if constexpr (FUSE)
 		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 }
 
@@ -485,8 +496,13 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 		int gb = mesh_grid_blocks(ctx, nb);
 		auto launch = [&](auto tag, auto l3) {
 			using Rt = decltype(tag);
-			k_force_update_tile<Rt, decltype(l3)::value><<<gb, 256, 0, ctx->stream>>>(
-			    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, ctx->xcd_remap, (Rt *)Unext, (Rt)tl, reunit_flag);
+			constexpr int l3v = decltype(l3)::value;
+			if (fused)
+				k_force_update_tile<Rt, l3v, true><<<gb, 256, 0, ctx->stream>>>(
+				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, ctx->xcd_remap, (Rt *)Unext, (Rt)tl, reunit_flag);
+			else
+				k_force_update_tile<Rt, l3v, false><<<gb, 256, 0, ctx->stream>>>(
+				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, ctx->xcd_remap, nullptr, (Rt)0, 0);
 		};
 		using I16 = std::integral_constant<int, 16>;
 		using I32 = std::integral_constant<int, 32>;
@@ -512,13 +528,17 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 	{
 		int nb = nblk(geo.V, SB);
 		int gb = mesh_grid_blocks(ctx, nb);
-		if (prec == MESH_F64)
-			k_force_update<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, (double *)P, k, nb,
-			                                                     ctx->xcd_remap, (double *)Unext, tl, reunit_flag);
-		else
-			k_force_update<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, (float)k, nb,
-			                                                    ctx->xcd_remap, (float *)Unext, (float)tl,
-			                                                    reunit_flag);
+		auto launch = [&](auto tag) {
+			using Rt = decltype(tag);
+			if (fused)
+				k_force_update<Rt, true><<<gb, 256, 0, ctx->stream>>>(geo, (const Rt *)U, (Rt *)P, (Rt)k, nb,
+				                                                      ctx->xcd_remap, (Rt *)Unext, (Rt)tl,
+				                                                      reunit_flag);
+			else
+				k_force_update<Rt, false><<<gb, 256, 0, ctx->stream>>>(geo, (const Rt *)U, (Rt *)P, (Rt)k, nb,
+				                                                       ctx->xcd_remap, nullptr, (Rt)0, 0);
+		};
+		prec == MESH_F64 ? launch(0.0) : launch(0.0f);
 	}
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);

@@ -123,6 +123,32 @@ void runHmd(GaugeField<G> &U, GaugeField<G> &P, double beta, std::vector<double>
 			                                  deltas[i]));
 }
 
+namespace detail {
+// same quantities as Hmc::print_summary (src/gauge/hmc.h:88-100)
+inline void print_hmc_summary(std::vector<double> const &plaq, std::vector<double> const &dH,
+                              std::vector<double> const &acc, std::vector<double> const &time)
+{
+	auto mean = [](std::vector<double> const &v) {
+		return v.empty() ? 0.0 : std::accumulate(v.begin(), v.end(), 0.0) / v.size();
+	};
+	double mp = mean(plaq), var = 0;
+	for (double p : plaq)
+		var += (p - mp) * (p - mp);
+	size_t n = plaq.size();
+	var = n > 1 ? var / (n - 1) : 0;
+	double emdh = 0;
+	for (double d : dH)
+		emdh += std::exp(-d);
+	std::printf("========== HMC summary ==========\n");
+	std::printf("time per step = %.3f s (min = %.3f, max = %.3f)\n", mean(time),
+	            n ? *std::min_element(time.begin(), time.end()) : 0.0,
+	            n ? *std::max_element(time.begin(), time.end()) : 0.0);
+	std::printf("plaquette = %.4f +- %.4f\n", mp, n ? std::sqrt(var / n) : 0.0);
+	std::printf("acceptance = %.2f\n", mean(acc));
+	std::printf("<exp(-dH)> = %.4f\n", dH.empty() ? 0.0 : emdh / dH.size());
+}
+} // namespace detail
+
 // HMC state and observables (src/gauge/hmc.h:61-115)
 template <class G> class Hmc
 {
@@ -183,27 +209,102 @@ template <class G> class Hmc
 		time_history.push_back(dt);
 	}
 
-	// same quantities as Hmc::print_summary (src/gauge/hmc.h:88-100)
 	void print_summary() const
 	{
-		auto mean = [](std::vector<double> const &v) {
-			return v.empty() ? 0.0 : std::accumulate(v.begin(), v.end(), 0.0) / v.size();
-		};
-		double mp = mean(plaq_history), var = 0;
-		for (double p : plaq_history)
-			var += (p - mp) * (p - mp);
-		size_t n = plaq_history.size();
-		var = n > 1 ? var / (n - 1) : 0;
-		double emdh = 0;
-		for (double d : deltaH_history)
-			emdh += std::exp(-d);
-		std::printf("========== HMC summary ==========\n");
-		std::printf("time per step = %.3f s (min = %.3f, max = %.3f)\n", mean(time_history),
-		            n ? *std::min_element(time_history.begin(), time_history.end()) : 0.0,
-		            n ? *std::max_element(time_history.begin(), time_history.end()) : 0.0);
-		std::printf("plaquette = %.4f +- %.4f\n", mp, n ? std::sqrt(var / n) : 0.0);
-		std::printf("acceptance = %.2f\n", mean(accept_history));
-		std::printf("<exp(-dH)> = %.4f\n", n ? emdh / n : 0.0);
+		detail::print_hmc_summary(plaq_history, deltaH_history, accept_history, time_history);
+	}
+};
+
+// U(1) links as angles on a 2-d lattice: theta[mu][x], FP32 (the reference's
+// GaugeField<U1<T>> of complex<T> links exp(i theta))
+class U1Field
+{
+	Grid const *g_;
+	void *p_ = nullptr;
+
+  public:
+	explicit U1Field(Grid const &g) : g_(&g)
+	{
+		if (g.ndim() != 2)
+			throw std::runtime_error("U(1) fields are implemented on 2-d lattices");
+		ctx().check(mesh_malloc(ctx().get(), &p_, bytes()));
+	}
+	~U1Field()
+	{
+		if (p_)
+			mesh_free(ctx().get(), p_);
+	}
+	U1Field(U1Field const &) = delete;
+	U1Field &operator=(U1Field const &) = delete;
+	Grid const &grid() const { return *g_; }
+	size_t bytes() const { return 2 * g_->size() * sizeof(float); }
+	float *data() const { return (float *)p_; }
+	// theta = 0: the unit configuration
+	void fill_unit() { ctx().check(mesh_memclear(ctx().get(), p_, bytes())); }
+	std::vector<float> download() const
+	{
+		std::vector<float> h(2 * g_->size());
+		ctx().check(mesh_memcpy_to_host(ctx().get(), h.data(), p_, bytes()));
+		return h;
+	}
+};
+
+// HMC for U(1) on a 2-d lattice (Hmc<U1<T>>, src/gauge/hmc.cpp:109-110): the
+// whole trajectory runs in one kernel (mesh_u1_hmc, batch of one)
+template <class T> class Hmc<U1<T>>
+{
+  public:
+	Grid const &g;
+	U1Field U;
+	uint64_t seed = 0;
+	uint32_t traj = 0;
+	std::vector<double> plaq_history, deltaH_history, accept_history, time_history;
+
+  private:
+	double *out_ = nullptr; // device: 4 HMC results + 4 observables
+
+  public:
+	explicit Hmc(Grid const &g_) : g(g_), U(g_)
+	{
+		ctx().check(mesh_malloc(ctx().get(), (void **)&out_, 8 * sizeof(double)));
+	}
+	~Hmc()
+	{
+		if (out_)
+			mesh_free(ctx().get(), out_);
+	}
+	Hmc(Hmc const &) = delete;
+	Hmc &operator=(Hmc const &) = delete;
+
+	void reset_observables()
+	{
+		plaq_history.clear();
+		deltaH_history.clear();
+		accept_history.clear();
+		time_history.clear();
+	}
+	void randomizeGaugeField() { ctx().check(mesh_u1_random(ctx().get(), g.c(), 1, U.data(), seed)); }
+
+	void runHmcUpdate(double beta, std::vector<double> const &deltas, bool metropolis = true)
+	{
+		auto t0 = std::chrono::steady_clock::now();
+		Context &c = ctx();
+		c.check(mesh_u1_hmc(c.get(), g.c(), 1, U.data(), deltas.data(), (int)deltas.size(), beta, seed, traj,
+		                    metropolis ? 1 : 0, out_));
+		c.check(mesh_u1_observables(c.get(), g.c(), 1, U.data(), out_ + 4));
+		double h[8];
+		c.check(mesh_memcpy_to_host(c.get(), h, out_, sizeof h));
+		++traj;
+		double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		accept_history.push_back(h[1]);
+		plaq_history.push_back(h[6] / (double)g.size()); // mean cos theta_P = Re tr P / Nc
+		deltaH_history.push_back(h[0]);
+		time_history.push_back(dt);
+	}
+
+	void print_summary() const
+	{
+		detail::print_hmc_summary(plaq_history, deltaH_history, accept_history, time_history);
 	}
 };
 
@@ -274,14 +375,24 @@ template <class G> struct Landau
 
 template <typename F> void dispatchByGroup(F f, std::string const &group, int precision)
 {
-	if (group != "su3")
-		throw std::runtime_error("unknown gauge group '" + group + "'");
-	if (precision == 1)
-		f.template operator()<SU3<float>>();
-	else if (precision == 2)
-		f.template operator()<SU3<double>>();
-	else
+	if (precision != 1 && precision != 2)
 		throw std::runtime_error("invlid precision level '" + std::to_string(precision) + "'");
+	if (group == "su3")
+	{
+		if (precision == 1)
+			f.template operator()<SU3<float>>();
+		else
+			f.template operator()<SU3<double>>();
+	}
+	else if (group == "u1")
+	{
+		if (precision == 1)
+			f.template operator()<U1<float>>();
+		else
+			f.template operator()<U1<double>>();
+	}
+	else
+		throw std::runtime_error("unknown gauge group '" + group + "'");
 }
 
 } // namespace mesh

@@ -165,6 +165,19 @@ template <class T> struct SU3
 	static constexpr int dim() { return 8; }
 };
 
+// U(1): the reference's complex<T> link exp(i theta) is stored as its angle,
+// in FP32 whatever T is (energies, sums and the Metropolis test are FP64);
+// 2-d lattices only
+template <class T> struct U1
+{
+	using real = T;
+	using element = float; // the angle theta in [-pi, pi)
+	static constexpr int prec = MESH_F32;
+	static std::string name() { return "u1"; }
+	static constexpr int Nc() { return 1; }
+	static constexpr int dim() { return 1; }
+};
+
 template <class G> struct GaugeTraits
 {
 	static std::string name() { return G::name(); }

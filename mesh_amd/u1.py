@@ -201,37 +201,46 @@ def run_chains(ens, beta, deltas, n_steps, seed, flow=None, n_therm=0, record_ev
     with a flow proposal (1:1), exact accept/reject against log q computed
     by the inverse flow for the state HMC produced.
 
-    Returns dict(Q (chains x steps), plaq, acc_hmc, acc_flow, seconds)."""
+    Per-step results stay on the device until the end, so the timed loop
+    only enqueues launches.  Returns dict(Q (chains x records), plaq,
+    acc_hmc, acc_flow, seconds)."""
     ctx = ens.ctx
-    Qs, Ps = [], []
-    acc_h, acc_f = [], []
+    B = ens.batch
+    d = np.ascontiguousarray(deltas, np.float64)
+    dptr = d.ctypes.data_as(C.POINTER(C.c_double))
+    n_tot = n_therm + n_steps
+    nrec = (n_steps + record_every - 1) // record_every
+    obs = Buffer(max(1, nrec) * 4 * B, np.float64, ctx)
+    hout = Buffer(n_tot * 4 * B, np.float64, ctx)
     if flow is not None:
-        prop = Buffer(ens.batch * 2 * ens.V, np.float32, ctx)
-        state = Buffer(2 * ens.batch, np.float64, ctx)
-        pout = Buffer(2 * ens.batch, np.float64, ctx)
-        accb = Buffer(ens.batch, np.float64, ctx)
+        prop = Buffer(B * 2 * ens.V, np.float32, ctx)
+        state = Buffer(2 * B, np.float64, ctx)
+        pout = Buffer(2 * B, np.float64, ctx)
+        accb = Buffer(n_tot * B, np.float64, ctx)
     ctx.synchronize()
-    t0 = None
-    for it in range(n_therm + n_steps):
+    t0 = time.perf_counter()
+    for it in range(n_tot):
         if it == n_therm:
             ctx.synchronize()
             t0 = time.perf_counter()
-        r = ens.hmc(beta, deltas, seed, it)
-        acc_h.append(r["acc"].mean())
+        ctx.call("mesh_u1_hmc", ens.grid.ref, B, ens.theta.ptr, dptr, len(d), _d(beta), C.c_uint64(seed),
+                 C.c_uint32(it), 1, hout.ptr.value + it * 4 * B * 8)
         if flow is not None:
             flow.logq(ens, out=state)
             flow.sample(ens, seed + 7919, it, theta_out=prop, out=pout)
-            ctx.call("mesh_flow_mh", ens.grid.ref, ens.batch, ens.theta.ptr, state.ptr, prop.ptr, pout.ptr,
-                     _d(beta), C.c_uint64(seed), C.c_uint32(it), accb.ptr)
-            acc_f.append(accb.download().mean())
+            ctx.call("mesh_flow_mh", ens.grid.ref, B, ens.theta.ptr, state.ptr, prop.ptr, pout.ptr,
+                     _d(beta), C.c_uint64(seed), C.c_uint32(it), accb.ptr.value + it * B * 8)
         if it >= n_therm and (it - n_therm) % record_every == 0:
-            o = ens.observables()
-            Qs.append(o["Q"])
-            Ps.append(o["cos"])
+            k = (it - n_therm) // record_every
+            ctx.call("mesh_u1_observables", ens.grid.ref, B, ens.theta.ptr, obs.ptr.value + k * 4 * B * 8)
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    return dict(Q=np.array(Qs).T, plaq=np.array(Ps).T, acc_hmc=float(np.mean(acc_h[n_therm:])),
-                acc_flow=float(np.mean(acc_f[n_therm:])) if acc_f else None, seconds=dt)
+    o = obs.download().reshape(-1, B, 4)[:nrec]
+    h = hout.download().reshape(n_tot, B, 4)
+    return dict(Q=o[:, :, 1].T.copy(), plaq=(o[:, :, 2].T / ens.V).copy(),
+                acc_hmc=float(h[n_therm:, :, 1].mean()),
+                acc_flow=float(accb.download().reshape(n_tot, B)[n_therm:].mean()) if flow is not None else None,
+                seconds=dt)
 
 
 def ess_per_second(res, obs="Q"):

@@ -1,9 +1,12 @@
 """The C++ host layer (mesh_amd/host/mesh/*.h) driven by the compiled
 programs: the restated reference test and the mesh_hmc driver."""
 import os
+import re
 import subprocess
 
 import pytest
+
+from oracle import u1_oracle as U
 
 pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mesh_amd", "bin")
@@ -25,3 +28,21 @@ def test_cpp_hmc_driver(precision):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "HMC summary" in r.stdout and "landau:" in r.stdout and "time in force_update" in r.stdout
+
+
+@pytest.mark.parametrize("precision", ["2", "1"])
+def test_cpp_hmc_driver_u1(precision):
+    """--group u1 on a 16x16 lattice (Hmc<U1<T>>): the mean plaquette of one
+    chain against the exact torus <cos theta_P> at beta = 2."""
+    r = subprocess.run([os.path.join(BIN, "mesh_hmc"), "--group", "u1", "--geom", "16", "16", "--beta", "2",
+                        "--count", "400", "--therm", "50", "--scheme", "2lf", "--epsilon", "0.5", "--substeps",
+                        "10", "--seed", "4", "--precision", precision],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"plaquette = ([0-9.]+) \+- ([0-9.]+)", r.stdout)
+    acc = float(re.search(r"acceptance = ([0-9.]+)", r.stdout).group(1))
+    p, err = float(m.group(1)), float(m.group(2))
+    assert acc > 0.7
+    # the printed error ignores autocorrelation: allow a generous multiple
+    assert abs(p - U.exact_mean_cos(2.0, 256)) < 10 * err + 2e-3, (p, err)
