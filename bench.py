@@ -59,6 +59,20 @@ def dominant(kern):
     return max(KERNELS, key=lambda k: (kern[k]["avg_ms"] or 0.0) * kern[k]["launches"])
 
 
+def measured_traffic(prec_name, L):
+    """HBM bytes per launch of the fused MD step from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/force_link_L<L>_traffic.md), or None"""
+    path = os.path.join(ROOT, "profiles", f"force_link_L{L}_traffic.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == prec_name:
+                return int(f[2])
+    except OSError:
+        pass
+    return None
+
+
 def run_gpu(prec, L, steps, warmup, therm, seed, scheme, eps, substeps):
     import numpy as np
 
@@ -124,7 +138,7 @@ def cpu_baseline(L, seed, scheme, eps, substeps):
                        f"{substeps} substeps) from the Haar start, CPU oracle with OpenMP; {dt:.2f} s")
 
 
-def kernel_only(name, prec, L, reps):
+def kernel_only(name, prec, L, reps, variant=0):
     """Profiling mode: `reps` launches of one kernel on a seeded L^4 field,
     preceded by `reps` plain copies of one gauge field (a transfer of known
     size, used to calibrate rocprof byte counters).  Prints one JSON line."""
@@ -132,6 +146,7 @@ def kernel_only(name, prec, L, reps):
     from mesh_amd import gauge
 
     ctx = mesh_amd.default_context()
+    ctx.set_option("force_variant", variant)
     g = mesh_amd.Grid((L,) * 4)
     U = mesh_amd.GaugeField(g, prec, ctx)
     U2 = mesh_amd.GaugeField(g, prec, ctx)
@@ -163,7 +178,8 @@ def kernel_only(name, prec, L, reps):
             raise SystemExit(f"unknown kernel {name}")
     ctx.synchronize()
     ms, n = ctx.profile_read(name)
-    print(json.dumps({"kernel": name, "prec": "f64" if prec == 0 else "f32", "L": L, "avg_ms": ms / n,
+    print(json.dumps({"kernel": name, "prec": "f64" if prec == 0 else "f32", "L": L, "variant": variant,
+                      "avg_ms": ms / n,
                       "launches": n, "copy_bytes_per_launch": 2 * U.bytes}), flush=True)
 
 
@@ -217,6 +233,7 @@ def main():
                     help="profiling mode: run one kernel (force_link, force_update, link_update, delta_h, "
                          "landau_or)")
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--force-variant", type=int, default=0, help="kernel-only mode: force kernel variant 0/1/2")
     ap.add_argument("--prec", default="f64")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -250,7 +267,7 @@ def main():
                                 a.seed)), flush=True)
         return
     if a.kernel_only:
-        kernel_only(a.kernel_only, F64 if a.prec == "f64" else F32, a.L, a.reps)
+        kernel_only(a.kernel_only, F64 if a.prec == "f64" else F32, a.L, a.reps, a.force_variant)
         return
 
     r64 = run_gpu(F64, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
@@ -278,7 +295,8 @@ def main():
                                f"substeps={a.substeps} (tau=1), FP64",
                    "L": a.L, "beta": 6.0, "thermalisation": a.therm},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("f64", a.L) if dom == "force_link" else None,
                      "kernel": dom, "avg_ms": fk["avg_ms"],
                      "algorithmic_bytes_per_launch": alg},
         "trajectory_roofline_frac": value * trajectory_bytes_per_site(r64["kernels"], 8, a.steps) / 8e12,

@@ -256,6 +256,26 @@ struct FlowShared
 	double red[4][4];
 };
 
+// MLP outputs of tiles it .. it+15 (four per wavefront) into Ys[tile][row*16+k]
+__device__ void mlp_round(FGeo const &g, const float *th, int mu, int a, int it, int NT, FlowShared &sh,
+                          float (*Ys)[256])
+{
+	const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+	Tile &T = sh.tile[w];
+	for (int q = 0; q < 4; ++q)
+	{
+		const int tl = q * 4 + w, tix = it + tl;
+		if (tix >= NT)
+			break;
+		features(g, th, mu, a, tix * 16, T);
+		wave_sync();
+		mlp_forward(sh.lw, T);
+		for (int e = l; e < 256; e += 64)
+			Ys[tl][e] = T.y[e];
+		wave_sync();
+	}
+}
+
 // z ~ uniform -> U.  theta_layers (optional): input of every layer, for the
 // backward pass.  out[c*2 + {0,1}] = {log q(U), S(U)/beta = sum(1 - cos)}.
 __global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, const float *__restrict__ params, uint64_t seed,
@@ -264,6 +284,7 @@ __global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, const float *_
 {
 	extern __shared__ float th[]; // 2V
 	__shared__ FlowShared sh;
+	__shared__ float Ys[16][256];
 	const int c = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
 	for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
 	{
@@ -284,24 +305,18 @@ __global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, const float *_
 				theta_layers[((long)c * NL + layer) * 2 * g.V + i] = th[i];
 		load_weights(sh.lw, params + layer * NP_LAYER);
 		__syncthreads();
-		for (int it = 0; it < (NT + 3) / 4; ++it)
+		// within a layer the MLP inputs (frozen plaquettes) never change, so
+		// all MLPs of a round run first and then every thread moves one link
+		for (int it = 0; it < NT; it += 16)
 		{
-			const int tix = it * 4 + w;
-			Tile &T = sh.tile[w];
-			if (tix < NT)
-			{
-				features(g, th, mu, a, tix * 16, T);
-				wave_sync();
-			}
+			mlp_round(g, th, mu, a, it, NT, sh, Ys);
 			__syncthreads();
-			if (tix < NT)
-				mlp_forward(sh.lw, T);
-			__syncthreads();
-			if (tix < NT && l < 16 && tix * 16 + l < g.NA)
+			const int r = it * 16 + threadIdx.x;
+			if (r < g.NA)
 			{
-				int r = tix * 16 + l, y0, y1;
+				int y0, y1;
 				active_corner(g, mu, a, r, y0, y1);
-				NCP p = ncp_params(T.y + l * 16);
+				NCP p = ncp_params(&Ys[threadIdx.x >> 4][(threadIdx.x & 15) * 16]);
 				float tp = plaqf(g, th, y0, y1);
 				float u = wrapf(tp - p.t), G, Gp;
 				ncp_eval(p, u, G, Gp);
@@ -336,13 +351,14 @@ __global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, const float *_
 		theta_out[(long)c * 2 * g.V + i] = th[i];
 }
 
-// U -> z and log q(U), layers in reverse with g^-1 by bisection.
+// U -> z and log q(U), layers in reverse with g^-1 by safeguarded Newton.
 // out[c*2 + {0,1}] = {log q(U), S(U)/beta}
 __global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, const float *__restrict__ params,
                                                           const float *__restrict__ theta_in, float *z_out, double *out)
 {
 	extern __shared__ float th[];
 	__shared__ FlowShared sh;
+	__shared__ float Ys[16][256];
 	const int c = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
 	for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
 		th[i] = theta_in[(long)c * 2 * g.V + i];
@@ -357,38 +373,43 @@ __global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, const float *_
 		__syncthreads();
 		load_weights(sh.lw, params + layer * NP_LAYER);
 		__syncthreads();
-		for (int it = 0; it < (NT + 3) / 4; ++it)
+		for (int it = 0; it < NT; it += 16)
 		{
-			const int tix = it * 4 + w;
-			Tile &T = sh.tile[w];
-			if (tix < NT)
-			{
-				features(g, th, mu, a, tix * 16, T);
-				wave_sync();
-			}
+			mlp_round(g, th, mu, a, it, NT, sh, Ys);
 			__syncthreads();
-			if (tix < NT)
-				mlp_forward(sh.lw, T);
-			__syncthreads();
-			if (tix < NT && l < 16 && tix * 16 + l < g.NA)
+			const int r = it * 16 + threadIdx.x;
+			if (r < g.NA)
 			{
-				int r = tix * 16 + l, y0, y1;
+				int y0, y1;
 				active_corner(g, mu, a, r, y0, y1);
-				NCP p = ncp_params(T.y + l * 16);
+				NCP p = ncp_params(&Ys[threadIdx.x >> 4][(threadIdx.x & 15) * 16]);
 				float tpn = plaqf(g, th, y0, y1);
 				float target = wrapf(tpn - p.t);
-				// g is increasing on [-pi, pi] with g(+-pi) = +-pi
-				float lo = -PI_F, hi = PI_F, G, Gp;
-				for (int b = 0; b < 32; ++b)
+				// g is increasing on [-pi, pi] with g(+-pi) = +-pi: Newton steps
+				// inside a bracket that every evaluation narrows, bisection when a
+				// step leaves it (at most 40 evaluations; typically 3-5)
+				float lo = -PI_F, hi = PI_F, u = target, G, Gp;
+				for (int b = 0; b < 40; ++b)
 				{
-					float mid = 0.5f * (lo + hi);
-					ncp_eval(p, mid, G, Gp);
-					if (G < target)
-						lo = mid;
+					ncp_eval(p, u, G, Gp);
+					float f = G - target;
+					if (f == 0.f)
+						break;
+					if (f < 0.f)
+						lo = u;
 					else
-						hi = mid;
+						hi = u;
+					float un = u - f / Gp;
+					if (un > lo && un < hi)
+					{
+						bool done = fabsf(un - u) <= 2e-7f;
+						u = un;
+						if (done)
+							break;
+					}
+					else
+						u = 0.5f * (lo + hi);
 				}
-				float u = 0.5f * (lo + hi);
 				ncp_eval(p, u, G, Gp);
 				int ul = updated_link(g, mu, y0, y1);
 				th[ul] = wrapf(th[ul] - tpn + u + p.t);
