@@ -90,10 +90,16 @@ int mesh_synchronize(mesh_ctx *ctx);
 void *mesh_stream(mesh_ctx *ctx);
 /* Kernel launch tuning (no effect on results):
  *   "xcd_remap" = workgroup order of the stencil kernels: 0 linear, 1 each
- *       XCD takes a contiguous site range, 2 (default) each XCD takes a range
- *       of x2/x3 row-blocks and walks x0 fastest, then x1 (4-d lattices);
- *   "force_variant" = 0 (default) wavefront per direction, 1 thread per
- *       site, 2 LDS-staged x2/x3 tile;
+ *       XCD takes a contiguous site range, 2 each XCD takes a range of x2/x3
+ *       row-blocks and walks x0 fastest, then x1; 3 (default) each XCD takes
+ *       a slab of L0/8 x0-slices and walks it x0 fastest, then "rb_tile"
+ *       row-blocks, then x1, then the next tile (4-d lattices with 8 | L0;
+ *       otherwise order 2 or 1 applies);
+ *   "rb_tile" = row-blocks per tile of order 3 (default 4; used when it
+ *       divides the row-blocks of an x2/x3 plane);
+ *   "force_variant" = -1 (default) per precision: 0 for FP64, 2 for FP32;
+ *       0 wavefront per direction, 1 thread per site, 2 LDS-staged x2/x3
+ *       tile;
  *   "fuse_md" = 1 (default): in mesh_su3_hmc_trajectory each momentum step
  *       and the link step after it run as one kernel (the thread that
  *       updates P_mu(x) writes exp(t P_mu(x)) U_mu(x) into a second field);
@@ -275,7 +281,8 @@ int mesh_u1_hmc(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, cons
  * computes from the frozen plaquettes around the active one.  Lattice extents
  * divisible by 4, V <= 4096.  params: mesh_flow_num_params() floats (device). */
 int mesh_flow_num_params(void);
-/* deterministic initial weights (identity transform) */
+/* deterministic initial weights (near the identity transform: last layer
+ * 1e-2 of the fan-in scale, bias of the identity) */
 int mesh_flow_init(mesh_ctx *ctx, float *params, uint64_t seed);
 /* batch samples U = f(z), z uniform from Philox(seed, step):
  * out_dev[2c + {0,1}] = {log q(U), S(U)/beta}; theta_layers may be NULL */
@@ -295,9 +302,39 @@ int mesh_flow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, const float
 int mesh_flow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, float *v, int t, double lr);
 /* independence Metropolis-Hastings: per chain accept the proposal with
  * probability min(1, exp[(-beta S' - log q') - (-beta S - log q)]); state and
- * prop hold {log q, S/beta} per chain; acc_dev[c] = 1 on acceptance. */
+ * prop hold {log q, S/beta} per chain; acc_dev[c] = 1 on acceptance.  Any 2-d
+ * lattice (proposals of either flow). */
 int mesh_flow_mh(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, double *state, const float *prop_theta,
                  const double *prop, double beta, uint64_t seed, uint32_t step, double *acc_dev);
+
+/* ---- U(1) plaquette flow (no counterpart in the reference) -----------------
+ *
+ * On a periodic 2-d lattice the plaquette angles are independent except for
+ * one constraint (their sum is 0 mod 2 pi).  The proposal draws every
+ * plaquette but the last (x0, x1) = (L0-1, L1-1) as theta_P = h(u), u uniform,
+ * with a learned circle map h (mixture of 8 non-compact projections,
+ * mesh_pflow_num_params() parameters), builds the links in a maximal-tree
+ * gauge from them and two uniform holonomies, and applies a uniform random
+ * gauge transformation.  The map is unimodular, so
+ *   log q(U) = sum_{P != last} log q1(theta_P(U)) - (V + 1) log 2 pi,
+ * q1 the density of h(u).  Any 2-d extents >= 2, V <= 4096. */
+int mesh_pflow_num_params(void);
+/* initial parameters: log s_k spread over [-3, 1], equal weights, t = 0 */
+int mesh_pflow_init(mesh_ctx *ctx, float *params);
+/* batch proposals (links) from Philox(seed, step) */
+int mesh_pflow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, uint64_t seed, uint32_t step,
+                      float *theta_out);
+/* out_dev[2c + {0,1}] = {log q(U), S(U)/beta} of configurations theta */
+int mesh_pflow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, const float *theta,
+                    double *out_dev);
+/* bytes of the workspace mesh_pflow_gradient needs */
+size_t mesh_pflow_workspace_bytes(const mesh_grid *g, int batch);
+/* reverse-KL gradient grad = (1/batch) sum_c d/dparams [log q + beta S] over a
+ * fresh batch of plaquette draws (Philox(seed, step)); out_dev as logq */
+int mesh_pflow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, uint64_t seed,
+                        uint32_t step, double beta, void *workspace, float *grad, double *out_dev);
+/* Adam step t >= 1 (moments zero-initialised by the caller) */
+int mesh_pflow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, float *v, int t, double lr);
 
 #ifdef __cplusplus
 }

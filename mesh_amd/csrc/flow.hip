@@ -705,11 +705,11 @@ __global__ void k_flow_reduce(const float *__restrict__ partial, int batch, floa
 }
 
 // Adam (Kingma & Ba 2015)
-__global__ void k_adam(float *__restrict__ p, const float *__restrict__ gr, float *__restrict__ m,
+__global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ gr, float *__restrict__ m,
                        float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= NL * NP_LAYER)
+	if (i >= n)
 		return;
 	float gi = gr[i];
 	float mi = fmaf(b1, m[i], (1.f - b1) * gi);
@@ -719,8 +719,9 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ gr, floa
 	p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
-// initial weights: W ~ N(0, 1/fan_in) for the hidden layers, last layer zero
-// with the bias of the identity transform (s_k = 1, w uniform, t = 0)
+// initial weights: W ~ N(0, 1/fan_in) for the hidden layers, the last layer
+// 1e-2 of that (identical components would receive identical gradients for
+// ever) with the bias of the identity transform (s_k = 1, w uniform, t = 0)
 __global__ void k_flow_init(float *p, uint64_t seed)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -728,18 +729,20 @@ __global__ void k_flow_init(float *p, uint64_t seed)
 		return;
 	int o = i % NP_LAYER;
 	float v = 0.f;
-	double fan = 0.0;
+	double sd = 0.0;
 	if (o >= OW1 && o < OB1)
-		fan = NI;
+		sd = 1.0 / sqrt((double)NI);
 	else if (o >= OW2 && o < OB2)
-		fan = NH;
-	if (fan > 0)
+		sd = 1.0 / sqrt((double)NH);
+	else if (o >= OW3 && o < OB3)
+		sd = 1e-2 / sqrt((double)NH); // small: breaks the symmetry of the NK components
+	if (sd > 0)
 	{
 		u4 ctr = {(uint32_t)i, 0u, 0u, TAG_FLOWI};
 		u4 r = philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
 		double z0, z1;
 		normal_pair(r, &z0, &z1);
-		v = (float)(z0 / sqrt(fan));
+		v = (float)(z0 * sd);
 	}
 	if (o == OB3 + 2 * NK)
 		v = 1.f;
@@ -889,9 +892,15 @@ int mesh_flow_mh(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, dou
                  const double *prop, double beta, uint64_t seed, uint32_t step, double *acc_dev)
 {
 	MESH_CHECK_CTX(ctx);
+	// any 2-d lattice: the step only copies configurations (proposals of
+	// either flow)
+	if (!g || g->nd != 2 || g->dims[0] < 2 || g->dims[1] < 2 || batch < 1)
+		return mesh_fail(ctx, MESH_E_SIZE, "flow_mh needs a 2-d lattice and batch >= 1");
 	FGeo fg;
-	if (int rc = flow_setup(ctx, g, batch, &fg))
-		return rc;
+	fg.L0 = g->dims[0];
+	fg.L1 = g->dims[1];
+	fg.V = g->dims[0] * g->dims[1];
+	fg.NA = fg.V / 4;
 	if (!theta || !state || !prop_theta || !prop || !acc_dev)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to flow_mh");
 	k_flow_mh<<<batch, THREADS, 0, ctx->stream>>>(fg, theta, state, prop_theta, prop, beta, seed, step, acc_dev);
@@ -904,13 +913,17 @@ int mesh_flow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, fl
 	MESH_CHECK_CTX(ctx);
 	if (!params || !grad || !m || !v || t < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad flow_adam arguments");
-	const double b1 = 0.9, b2 = 0.999;
-	float c1 = (float)(1.0 / (1.0 - std::pow(b1, t))), c2 = (float)(1.0 / (1.0 - std::pow(b2, t)));
-	int n = NL * NP_LAYER;
-	k_adam<<<(n + 255) / 256, 256, 0, ctx->stream>>>(params, grad, m, v, (float)lr, (float)b1, (float)b2, 1e-8f, c1,
-	                                                 c2);
-	MESH_LAUNCHED(ctx);
-	return MESH_OK;
+	return mesh_launch_adam(ctx, NL * NP_LAYER, params, grad, m, v, t, lr);
 }
 
 } // extern "C"
+
+int mesh_launch_adam(mesh_ctx *ctx, int n, float *params, const float *grad, float *m, float *v, int t, double lr)
+{
+	const double b1 = 0.9, b2 = 0.999;
+	float c1 = (float)(1.0 / (1.0 - std::pow(b1, t))), c2 = (float)(1.0 / (1.0 - std::pow(b2, t)));
+	k_adam<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, params, grad, m, v, (float)lr, (float)b1, (float)b2, 1e-8f,
+	                                                 c1, c2);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}

@@ -40,8 +40,9 @@ struct mesh_ctx
 	double *aux = nullptr; // small device buffer (integrator step tables)
 	size_t aux_cap = 0;    // doubles
 	std::vector<double> aux_host; // host copy of what aux holds
-	int xcd_remap = 2;     // block order of the stencil kernels (ordered_block)
-	int force_variant = 0;
+	int xcd_remap = 3;     // block order of the stencil kernels (ordered_block)
+	int rb_tile = 4;       // row-blocks per traversal tile of order 3
+	int force_variant = -1; // -1: per precision (FP64 wavefront, FP32 LDS tile)
 	int fuse_md = 1;        // fuse each momentum step with the next link step
 	void *md_spare = nullptr; // third gauge field for the fused trajectory
 	size_t md_spare_bytes = 0;
@@ -96,12 +97,16 @@ int mesh_check_prec(mesh_ctx *ctx, int prec);
 int mesh_reserve_partials(mesh_ctx *ctx, size_t n);
 // copies n host doubles into ctx->aux (grown as needed), on the ctx stream
 int mesh_stage_doubles(mesh_ctx *ctx, const double *host, size_t n);
+// one Adam step (Kingma & Ba) over n parameters, t >= 1
+int mesh_launch_adam(mesh_ctx *ctx, int n, float *params, const float *grad, float *m, float *v, int t, double lr);
 // sums nq quantities of nb partials each into host out[nq] (synchronises)
 int mesh_finish_reduction(mesh_ctx *ctx, int nb, int nq, double *out);
 
 // launch shape of the site-blocked kernels: nb logical blocks; with XCD
 // remapping the grid is padded to a multiple of 8
 inline int mesh_grid_blocks(mesh_ctx *ctx, int nb) { return ctx->xcd_remap ? ((nb + 7) / 8) * 8 : nb; }
+// block-order argument of the stencil kernels: order | (tile << 4)
+inline int mesh_order(mesh_ctx *ctx) { return ctx->xcd_remap | (ctx->rb_tile << 4); }
 
 namespace mesh_dev {
 
@@ -118,26 +123,48 @@ __device__ __forceinline__ int logical_block(int nb, int remap)
 	return (b & 7) * chunk + (b >> 3);
 }
 
-// Block order of the stencil kernels (sbk consecutive sites per workgroup):
+// Block order of the stencil kernels (sbk consecutive sites per workgroup,
+// i.e. one "row-block" rb of the x2/x3 plane), argument order | (T << 4):
 //   0: linear;  1: XCD-contiguous linear;
 //   2: cache order for 4-d lattices with sbk | L2*L3 -- each XCD walks its own
-//      range of x2/x3 row-blocks and, within it, x0 fastest then x1, so the
+//      range of row-blocks and, within it, x0 fastest then x1, so the
 //      x0-neighbours of a workgroup were staged by the previous one and the
-//      x1-neighbours L0 workgroups earlier, all in the same L2.
+//      x1-neighbours L0 workgroups earlier, all in the same L2;
+//   3: slab order for 4-d lattices with 8 | L0 and T | (L2*L3/sbk) -- XCD k
+//      owns the slab x0 in [k L0/8, (k+1) L0/8) and walks it x0 fastest, then
+//      T row-blocks, then x1, then the next tile of T row-blocks, so the x1-
+//      and rb-neighbours of a workgroup are at most (L0/8) T workgroups back
+//      and only the two x0 faces of the slab come from another XCD.
 // Returns the memory-order block index (-1: padding workgroup).  The order
 // only permutes workgroups; every site's arithmetic is unchanged.
-__device__ __forceinline__ int ordered_block(Geo const &g, int nb, int sbk, int order)
+__device__ __forceinline__ int ordered_block(Geo const &g, int nb, int sbk, int arg)
 {
+	const int order = arg & 15, T = arg >> 4;
 	int b = blockIdx.x;
 	if (order == 0)
 		return b < nb ? b : -1;
+	const bool planes = g.nd == 4 && (g.L[2] * g.L[3]) % sbk == 0;
+	const int nb2 = planes ? (g.L[2] * g.L[3]) / sbk : 1;
+	if (order == 3 && planes && (g.L[0] & 7) == 0 && T > 0 && nb2 % T == 0)
+	{
+		const int X = g.L[0] >> 3;
+		const int j = b >> 3;
+		if (j >= nb >> 3)
+			return -1;
+		int x0 = (b & 7) * X + j % X;
+		int t = j / X;
+		int rb = t % T;
+		t /= T;
+		int x1 = t % g.L[1];
+		rb += (t / g.L[1]) * T;
+		return (x0 * g.L[1] + x1) * nb2 + rb;
+	}
 	int chunk = (nb + 7) >> 3;
 	int o = (b & 7) * chunk + (b >> 3);
 	if (o >= nb)
 		return -1;
-	if (order == 1 || g.nd != 4 || (g.L[2] * g.L[3]) % sbk != 0)
+	if (order == 1 || !planes)
 		return o;
-	int nb2 = (g.L[2] * g.L[3]) / sbk;
 	int x0 = o % g.L[0];
 	int t = o / g.L[0];
 	int x1 = t % g.L[1];

@@ -312,9 +312,11 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value)
 	MESH_CHECK_CTX(ctx);
 	if (!name)
 		return mesh_fail(ctx, MESH_E_ARG, "option name is NULL");
-	if (!std::strcmp(name, "xcd_remap") && value >= 0 && value <= 2)
+	if (!std::strcmp(name, "xcd_remap") && value >= 0 && value <= 3)
 		ctx->xcd_remap = (int)value;
-	else if (!std::strcmp(name, "force_variant") && value >= 0 && value <= 2)
+	else if (!std::strcmp(name, "rb_tile") && value >= 1 && value <= 64)
+		ctx->rb_tile = (int)value;
+	else if (!std::strcmp(name, "force_variant") && value >= -1 && value <= 2)
 		ctx->force_variant = (int)value;
 	else if (!std::strcmp(name, "fuse_md"))
 		ctx->fuse_md = value != 0;

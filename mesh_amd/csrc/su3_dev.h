@@ -409,6 +409,35 @@ template <class R> MD void store(R *__restrict__ base, int f, Geo const &g, long
 	}
 }
 
+// The same with non-temporal hints, for fields a kernel reads or writes once
+// (momenta, the next link field): they should not evict the links that
+// neighbouring workgroups still read through the L2.
+template <class R> using nt2_t = R __attribute__((ext_vector_type(2)));
+template <class R> MD m3<R> load_stream(const R *__restrict__ base, int f, Geo const &g, long x)
+{
+	const nt2_t<R> *p = reinterpret_cast<const nt2_t<R> *>(base) + site_off(g, f, x);
+	const long ks = plane_step(g);
+	m3<R> m;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		nt2_t<R> v = __builtin_nontemporal_load(p + k * ks);
+		m.e[k] = {v.x, v.y};
+	}
+	return m;
+}
+template <class R> MD void store_stream(R *__restrict__ base, int f, Geo const &g, long x, m3<R> const &m)
+{
+	nt2_t<R> *p = reinterpret_cast<nt2_t<R> *>(base) + site_off(g, f, x);
+	const long ks = plane_step(g);
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		nt2_t<R> v = {m.e[k].re, m.e[k].im};
+		__builtin_nontemporal_store(v, p + k * ks);
+	}
+}
+
 MD int coord(Geo const &g, long x, int mu) { return (int)((x / g.st[mu]) % g.L[mu]); }
 MD long shift_c(Geo const &g, long x, int mu, int c, int s)
 {

@@ -28,7 +28,28 @@ using namespace mesh_dev;
 #define FORCE_OCC
 #endif
 
+// non-temporal hints on the once-touched fields of the force kernels
+#ifndef MESH_STREAM_HINTS
+#define MESH_STREAM_HINTS 1
+#endif
+
 namespace {
+
+template <class R> __device__ __forceinline__ m3<R> load_once(const R *__restrict__ b, int f, Geo const &g, long x)
+{
+	if constexpr (MESH_STREAM_HINTS != 0)
+		return load_stream(b, f, g, x);
+	else
+		return load(b, f, g, x);
+}
+template <class R>
+__device__ __forceinline__ void store_once(R *__restrict__ b, int f, Geo const &g, long x, m3<R> const &m)
+{
+	if constexpr (MESH_STREAM_HINTS != 0)
+		store_stream(b, f, g, x, m);
+	else
+		store(b, f, g, x, m);
+}
 
 constexpr int SB = 64; // sites per workgroup in the stencil kernels (one wavefront)
 
@@ -219,7 +240,7 @@ __device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Un
 	m3<R> un = mul_nn(expsu3(X), u);
 	if (reunit_flag)
 		un = reunit(un);
-	store(Unext, mu, g, x, un);
+	store_once(Unext, mu, g, x, un);
 }
 
 // fused momentum step: workgroup = 64 sites x 4 directions, wavefront w
@@ -243,14 +264,14 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 	m3<R> S = staple_at(g, U, mu, x, c);
 	m3<R> u = load(U, mu, g, x);
 	m3<R> A = ta(mul_nn(u, S));
-	m3<R> p = load(P, mu, g, x);
+	m3<R> p = load_once(P, mu, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
-	store(P, mu, g, x, p);
+	store_once(P, mu, g, x, p);
 	if constexpr (FUSE)
 		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 }
@@ -340,14 +361,14 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 	}
 	m3<R> u = from_lds(mu, 0, 0);
 	m3<R> A = ta(mul_nn(u, S));
-	m3<R> p = load(P, mu, g, x);
+	m3<R> p = load_once(P, mu, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
-	store(P, mu, g, x, p);
+	store_once(P, mu, g, x, p);
 	if constexpr (FUSE)
 		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 }
@@ -466,9 +487,9 @@ int plaq_sum(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, double *out
 	int gb = mesh_grid_blocks(ctx, nb);
 	int ph = mesh_prof_begin(ctx, PROF_PLAQ);
 	if (prec == MESH_F64)
-		k_plaq<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, ctx->partials, nb, ctx->xcd_remap);
+		k_plaq<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, ctx->partials, nb, mesh_order(ctx));
 	else
-		k_plaq<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, ctx->partials, nb, ctx->xcd_remap);
+		k_plaq<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, ctx->partials, nb, mesh_order(ctx));
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return mesh_finish_reduction(ctx, nb, 1, out);
@@ -490,7 +511,10 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 	const int L3 = geo.L[3], RW = L3 > 0 ? 64 / L3 : 0;
 	const bool tile_ok = geo.nd == 4 && (L3 == 16 || L3 == 32 || L3 == 64) && geo.L[2] % RW == 0;
 	bool fused = Unext != nullptr;
-	if (ctx->force_variant == 2 && tile_ok)
+	// default: FP64 is bound by HBM traffic and register pressure (wavefront
+	// per direction); FP32 by issue, where the LDS tile's fewer loads pay
+	const int variant = ctx->force_variant >= 0 ? ctx->force_variant : (prec == MESH_F32 ? 2 : 0);
+	if (variant == 2 && tile_ok)
 	{
 		int nb = (int)(geo.V / 64);
 		int gb = mesh_grid_blocks(ctx, nb);
@@ -499,10 +523,10 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 			constexpr int l3v = decltype(l3)::value;
 			if (fused)
 				k_force_update_tile<Rt, l3v, true><<<gb, 256, 0, ctx->stream>>>(
-				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, ctx->xcd_remap, (Rt *)Unext, (Rt)tl, reunit_flag);
+				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else
 				k_force_update_tile<Rt, l3v, false><<<gb, 256, 0, ctx->stream>>>(
-				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, ctx->xcd_remap, nullptr, (Rt)0, 0);
+				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), nullptr, (Rt)0, 0);
 		};
 		using I16 = std::integral_constant<int, 16>;
 		using I32 = std::integral_constant<int, 32>;
@@ -512,7 +536,7 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 		else
 			L3 == 16 ? launch(0.0f, I16{}) : L3 == 32 ? launch(0.0f, I32{}) : launch(0.0f, I64{});
 	}
-	else if (ctx->force_variant == 1)
+	else if (variant == 1)
 	{
 		int nb = nblk(geo.V, 256);
 		int gb = mesh_grid_blocks(ctx, nb);
@@ -532,11 +556,11 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 			using Rt = decltype(tag);
 			if (fused)
 				k_force_update<Rt, true><<<gb, 256, 0, ctx->stream>>>(geo, (const Rt *)U, (Rt *)P, (Rt)k, nb,
-				                                                      ctx->xcd_remap, (Rt *)Unext, (Rt)tl,
+				                                                      mesh_order(ctx), (Rt *)Unext, (Rt)tl,
 				                                                      reunit_flag);
 			else
 				k_force_update<Rt, false><<<gb, 256, 0, ctx->stream>>>(geo, (const Rt *)U, (Rt *)P, (Rt)k, nb,
-				                                                       ctx->xcd_remap, nullptr, (Rt)0, 0);
+				                                                       mesh_order(ctx), nullptr, (Rt)0, 0);
 		};
 		prec == MESH_F64 ? launch(0.0) : launch(0.0f);
 	}
@@ -588,11 +612,11 @@ int delta_h(mesh_ctx *ctx, Geo const &geo, int prec, const void *Uo, const void 
 	if (prec == MESH_F64)
 		k_delta_h<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)Uo, (const double *)Un,
 		                                               (const double *)P, Ko, c3, ctx->partials, nb,
-		                                               ctx->xcd_remap);
+		                                               mesh_order(ctx));
 	else
 		k_delta_h<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)Uo, (const float *)Un,
 		                                              (const float *)P, Ko, c3, ctx->partials, nb,
-		                                              ctx->xcd_remap);
+		                                              mesh_order(ctx));
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	double r[5];

@@ -168,6 +168,73 @@ class Flow:
         return out
 
 
+class PlaqFlow:
+    """Plaquette flow on an L0 x L1 lattice: V - 1 independent plaquettes
+    through a learned circle map, links by a maximal-tree gauge, uniform
+    holonomies and gauge; exact log q of any configuration from its
+    plaquettes (pflow.hip).  Same interface as Flow for run_chains."""
+
+    def __init__(self, L0, L1, ctx=None):
+        self.ctx = ctx or default_context()
+        self.grid = Grid((L0, L1))
+        self.L0, self.L1, self.V = L0, L1, L0 * L1
+        self.n = lib().mesh_pflow_num_params()
+        self.params = Buffer(self.n, np.float32, self.ctx)
+        self.m = Buffer(self.n, np.float32, self.ctx).zero()
+        self.v = Buffer(self.n, np.float32, self.ctx).zero()
+        self.grad = Buffer(self.n, np.float32, self.ctx)
+        self.t = 0
+        self._ws = None
+        self._ws_batch = 0
+        self.ctx.call("mesh_pflow_init", self.params.ptr)
+
+    def get_params(self):
+        return self.params.download()
+
+    def set_params(self, p):
+        self.params.upload(p)
+
+    def gradient(self, batch, beta, seed, step):
+        """Reverse-KL gradient over a fresh batch; returns (loss, log q, S/beta)."""
+        if self._ws is None or self._ws_batch != batch:
+            self._ws = Buffer(lib().mesh_pflow_workspace_bytes(self.grid.ref, batch), np.uint8, self.ctx)
+            self._out = Buffer(2 * batch, np.float64, self.ctx)
+            self._ws_batch = batch
+        self.ctx.call("mesh_pflow_gradient", self.grid.ref, batch, self.params.ptr, C.c_uint64(seed),
+                      C.c_uint32(step), _d(beta), self._ws.ptr, self.grad.ptr, self._out.ptr)
+        o = self._out.download().reshape(batch, 2)
+        return float(np.mean(o[:, 0] + beta * o[:, 1])), o[:, 0], o[:, 1]
+
+    def train(self, beta, steps, batch=256, lr=1e-2, seed=1, log_every=0, lr_final=None):
+        """Adam on the reverse KL; returns the history of E[log q + beta S]."""
+        hist = []
+        for i in range(steps):
+            loss, _, _ = self.gradient(batch, beta, seed, self.t)
+            self.t += 1
+            lr_i = lr if lr_final is None else lr * (lr_final / lr) ** (i / max(1, steps - 1))
+            self.ctx.call("mesh_pflow_adam", self.params.ptr, self.grad.ptr, self.m.ptr, self.v.ptr, self.t,
+                          _d(lr_i))
+            hist.append(loss)
+            if log_every and (i % log_every == 0 or i == steps - 1):
+                print(f"pflow step {i}: loss {loss:.4f}", flush=True)
+        return np.array(hist)
+
+    def sample(self, ens, seed, step, theta_out=None, out=None):
+        """Proposals into `theta_out` (or the ensemble) and their per-chain
+        (log q, S/beta) as a device buffer of 2*batch doubles."""
+        tgt = theta_out if theta_out is not None else ens.theta
+        out = out or Buffer(2 * ens.batch, np.float64, self.ctx)
+        self.ctx.call("mesh_pflow_sample", self.grid.ref, ens.batch, self.params.ptr, C.c_uint64(seed),
+                      C.c_uint32(step), tgt.ptr)
+        self.ctx.call("mesh_pflow_logq", self.grid.ref, ens.batch, self.params.ptr, tgt.ptr, out.ptr)
+        return out
+
+    def logq(self, ens, out=None, z_out=None):
+        out = out or Buffer(2 * ens.batch, np.float64, self.ctx)
+        self.ctx.call("mesh_pflow_logq", self.grid.ref, ens.batch, self.params.ptr, ens.theta.ptr, out.ptr)
+        return out
+
+
 def tau_int(series, S=1.5):
     """Integrated autocorrelation time by the Gamma method with Wolff's
     automatic window (U. Wolff, Comput. Phys. Commun. 156 (2004) 143).

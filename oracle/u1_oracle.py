@@ -241,3 +241,123 @@ def flow_loss(params, z, beta):
 def log_prior(L0, L1):
     return -2 * L0 * L1 * math.log(2 * math.pi)
 
+
+
+# ---------------------------------------------------------------- plaquette flow
+#
+# Restatement of mesh_amd/csrc/pflow.hip: V - 1 plaquettes theta_P = h(u),
+# h(u) = g(wrap(u - t)) + t, g = sum_k w_k 2 atan(s_k tan(v/2)); links in the
+# maximal-tree gauge (theta_1(x0, x1 < L1-1) = 0, theta_0(x0 < L0-1, L1-1) = 0)
+# from the plaquettes and holonomies h0, h1, then a gauge transformation a(x).
+
+PK = 8
+NPF = 2 * PK + 2
+TAG_PFLOW = 0x50464C57
+
+
+def _uniform_angle(seed, step, a, b):
+    u = O.philox_draw(seed, a, b, step << 5, TAG_PFLOW)[0] - 2.0 ** -53
+    return np.float32(6.283185307179586477 * u - 3.141592653589793238)
+
+
+def pflow_draws(seed, step, chain, L0, L1):
+    """(u[V-1], a[V], h0, h1) the GPU draws for (seed, step, chain), float32."""
+    V = L0 * L1
+    cc = chain << 1
+    u = np.array([_uniform_angle(seed, step, x, cc) for x in range(V - 1)], np.float32)
+    a = np.array([_uniform_angle(seed, step, x, cc | 1) for x in range(V)], np.float32)
+    return u, a, _uniform_angle(seed, step, V, cc), _uniform_angle(seed, step, V + 1, cc)
+
+
+def pflow_links(P_free, h0, h1, a, L0, L1):
+    """links theta[2, L0, L1] (float64, wrapped) with free plaquettes P_free
+    (V-1, row-major, the last plaquette omitted), holonomies and gauge a[V]."""
+    V = L0 * L1
+    P = np.zeros(V)
+    P[:V - 1] = P_free
+    P = P.reshape(L0, L1)
+    t0 = np.zeros((L0, L1))
+    t1 = np.zeros((L0, L1))
+    for x0 in range(L0):
+        acc = float(h0) if x0 == L0 - 1 else 0.0
+        t0[x0, L1 - 1] = acc
+        for x1 in range(L1 - 2, -1, -1):
+            acc += P[x0, x1]
+            t0[x0, x1] = acc
+    acc = float(h1)
+    t1[0, L1 - 1] = acc
+    for x0 in range(L0 - 1):
+        acc += P[x0, L1 - 1] - t0[x0, L1 - 1] + t0[x0, 0]
+        t1[x0 + 1, L1 - 1] = acc
+    a = np.asarray(a, np.float64).reshape(L0, L1)
+    t0 = t0 + a - np.roll(a, -1, axis=0)
+    t1 = t1 + a - np.roll(a, -1, axis=1)
+    return wrap(np.stack([t0, t1]))
+
+
+def _pflow_map(params):
+    import torch
+
+    y = params
+    s = torch.exp(torch.clamp(y[:PK], -8.0, 8.0))
+    w = torch.softmax(y[PK:2 * PK], dim=0)
+    t = torch.atan2(y[2 * PK + 1], y[2 * PK])
+    return s, w, t
+
+
+def _pflow_g(s, w, v):
+    """g(v), g'(v) for v of any shape (torch)"""
+    import torch
+
+    c, sn = torch.cos(0.5 * v).unsqueeze(-1), torch.sin(0.5 * v).unsqueeze(-1)
+    G = torch.sum(w * 2 * torch.atan2(s * sn, c), dim=-1)
+    Gp = torch.sum(w * s / (c * c + s * s * sn * sn), dim=-1)
+    return G, Gp
+
+
+def pflow_h(params, u):
+    """theta = h(u) (unwrapped) and -log g'(v) for u (torch float64)"""
+    import torch
+
+    s, w, t = _pflow_map(params)
+    v = _twrap(u - t)
+    G, Gp = _pflow_g(s, w, v)
+    return G + t, -torch.log(Gp)
+
+
+def pflow_logq(params, th):
+    """(log q, S/beta) of configurations th[..., 2, L0, L1] by inverting g by
+    bisection in float64 (numpy in, numpy out)."""
+    import torch
+
+    p = torch.as_tensor(np.asarray(params, np.float64))
+    s, w, t = _pflow_map(p)
+    th = np.asarray(th, np.float64)
+    P = plaq(th)
+    V = P.shape[-1] * P.shape[-2]
+    flat = P.reshape(P.shape[:-2] + (V,))
+    target = torch.as_tensor(wrap(flat[..., :V - 1] - t.item()))
+    lo = torch.full_like(target, -math.pi)
+    hi = torch.full_like(target, math.pi)
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        G, _ = _pflow_g(s, w, mid)
+        less = G < target
+        lo = torch.where(less, mid, lo)
+        hi = torch.where(less, hi, mid)
+    _, Gp = _pflow_g(s, w, 0.5 * (lo + hi))
+    logq = -torch.sum(torch.log(Gp), dim=-1).numpy() - 2 * V * math.log(2 * math.pi)
+    return logq, np.sum(1 - np.cos(flat), axis=-1)
+
+
+def pflow_loss(params, u, beta):
+    """per-configuration log q + beta S for draws u[..., V-1] (torch float64),
+    with the last plaquette = -sum of the others; returns (loss, log q, S)."""
+    import torch
+
+    th, nlg = pflow_h(params, u)
+    V = u.shape[-1] + 1
+    last = -torch.sum(th, dim=-1)
+    S = torch.sum(1 - torch.cos(th), dim=-1) + (1 - torch.cos(last))
+    logq = torch.sum(nlg, dim=-1) - 2 * V * math.log(2 * math.pi)
+    return logq + beta * S, logq, S

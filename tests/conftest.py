@@ -1,3 +1,4 @@
+import contextlib
 import os
 import sys
 
@@ -25,3 +26,19 @@ def ctx():
 def pytest_runtest_setup(item):
     if item.get_closest_marker("gpu"):
         print(f"\n[gpu] {item.nodeid}", flush=True)
+
+
+# library defaults of the tuning options (mesh_set_option)
+OPTION_DEFAULTS = {"force_variant": -1, "xcd_remap": 3, "rb_tile": 4, "fuse_md": 1}
+
+
+@contextlib.contextmanager
+def options(ctx, **kw):
+    """set tuning options for a block and restore the library defaults"""
+    try:
+        for k, v in kw.items():
+            ctx.set_option(k, v)
+        yield
+    finally:
+        for k in kw:
+            ctx.set_option(k, OPTION_DEFAULTS[k])

@@ -10,6 +10,7 @@ import pytest
 
 import mesh_amd
 from mesh_amd import F32, F64, gauge
+from conftest import options
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -112,7 +113,7 @@ def test_force_update_bitexact(ctx, shape, prec):
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("prec", [F64, F32])
 @pytest.mark.parametrize("shape", [(4, 4, 2, 32), (2, 3, 4, 16), (2, 2, 2, 64), (3, 2, 6, 32), (4, 4, 4, 4)])
-@pytest.mark.parametrize("remap", [0, 1, 2])
+@pytest.mark.parametrize("remap", [0, 1, 2, 3])
 def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
     """Every force-kernel variant (wavefront per direction, thread per site,
     LDS tile; tile-eligible shapes included) gives the oracle's bits."""
@@ -120,14 +121,43 @@ def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
     P, _ = O.momenta(list(shape), 22, 0, _pname(prec))
     g, fu = _field(ctx, shape, prec, U)
     _, fp = _field(ctx, shape, prec, P)
-    ctx.set_option("force_variant", variant)
-    ctx.set_option("xcd_remap", remap)
-    try:
+    with options(ctx, force_variant=variant, xcd_remap=remap):
         gauge.momentumStep(fu, fp, 6.0, 0.0833)
-    finally:
-        ctx.set_option("force_variant", 0)
-        ctx.set_option("xcd_remap", 2)
     np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
+
+
+@pytest.mark.parametrize("tile", [1, 2, 4])
+@pytest.mark.parametrize("variant", [0, 2])
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", [(8, 2, 4, 32), (16, 2, 8, 16), (8, 3, 8, 32)])
+def test_slab_order_bitexact(ctx, shape, prec, variant, tile):
+    """Block order 3 (x0 slabs per XCD, row-block tiles) with every tile that
+    divides the row-blocks: force update bit-exact, plaquette to 1e-12."""
+    U = O.random_gauge(list(shape), 23, _pname(prec))
+    P, _ = O.momenta(list(shape), 23, 0, _pname(prec))
+    g, fu = _field(ctx, shape, prec, U)
+    _, fp = _field(ctx, shape, prec, P)
+    with options(ctx, force_variant=variant, xcd_remap=3, rb_tile=tile):
+        gauge.momentumStep(fu, fp, 6.0, 0.0833)
+        pl = gauge.plaquette(fu)
+    np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
+    ref = O.plaquette(U, list(shape))
+    assert abs(pl - ref) <= 1e-12 * max(abs(ref), 1e-3)
+
+
+def test_slab_order_trajectory(ctx):
+    """A fused trajectory in block order 3 gives the oracle's field bits."""
+    shape = [8, 2, 4, 32]
+    with options(ctx, xcd_remap=3, rb_tile=2):
+        hmc = gauge.Hmc(mesh_amd.Grid(shape), F64, seed=7, ctx=ctx)
+        hmc.randomizeGaugeField()
+        U = O.random_gauge(shape, 7, "f64")
+        d = gauge.makeDeltas("2mn", 0.5, 2)
+        r = hmc.runHmcUpdate(6.0, d, metropolis=True)
+    o = O.hmc_trajectory(U, shape, 6.0, d, 7, 0, metropolis=True)
+    assert r["accepted"] == o["accepted"]
+    assert abs(r["dH"] - o["dH"]) <= 1e-12 * max(1.0, abs(o["dH"]))
+    np.testing.assert_array_equal(hmc.U.download(), U)
 
 
 @pytest.mark.parametrize("prec", [F64, F32])

@@ -125,8 +125,17 @@ def test_u1_hmc_ensemble_matches_exact(ctx):
 
 
 def test_flow_identity_initialisation(ctx):
+    """the initial weights with the last layer zeroed are the identity map;
+    the initial last layer is small (it only breaks the components' symmetry)"""
     L, B = 8, 4
     f = U1.Flow(L, L, ctx)
+    p = f.get_params()
+    for layer in range(U.NL):
+        o = layer * U.NP_LAYER + U.NP_LAYER - U.NO - U.NH * U.NO
+        w3 = p[o:o + U.NH * U.NO]
+        assert 0 < np.abs(w3).max() < 0.01
+        p[o:o + U.NH * U.NO] = 0
+    f.set_params(p)
     ens = U1.U1Ensemble(L, L, B, ctx)
     out = f.sample(ens, 3, 0).download().reshape(B, 2)
     th = ens.angles()
@@ -246,3 +255,104 @@ def test_flow_errors(ctx):
         U1.Flow(6, 8, ctx).sample(U1.U1Ensemble(6, 8, 2, ctx), 0, 0)
     with pytest.raises(mesh_amd.MeshError):
         U1.U1Ensemble(128, 128, 1, ctx).hmc(1.0, [0.1], 0, 0)
+
+
+def _pparams(seed, scale):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(size=U.NPF) * scale
+    p[2 * U.PK] += 1.0
+    return p.astype(np.float32)
+
+
+def test_pflow_identity(ctx):
+    """identity map: the free plaquettes of a proposal are the uniform draws
+    and log q is the uniform density (2 pi)^-2V"""
+    L, B = 8, 3
+    f = U1.PlaqFlow(L, L, ctx)
+    p = f.get_params()
+    np.testing.assert_allclose(p[:U.PK], np.linspace(-3, 1, U.PK), atol=1e-6)  # distinct components
+    p[:U.PK] = 0
+    f.set_params(p)
+    ens = U1.U1Ensemble(L, L, B, ctx)
+    out = f.sample(ens, 5, 1).download().reshape(B, 2)
+    th = ens.angles().astype(np.float64)
+    for c in range(B):
+        u, _, _, _ = U.pflow_draws(5, 1, c, L, L)
+        P = U.plaq(th[c]).reshape(-1)
+        assert _wrapdiff(P[:-1], u).max() < 2e-5
+    np.testing.assert_allclose(out[:, 0], -2 * L * L * math.log(2 * math.pi), atol=1e-3)
+
+
+@pytest.mark.parametrize("L0,L1", [(8, 8), (6, 10)])
+def test_pflow_sample_against_oracle(ctx, L0, L1):
+    torch = pytest.importorskip("torch")
+    B, seed, step = 3, 61, 4
+    p = _pparams(62, 0.5)
+    f = U1.PlaqFlow(L0, L1, ctx)
+    f.set_params(p)
+    ens = U1.U1Ensemble(L0, L1, B, ctx)
+    out = f.sample(ens, seed, step).download().reshape(B, 2)
+    th = ens.angles().astype(np.float64)
+    pt = torch.tensor(p.astype(np.float64))
+    for c in range(B):
+        u, a, h0, h1 = U.pflow_draws(seed, step, c, L0, L1)
+        thP, nlg = U.pflow_h(pt, torch.tensor(u.astype(np.float64)))
+        ref = U.pflow_links(thP.numpy(), h0, h1, a, L0, L1)
+        assert _wrapdiff(th[c], ref).max() < 2e-4, c
+        lq, S = U.pflow_logq(p, th[c])
+        assert abs(out[c, 0] - lq) < 5e-3, (c, out[c, 0], lq)
+        assert abs(lq - (nlg.sum().item() - 2 * L0 * L1 * math.log(2 * math.pi))) < 5e-3
+        assert out[c, 1] == pytest.approx(S, rel=1e-5)
+
+
+def test_pflow_logq_of_given_configurations(ctx):
+    L0, L1, B = 8, 12, 4
+    p = _pparams(63, 0.7)
+    f = U1.PlaqFlow(L0, L1, ctx)
+    f.set_params(p)
+    ens = U1.U1Ensemble(L0, L1, B, ctx)
+    ens.random_start(64)
+    out = f.logq(ens).download().reshape(B, 2)
+    lq, S = U.pflow_logq(p, ens.angles().astype(np.float64))
+    np.testing.assert_allclose(out[:, 0], lq, atol=5e-3)
+    np.testing.assert_allclose(out[:, 1], S, rtol=1e-5)
+
+
+def test_pflow_gradient_against_autograd(ctx):
+    torch = pytest.importorskip("torch")
+    L, B, beta, seed, step = 8, 6, 2.0, 65, 3
+    p = _pparams(66, 0.5)
+    f = U1.PlaqFlow(L, L, ctx)
+    f.set_params(p)
+    loss, logq, S = f.gradient(B, beta, seed, step)
+    g = f.grad.download().astype(np.float64)
+    pt = torch.tensor(p.astype(np.float64), requires_grad=True)
+    u = torch.tensor(np.stack([U.pflow_draws(seed, step, c, L, L)[0] for c in range(B)]).astype(np.float64))
+    ref, ref_lq, ref_S = U.pflow_loss(pt, u, beta)
+    ref.mean().backward()
+    np.testing.assert_allclose(logq, ref_lq.detach().numpy(), atol=2e-3)
+    np.testing.assert_allclose(S, ref_S.detach().numpy(), rtol=1e-5)
+    assert loss == pytest.approx(ref.mean().item(), abs=5e-3)
+    gr = pt.grad.numpy()
+    assert np.linalg.norm(g - gr) <= 1e-3 * np.linalg.norm(gr), (g, gr)
+
+
+def test_pflow_training_and_chains(ctx):
+    """Trained at beta 2 on 8x8: the reverse KL stays above -log Z, the MH
+    acceptance of its proposals is high, and HMC + pflow MH chains reproduce
+    the exact torus <cos theta_P>."""
+    L, beta = 8, 2.0
+    f = U1.PlaqFlow(L, L, ctx)
+    hist = f.train(beta, 400, batch=256, lr=2e-2, lr_final=2e-3, seed=67)
+    bound = -_log_z(beta, L * L)
+    assert hist[-40:].mean() > bound - 0.3, (hist[-40:].mean(), bound)
+    assert hist[-40:].mean() < hist[:5].mean() - 10
+    B = 512
+    ens = U1.U1Ensemble(L, L, B, ctx)
+    ens.random_start(68)
+    r = U1.run_chains(ens, beta, gauge.makeDeltas("2lf", 0.5, 10), 60, 69, flow=f, n_therm=20, record_every=2)
+    assert r["acc_flow"] > 0.3, r["acc_flow"]
+    chain_means = r["plaq"].mean(axis=1)
+    est, err = chain_means.mean(), chain_means.std(ddof=1) / math.sqrt(B)
+    exact = U.exact_mean_cos(beta, L * L)
+    assert abs(est - exact) < 5 * err + 1e-4, (est, exact, err)

@@ -147,3 +147,83 @@ def test_tau_int_ar1():
         tau, err = tau_int(x)
         exact = (1 + rho) / (2 * (1 - rho))
         assert abs(tau - exact) < max(4 * err, 0.05 * exact), (rho, tau, err, exact)
+
+
+def _pflow_params(seed, scale):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(size=U.NPF) * scale
+    p[2 * U.PK] += 1.0
+    return p
+
+
+def test_pflow_links_have_the_drawn_plaquettes():
+    """Tree-gauge construction: every free plaquette of the built links is the
+    drawn one, the last is minus their sum, and gauge/holonomy draws do not
+    touch any plaquette."""
+    rng = np.random.default_rng(20)
+    for L0, L1 in ((4, 4), (3, 5), (6, 2)):
+        V = L0 * L1
+        Pf = rng.uniform(-np.pi, np.pi, V - 1)
+        th = U.pflow_links(Pf, rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(-3, 3, V), L0, L1)
+        P = U.plaq(th).reshape(-1)
+        assert np.abs(U.wrap(P[:V - 1] - Pf)).max() < 1e-12
+        assert abs(U.wrap(P[V - 1] + Pf.sum())) < 1e-12
+
+
+def test_pflow_map_is_unimodular():
+    """(free plaquettes, h0, h1, gauge a(x != 0)) -> links is integer-linear
+    with |det| = 1, so the density of the links is the product of the
+    plaquette densities times (2 pi)^-(V+1) (pins the log q normalisation)."""
+    L0, L1 = 3, 4
+    V = L0 * L1
+    eps = 1e-3
+    cols = []
+    for k in range(2 * V):
+        Pf, h, a = np.zeros(V - 1), np.zeros(2), np.zeros(V)
+        if k < V - 1:
+            Pf[k] = eps
+        elif k < V + 1:
+            h[k - (V - 1)] = eps
+        else:
+            a[k - (V + 1) + 1] = eps
+        cols.append(U.pflow_links(Pf, h[0], h[1], a, L0, L1).reshape(-1) / eps)
+    M = np.array(cols).T
+    np.testing.assert_allclose(M, np.rint(M), atol=1e-9)
+    assert abs(abs(np.linalg.det(M)) - 1) < 1e-9
+
+
+def test_pflow_logq_identity_and_inverse():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(21)
+    L0, L1 = 4, 6
+    V = L0 * L1
+    th = rng.uniform(-np.pi, np.pi, size=(3, 2, L0, L1))
+    ident = np.zeros(U.NPF)
+    ident[2 * U.PK] = 1.0
+    lq, _ = U.pflow_logq(ident, th)
+    np.testing.assert_allclose(lq, -2 * V * math.log(2 * math.pi), atol=1e-12)
+    # log q of built links = the forward map's own -log g' (inverse consistency)
+    p = _pflow_params(22, 0.5)
+    u = torch.tensor(rng.uniform(-np.pi, np.pi, size=(3, V - 1)))
+    thP, nlg = U.pflow_h(torch.tensor(p), u)
+    for c in range(3):
+        links = U.pflow_links(thP[c].numpy(), 0.3, -1.1, rng.uniform(-3, 3, V), L0, L1)
+        lq, _ = U.pflow_logq(p, links)
+        assert abs(lq - (nlg[c].sum().item() - 2 * V * math.log(2 * math.pi))) < 1e-9
+
+
+def test_pflow_loss_gradient_finite_difference():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(23)
+    p = torch.tensor(_pflow_params(24, 0.5), requires_grad=True)
+    u = torch.tensor(rng.uniform(-np.pi, np.pi, size=(4, 15)))
+    loss = U.pflow_loss(p, u, 2.0)[0].mean()
+    loss.backward()
+    g = p.grad.numpy()
+    h = 1e-6
+    for i in range(U.NPF):
+        pp, pm = p.detach().clone(), p.detach().clone()
+        pp[i] += h
+        pm[i] -= h
+        fd = (U.pflow_loss(pp, u, 2.0)[0].mean() - U.pflow_loss(pm, u, 2.0)[0].mean()).item() / (2 * h)
+        assert abs(fd - g[i]) < 1e-6 * (1 + abs(g[i])), (i, fd, g[i])
