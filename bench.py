@@ -237,6 +237,53 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
     return out
 
 
+def landau_bench(L, seed, therm, max_sweeps, omega=1.7, tol=1e-12):
+    """Landau gauge fixing of a thermalised SU(3) L^4 configuration (HMC as in
+    the main workload, `therm` trajectories) by even/odd overrelaxation, in
+    FP32 links (the measured path) and FP64; theta checked every 10 sweeps."""
+    import mesh_amd
+    from mesh_amd import F32, F64, gauge
+
+    ctx = mesh_amd.default_context()
+    g = mesh_amd.Grid((L,) * 4)
+    V = g.size()
+    hmc = gauge.Hmc(g, F64, seed=seed, ctx=ctx)
+    hmc.randomizeGaugeField()
+    d = gauge.makeDeltas("2mn", 0.5, 12)
+    for _ in range(therm):
+        hmc.runHmcUpdate(6.0, d, metropolis=False)
+    out = {"metric": f"Landau gauge fixing site-sweeps/s at {L}^4 (FP32 links)",
+           "config": {"workload": f"SU(3) {L}^4 beta=6.0 after {therm} HMC trajectories; overrelaxation "
+                                  f"omega={omega}, theta < {tol:g} checked every 10 sweeps", "L": L},
+           "data": "synthetic: Philox-seeded Haar start, thermalised by HMC"}
+    for prec in (F32, F64):
+        R = 4 if prec == F32 else 8
+        U = hmc.U.convert(prec) if prec == F32 else hmc.U.clone()
+        lan = gauge.Landau(U)
+        ctx.synchronize()
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        sweeps, theta = lan.run(max_sweeps, omega=omega, tol=tol, check_every=10)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        ctx.profile(False)
+        ms, n = ctx.profile_read("landau_or")
+        # per half-sweep: every link read and written once, the transformation of
+        # the updated half read and written: (144 + 18) R per site
+        alg = 162 * R * V
+        name = "f32" if prec == F32 else "f64"
+        out[name] = {"sweeps": sweeps, "theta": theta, "functional": lan(), "plaquette": gauge.plaquette(U),
+                     "seconds": dt, "site_sweeps_per_s": V * sweeps / dt,
+                     "half_sweep_ms": ms / max(n, 1),
+                     "half_sweep_frac_of_hbm": alg / (ms / max(n, 1) * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+    out["plaquette_before"] = gauge.plaquette(hmc.U)
+    out["value"] = out["f32"]["site_sweeps_per_s"]
+    out["unit"] = "site-sweeps/s"
+    out["higher_is_better"] = True
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--kernel-only", default=None,
@@ -257,6 +304,9 @@ def main():
     ap.add_argument("--substeps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32-mixed run")
+    ap.add_argument("--landau", action="store_true",
+                    help="instead: Landau gauge fixing of a thermalised 32^4 configuration (one JSON line)")
+    ap.add_argument("--landau-sweeps", type=int, default=30000)
     ap.add_argument("--u1", action="store_true",
                     help="instead: U(1) 2-d ESS/s of Q, flow-assisted vs plain HMC (one JSON line)")
     ap.add_argument("--u1-L", type=int, default=32)
@@ -274,6 +324,9 @@ def main():
 
     from mesh_amd import F32, F64
 
+    if a.landau:
+        print(json.dumps(landau_bench(a.L, a.seed, a.therm, a.landau_sweeps)), flush=True)
+        return
     if a.u1:
         print(json.dumps(u1_ess(a.u1_L, a.u1_beta, a.u1_batch, a.u1_train, a.u1_hmc_steps, a.u1_flow_steps,
                                 a.seed, tuple(a.u1_flows.split(",")))), flush=True)

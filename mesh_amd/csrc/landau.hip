@@ -174,15 +174,10 @@ template <class R> __device__ __forceinline__ m3<R> or_transform(m3<R> K, R omeg
 	return G;
 }
 
-// thread t -> t-th site of the given parity; rows along the fastest
-// dimension (even extent) hold half their sites in each parity
-template <class R>
-__global__ __launch_bounds__(256) void k_or_half(Geo g, R *U, R *h, int parity, R omega)
+// overrelaxation at the t-th site of the given parity; rows along the
+// fastest dimension (even extent) hold half their sites in each parity
+template <class R> __device__ __forceinline__ void or_site(Geo const &g, R *U, R *h, int parity, R omega, long t)
 {
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	long half = g.V >> 1;
-	if (t >= half)
-		return;
 	int f = g.nd - 1;
 	int hl = g.L[f] >> 1;
 	long row = t / hl;
@@ -218,6 +213,26 @@ __global__ __launch_bounds__(256) void k_or_half(Geo g, R *U, R *h, int parity, 
 	}
 	if (h)
 		store(h, 0, g, x, mul_nn(G, load(h, 0, g, x)));
+}
+
+template <class R> __global__ __launch_bounds__(256) void k_or_half(Geo g, R *U, R *h, int parity, R omega)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < (g.V >> 1))
+		or_site(g, U, h, parity, omega, t);
+}
+
+// The same with a workgroup per 256-site span (whole rows: L_fast | 256), the
+// spans in the block order of the force kernels, so that a span and the spans
+// holding its backward neighbours run close together and share the L2 lines
+// that each half-sweep touches half of.
+template <class R>
+__global__ __launch_bounds__(128) void k_or_half_span(Geo g, R *U, R *h, int parity, R omega, int nb, int order)
+{
+	int lb = ordered_block(g, nb, 256, order);
+	if (lb < 0)
+		return;
+	or_site(g, U, h, parity, omega, (long)lb * 128 + threadIdx.x);
 }
 
 template <class R>
@@ -318,12 +333,27 @@ int mesh_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, int prec, void *
 	for (int i = 0; i < geo.nd; ++i)
 		if (geo.L[i] % 2)
 			return mesh_fail(ctx, MESH_E_SIZE, "checkerboard update needs even extents");
-	int nb = nblk(geo.V / 2);
 	int ph = mesh_prof_begin(ctx, PROF_LANDAU_OR);
-	if (prec == MESH_F64)
-		k_or_half<double><<<nb, 256, 0, ctx->stream>>>(geo, (double *)U, (double *)h, parity, omega);
+	const int lf = geo.L[geo.nd - 1];
+	if (geo.V % 256 == 0 && 256 % lf == 0)
+	{
+		int nb = (int)(geo.V / 256);
+		int gb = mesh_grid_blocks(ctx, nb);
+		if (prec == MESH_F64)
+			k_or_half_span<double><<<gb, 128, 0, ctx->stream>>>(geo, (double *)U, (double *)h, parity, omega, nb,
+			                                                    mesh_order(ctx));
+		else
+			k_or_half_span<float><<<gb, 128, 0, ctx->stream>>>(geo, (float *)U, (float *)h, parity, (float)omega,
+			                                                   nb, mesh_order(ctx));
+	}
 	else
-		k_or_half<float><<<nb, 256, 0, ctx->stream>>>(geo, (float *)U, (float *)h, parity, (float)omega);
+	{
+		int nb = nblk(geo.V / 2);
+		if (prec == MESH_F64)
+			k_or_half<double><<<nb, 256, 0, ctx->stream>>>(geo, (double *)U, (double *)h, parity, omega);
+		else
+			k_or_half<float><<<nb, 256, 0, ctx->stream>>>(geo, (float *)U, (float *)h, parity, (float)omega);
+	}
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;

@@ -53,9 +53,11 @@ def test_deriv_f64(ctx, shape):
 
 
 @pytest.mark.parametrize("prec", [F64, F32])
-@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("shape", SHAPES + [(8, 2, 16, 64), (16, 2, 8, 32)])
 @pytest.mark.parametrize("track", [False, True])
 def test_or_halfsweep_bitexact(ctx, shape, prec, track):
+    """both half-sweep kernels (256-site spans in block order when the fast
+    extent divides 256 and 256 divides V, else one thread per parity site)"""
     sh = list(shape)
     U = O.random_gauge(sh, 41, _pname(prec))
     h = np.zeros((1,) + U.shape[1:], U.dtype)

@@ -292,3 +292,43 @@ def test_nan_momenta_rejects(ctx):
     r = hmc.runHmcUpdate(6.0, np.array([np.nan, 0.1, np.nan]), metropolis=True)
     assert not r["accepted"]
     np.testing.assert_array_equal(hmc.U.download(), before)
+
+
+def test_c1_workload_matches_oracle(ctx):
+    """The small reference workload (8^4, beta 6, FP64, Haar start seed 1,
+    2lf eps 0.5 x 10, 20 Metropolis trajectories): every trajectory's
+    accept decision, plaquette and Delta H against the oracle, and the final
+    field bit for bit."""
+    shape = [8, 8, 8, 8]
+    hmc = gauge.Hmc(mesh_amd.Grid(shape), F64, seed=1, ctx=ctx)
+    hmc.randomizeGaugeField()
+    U = O.random_gauge(shape, 1, "f64")
+    d = gauge.makeDeltas("2lf", 0.5, 10)
+    for traj in range(20):
+        r = hmc.runHmcUpdate(6.0, d, metropolis=True)
+        o = O.hmc_trajectory(U, shape, 6.0, d, 1, traj, metropolis=True)
+        assert r["accepted"] == o["accepted"], traj
+        assert abs(r["dH"] - o["dH"]) <= 1e-12 * max(1.0, abs(o["dH"])), (traj, r["dH"], o["dH"])
+        assert abs(r["plaq"] - o["plaq"]) <= 1e-13
+    np.testing.assert_array_equal(hmc.U.download(), U)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_c2_force_parity_32(ctx, prec):
+    """Full-size force parity: the fused momentum step on a 32^4 configuration
+    thermalised by 20 HMC trajectories, against the oracle element by
+    element (bit-exact; the required bound is 1e-12 relative)."""
+    shape = [32, 32, 32, 32]
+    g = mesh_amd.Grid(shape)
+    hmc = gauge.Hmc(g, F64, seed=2, ctx=ctx)
+    hmc.randomizeGaugeField()
+    d = gauge.makeDeltas("2mn", 0.5, 12)
+    for _ in range(20):
+        hmc.runHmcUpdate(6.0, d, metropolis=False)
+    U = hmc.U if prec == F64 else hmc.U.convert(prec)
+    P = mesh_amd.GaugeField(g, prec, ctx)
+    gauge.randomMomenta(P, 2, 99, None)
+    Uh, Ph = U.download(), P.download()
+    gauge.momentumStep(U, P, 6.0, 1.0 / 12)
+    ref = O.force_update(Uh, Ph, shape, 6.0, 1.0 / 12)
+    np.testing.assert_array_equal(P.download(), ref)
