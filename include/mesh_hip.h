@@ -14,14 +14,17 @@
  * sum, a force or a gauge transformation):
  *
  *     matrix f < nf, complex entry k < 9 (row-major, k = 3 i + j), site x:
- *         elem(f, k, x) = base[f * 9 * S + (x / B) * 9 * B + k * B + x % B]
+ *         elem(f, k, x) = base[f * 9 * S + (x / B) * 9 * B + k * B + slot(x % B)]
+ *         slot(l) = (l % 2) * B / 2 + l / 2
  *     (complex: re, im), S = mesh_site_stride(grid) = V rounded up to a
  *     multiple of 256, B = mesh_site_block() (a power of two <= 256).
  *
  * i.e. each direction holds blocks of B consecutive sites; a block stores the
- * 9 complex entries as 9 planes of B sites, so consecutive lanes of a
- * wavefront read consecutive sites (16 B per lane in FP64, 8 B in FP32) and
- * one matrix of a 64-site wavefront is one contiguous 9 B * 16 B range.  Sites
+ * 9 complex entries as 9 planes of B sites (the block's even sites first,
+ * then its odd sites, so each checkerboard parity of a row with an even
+ * fastest extent is contiguous), and the lanes of a wavefront over B
+ * consecutive sites read one contiguous B * 16 B (FP64) / B * 8 B (FP32)
+ * segment per entry.  Sites
  * are row-major with dimension 0 slowest (src/lattice/grid.h:39-49).  Matrix
  * f of every site is the nf = 1 field at base + f * 9 * S complex elements.
  *

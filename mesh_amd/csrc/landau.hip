@@ -34,7 +34,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_functional(Geo g, const R *__restrict__ U, const R *__restrict__ h,
                                                     double *partials, int nb)
 {
-	long x = (long)blockIdx.x * 256 + threadIdx.x;
+	long x = lane_site(g, (long)blockIdx.x * 256 + threadIdx.x);
 	double v[1] = {0.0};
 	if (x < g.V)
 	{
@@ -53,7 +53,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_divsum(Geo g, const R *__restrict__ U, const R *__restrict__ h,
                                                 double *partials, int nb)
 {
-	long x = (long)blockIdx.x * 256 + threadIdx.x;
+	long x = lane_site(g, (long)blockIdx.x * 256 + threadIdx.x);
 	double v[1] = {0.0};
 	if (x < g.V)
 	{
@@ -80,7 +80,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_deriv(Geo g, const R *__restrict__ U, const R *__restrict__ h, double c,
                                                R *__restrict__ F)
 {
-	long x = (long)blockIdx.x * 256 + threadIdx.x;
+	long x = lane_site(g, (long)blockIdx.x * 256 + threadIdx.x);
 	if (x >= g.V)
 		return;
 	m3<double> d = zero_m3<double>();
@@ -189,8 +189,16 @@ template <class R> __device__ __forceinline__ void or_site(Geo const &g, R *U, R
 	long x = x0 + 2 * j + ((parity + ps) & 1);
 	m3<R> K = zero_m3<R>();
 	long xm[4];
-	for (int mu = 0; mu < g.nd; ++mu)
+	// The links are kept in registers for the update, or the L2 has evicted
+	// them by then and the field is read twice per half-sweep: FP32 keeps all
+	// 2 Nd, FP64 the Nd forward ones (all would cost 288 registers)
+	constexpr bool keep = sizeof(R) == 4;
+	m3<R> la[4], lb[keep ? 4 : 1];
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
 	{
+		if (mu >= g.nd)
+			break;
 		xm[mu] = shift_c(g, x, mu, coord(g, x, mu), -1);
 		m3<R> a = load(U, mu, g, x);
 		m3<R> b = load(U, mu, g, xm[mu]);
@@ -204,12 +212,26 @@ template <class R> __device__ __forceinline__ void or_site(Geo const &g, R *U, R
 				K.e[3 * i + jj].re += b.e[3 * jj + i].re;
 				K.e[3 * i + jj].im -= b.e[3 * jj + i].im;
 			}
+		la[mu] = a;
+		if constexpr (keep)
+			lb[mu] = b;
 	}
 	m3<R> G = or_transform(K, omega);
-	for (int mu = 0; mu < g.nd; ++mu)
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
 	{
-		store(U, mu, g, x, mul_nn(G, load(U, mu, g, x)));
-		store(U, mu, g, xm[mu], mul_nd(load(U, mu, g, xm[mu]), G));
+		if (mu >= g.nd)
+			break;
+		if constexpr (keep)
+		{
+			store(U, mu, g, x, mul_nn(G, la[mu]));
+			store(U, mu, g, xm[mu], mul_nd(lb[mu], G));
+		}
+		else
+		{
+			store(U, mu, g, x, mul_nn(G, la[mu]));
+			store(U, mu, g, xm[mu], mul_nd(load(U, mu, g, xm[mu]), G));
+		}
 	}
 	if (h)
 		store(h, 0, g, x, mul_nn(G, load(h, 0, g, x)));
@@ -243,7 +265,7 @@ __global__ __launch_bounds__(256) void k_gauge_transform(Geo g, R *__restrict__ 
 	if (i >= g.V * g.nd)
 		return;
 	int mu = (int)(i / g.V);
-	long x = i - (long)mu * g.V;
+	long x = lane_site(g, i - (long)mu * g.V);
 	long xp = shift_c(g, x, mu, coord(g, x, mu), 1);
 	m3<R> u = mul_nn(load(h, 0, g, x), load(U, mu, g, x));
 	store(Uo, mu, g, x, mul_nd(u, load(h, 0, g, xp)));

@@ -446,7 +446,8 @@ template <class R> __global__ void k_aos_to_soa(const R *aos, R *soa, Geo g, int
 	if (i >= g.V * nf)
 		return;
 	int f = (int)(i / g.V);
-	long x = i - (long)f * g.V;
+	long x = lane_site(g, i - (long)f * g.V);
+	i = (long)f * g.V + x;
 	m3<R> m;
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
@@ -460,7 +461,8 @@ template <class R> __global__ void k_soa_to_aos(const R *soa, R *aos, Geo g, int
 	if (i >= g.V * nf)
 		return;
 	int f = (int)(i / g.V);
-	long x = i - (long)f * g.V;
+	long x = lane_site(g, i - (long)f * g.V);
+	i = (long)f * g.V + x;
 	m3<R> m = load(soa, f, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
@@ -504,7 +506,7 @@ template <class R> __global__ void k_unit(R *dst, Geo g, int nf)
 	if (i >= g.V * nf)
 		return;
 	int f = (int)(i / g.V);
-	store(dst, f, g, i - (long)f * g.V, unit_m3<R>());
+	store(dst, f, g, lane_site(g, i - (long)f * g.V), unit_m3<R>());
 }
 
 template <class R>

@@ -365,22 +365,54 @@ struct Geo
 // ---- field access -------------------------------------------------------------
 //
 // Matrix f of site x, complex entry k:
-//   base[f * 9 * stride + (x / B) * 9 * B + k * B + (x % B)]
+//   base[f * 9 * stride + (x / B) * 9 * B + k * B + slot(x % B)]
 // i.e. each direction is split into blocks of B = 2^MESH_SITE_LB consecutive
 // sites and a block stores its 9 complex planes back to back
-// (MESH_SITE_LB = 0: plain SoA, one block of `stride` sites).
+// (MESH_SITE_LB = 0: plain SoA, one block of `stride` sites).  Within a
+// plane the even sites of the block come first, then the odd ones:
+// slot(l) = (l & 1) * B/2 + l / 2 (MESH_SITE_PAIRS = 1), so that with an even
+// fastest extent each checkerboard parity of a row is a contiguous run (the
+// Landau half-sweep reads and writes whole cache lines); a wavefront of B
+// consecutive sites still covers the same contiguous B-site plane segment.
 #ifndef MESH_SITE_LB
 #define MESH_SITE_LB 6
+#endif
+#ifndef MESH_SITE_PAIRS
+#define MESH_SITE_PAIRS 1
 #endif
 
 MD long site_off(Geo const &g, int f, long x)
 {
 	long o = (long)(f * 9) * g.stride;
 	if (MESH_SITE_LB)
-		return o + (((x >> MESH_SITE_LB) * 9) << MESH_SITE_LB) + (x & ((1L << MESH_SITE_LB) - 1));
+	{
+		long l = x & ((1L << MESH_SITE_LB) - 1);
+		if (MESH_SITE_PAIRS)
+			l = ((l & 1) << (MESH_SITE_LB - 1)) | (l >> 1);
+		return o + (((x >> MESH_SITE_LB) * 9) << MESH_SITE_LB) + l;
+	}
 	return o + x;
 }
 MD long plane_step(Geo const &g) { return MESH_SITE_LB ? (1L << MESH_SITE_LB) : g.stride; }
+
+// Site of the linear thread index t in kernels with one thread per site: the
+// inverse of the slot order within each B-site group, so that lane l of a
+// wavefront touches plane slot l and a wavefront's accesses stay contiguous.
+// Identity unless V is a multiple of B (the permutation must map [0, V) onto
+// itself).
+MD long lane_site(Geo const &g, long t)
+{
+#if MESH_SITE_PAIRS && MESH_SITE_LB
+	constexpr long B = 1L << MESH_SITE_LB, hb = B / 2;
+	if ((g.V & (B - 1)) == 0)
+	{
+		long l = t & (B - 1);
+		l = l < hb ? 2 * l : 2 * (l - hb) + 1;
+		return (t & ~(B - 1)) | l;
+	}
+#endif
+	return t;
+}
 
 template <class R> MD m3<R> load(const R *__restrict__ base, int f, Geo const &g, long x)
 {

@@ -124,7 +124,7 @@ template <class R> __global__ __launch_bounds__(256) void k_random_gauge(Geo g, 
 	if (i >= g.V * nf)
 		return;
 	int f = (int)(i / g.V);
-	long x = i - (long)f * g.V;
+	long x = lane_site(g, i - (long)f * g.V);
 	double z[18];
 	link_normals<18>(seed, tag, 0u, (uint32_t)x, (uint32_t)f, z);
 	store(U, f, g, x, from_f64<R>(haar_from_normals(z)));
@@ -133,7 +133,7 @@ template <class R> __global__ __launch_bounds__(256) void k_random_gauge(Geo g, 
 template <class R> __global__ __launch_bounds__(256) void k_momenta(Geo g, R *P, double *K, uint64_t seed,
                                                                    uint32_t traj)
 {
-	long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	long x = lane_site(g, (long)blockIdx.x * blockDim.x + threadIdx.x);
 	if (x >= g.V)
 		return;
 	double k = 0.0;
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_plaq(Geo g, const R *__restrict__ U, do
 	int lb = ordered_block(g, nb, 256, remap);
 	if (lb < 0)
 		return;
-	long x = (long)lb * 256 + threadIdx.x;
+	long x = lane_site(g, (long)lb * 256 + threadIdx.x);
 	double v[1] = {0.0};
 	if (x < g.V)
 	{
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_delta_h(Geo g, const R *__restrict__ Uo
 	int lb = ordered_block(g, nb, 256, remap);
 	if (lb < 0)
 		return;
-	long x = (long)lb * 256 + threadIdx.x;
+	long x = lane_site(g, (long)lb * 256 + threadIdx.x);
 	double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 	if (x < g.V)
 	{
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void k_delta_h(Geo g, const R *__restrict__ Uo
 template <class R>
 __global__ __launch_bounds__(256) void k_staple(Geo g, const R *__restrict__ U, int mu, R *__restrict__ S)
 {
-	long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	long x = lane_site(g, (long)blockIdx.x * blockDim.x + threadIdx.x);
 	if (x >= g.V)
 		return;
 	int c[4];
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_staple(Geo g, const R *__restrict__ U, 
 template <class R>
 __global__ __launch_bounds__(256) void k_force(Geo g, const R *__restrict__ U, int mu, R coef, R *__restrict__ F)
 {
-	long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	long x = lane_site(g, (long)blockIdx.x * blockDim.x + threadIdx.x);
 	if (x >= g.V)
 		return;
 	int c[4];
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 	if (lb < 0)
 		return;
 	int mu = threadIdx.x >> 6;
-	long x = (long)lb * SB + (threadIdx.x & 63);
+	long x = lane_site(g, (long)lb * SB + (threadIdx.x & 63));
 	if (mu >= g.nd || x >= g.V)
 		return;
 	int c[4];
@@ -301,6 +301,8 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 	for (int e = threadIdx.x; e < NR * 4 * 9 * L3; e += 256)
 	{
 		int c = e % L3;
+		if (MESH_SITE_PAIRS) // consecutive threads on consecutive plane slots
+			c = c < L3 / 2 ? 2 * c : 2 * (c - L3 / 2) + 1;
 		int k = (e / L3) % 9;
 		int d = (e / (9 * L3)) % 4;
 		int lr = e / (36 * L3);
@@ -312,9 +314,8 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 	__syncthreads();
 
 	const int mu = threadIdx.x >> 6;
-	const int lane = threadIdx.x & 63;
-	const int r = lane / L3, c3 = lane % L3;
-	const long x = base + lane;
+	const long x = lane_site(g, base + (threadIdx.x & 63));
+	const int r = (int)(x - base) / L3, c3 = (int)(x - base) % L3;
 	int c[4];
 	coords<R>(g, x, c);
 	// matrix of direction d at the site displaced by (dx2, dx3) from x, from LDS
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(256) void k_force_update_site(Geo g, const R *__res
 	int lb = ordered_block(g, nb, 256, remap);
 	if (lb < 0)
 		return;
-	long x = (long)lb * 256 + threadIdx.x;
+	long x = lane_site(g, (long)lb * 256 + threadIdx.x);
 	if (x >= g.V)
 		return;
 	int c[4];
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(256) void k_link_update(Geo g, int nf, R *Uout, con
 	if (i >= g.V * nf)
 		return;
 	int f = (int)(i / g.V);
-	long x = i - (long)f * g.V;
+	long x = lane_site(g, i - (long)f * g.V);
 	m3<R> X = ta(load(P, f, g, x));
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
@@ -430,7 +431,7 @@ template <class R> __global__ __launch_bounds__(256) void k_reunit(Geo g, int nf
 	if (i >= g.V * nf)
 		return;
 	int f = (int)(i / g.V);
-	long x = i - (long)f * g.V;
+	long x = lane_site(g, i - (long)f * g.V);
 	store(U, f, g, x, reunit(load(U, f, g, x)));
 }
 
@@ -439,7 +440,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_trace_sum(Geo g, int nf, const R *__restrict__ A, const R *__restrict__ B,
                                                    double *partials, int nb)
 {
-	long x = (long)blockIdx.x * 256 + threadIdx.x;
+	long x = lane_site(g, (long)blockIdx.x * 256 + threadIdx.x);
 	double v[1] = {0.0};
 	if (x < g.V)
 	{
