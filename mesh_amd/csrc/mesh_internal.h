@@ -137,10 +137,9 @@ __device__ __forceinline__ int logical_block(int nb, int remap)
 //      and only the two x0 faces of the slab come from another XCD.
 // Returns the memory-order block index (-1: padding workgroup).  The order
 // only permutes workgroups; every site's arithmetic is unchanged.
-__device__ __forceinline__ int ordered_block(Geo const &g, int nb, int sbk, int arg)
+__device__ __forceinline__ int ordered_block_of(Geo const &g, int nb, int sbk, int arg, int b)
 {
 	const int order = arg & 15, T = arg >> 4;
-	int b = blockIdx.x;
 	if (order == 0)
 		return b < nb ? b : -1;
 	const bool planes = g.nd == 4 && (g.L[2] * g.L[3]) % sbk == 0;
@@ -170,6 +169,10 @@ __device__ __forceinline__ int ordered_block(Geo const &g, int nb, int sbk, int 
 	int x1 = t % g.L[1];
 	int rb = t / g.L[1];
 	return (x0 * g.L[1] + x1) * nb2 + rb;
+}
+__device__ __forceinline__ int ordered_block(Geo const &g, int nb, int sbk, int arg)
+{
+	return ordered_block_of(g, nb, sbk, arg, blockIdx.x);
 }
 
 // deterministic block sum of NQ doubles; thread 0 writes partials[q*nb + lb]

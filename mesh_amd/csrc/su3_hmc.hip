@@ -32,6 +32,10 @@ using namespace mesh_dev;
 #ifndef MESH_STREAM_HINTS
 #define MESH_STREAM_HINTS 1
 #endif
+// staples with the direction as a compile-time constant (4-d lattices)
+#ifndef MESH_STATIC_MU
+#define MESH_STATIC_MU 1
+#endif
 
 namespace {
 
@@ -85,6 +89,52 @@ template <class R> __device__ __forceinline__ m3<R> staple_at(Geo const &g, cons
 		}
 	}
 	return S;
+}
+
+// The same for a compile-time direction of a 4-d lattice: no branch between
+// the six staples, so the compiler can issue the loads of later staples
+// while earlier ones multiply (same operations in the same order).
+template <int MU, class R>
+__device__ __forceinline__ m3<R> staple_4d(Geo const &g, const R *__restrict__ U, long x, int const *c)
+{
+	m3<R> S = zero_m3<R>();
+	long xpm = shift_c(g, x, MU, c[MU], 1);
+#pragma unroll
+	for (int nu = 0; nu < 4; ++nu)
+	{
+		if (nu == MU)
+			continue;
+		long xpn = shift_c(g, x, nu, c[nu], 1);
+		long xmn = shift_c(g, x, nu, c[nu], -1);
+		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
+		{
+			m3<R> T1 = mul_nd(load(U, nu, g, xpm), load(U, MU, g, xpn));
+			madd_nd(S, T1, load(U, nu, g, x));
+		}
+		{
+			m3<R> T2 = mul_dd(load(U, nu, g, xpmmn), load(U, MU, g, xmn));
+			madd_nn(S, T2, load(U, nu, g, xmn));
+		}
+	}
+	return S;
+}
+
+template <class R>
+__device__ __forceinline__ m3<R> staple_any(Geo const &g, const R *__restrict__ U, int mu, long x, int const *c)
+{
+	if (g.nd != 4)
+		return staple_at(g, U, mu, x, c);
+	switch (mu)
+	{
+	case 0:
+		return staple_4d<0>(g, U, x, c);
+	case 1:
+		return staple_4d<1>(g, U, x, c);
+	case 2:
+		return staple_4d<2>(g, U, x, c);
+	default:
+		return staple_4d<3>(g, U, x, c);
+	}
 }
 
 // Re tr of all plaquettes at x, in double
@@ -261,7 +311,7 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 		return;
 	int c[4];
 	coords<R>(g, x, c);
-	m3<R> S = staple_at(g, U, mu, x, c);
+	m3<R> S = MESH_STATIC_MU ? staple_any(g, U, mu, x, c) : staple_at(g, U, mu, x, c);
 	m3<R> u = load(U, mu, g, x);
 	m3<R> A = ta(mul_nn(u, S));
 	m3<R> p = load_once(P, mu, g, x);

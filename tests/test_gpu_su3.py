@@ -230,16 +230,18 @@ def test_hmc_trajectories_match_oracle(ctx, prec):
         np.testing.assert_array_equal(hmc.U.download(), U)
 
 
+@pytest.mark.parametrize("variant", [-1, 0, 2])
 @pytest.mark.parametrize("fuse", [0, 1])
 @pytest.mark.parametrize("scheme,n", [("2lf", 3), ("2mn", 2), ("4mn", 1), ("2lf", 4)])
 @pytest.mark.parametrize("prec", [F64, F32])
-def test_hmc_fused_md_bitexact(ctx, prec, scheme, n, fuse):
+def test_hmc_fused_md_bitexact(ctx, prec, scheme, n, fuse, variant):
     """The fused momentum+link step and the separate kernels give the oracle's
     bits for every integrator; odd and even numbers of fused pairs exercise
-    both parities of the field ping-pong."""
+    both parities of the field ping-pong, with both force kernels."""
     shape = [4, 4, 6, 4]
     g = mesh_amd.Grid(shape)
     ctx.set_option("fuse_md", fuse)
+    ctx.set_option("force_variant", variant)
     try:
         hmc = gauge.Hmc(g, prec, seed=5, ctx=ctx)
         hmc.randomizeGaugeField()
@@ -252,6 +254,7 @@ def test_hmc_fused_md_bitexact(ctx, prec, scheme, n, fuse):
             np.testing.assert_array_equal(hmc.U.download(), U)
     finally:
         ctx.set_option("fuse_md", 1)
+        ctx.set_option("force_variant", -1)
 
 
 def test_f32_vs_f64_same_config(ctx):
