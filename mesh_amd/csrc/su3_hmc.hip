@@ -363,65 +363,90 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 	}
 	__syncthreads();
 
-	const int mu = threadIdx.x >> 6;
-	const long x = lane_site(g, base + (threadIdx.x & 63));
-	const int r = (int)(x - base) / L3, c3 = (int)(x - base) % L3;
-	int c[4];
-	coords<R>(g, x, c);
-	// matrix of direction d at the site displaced by (dx2, dx3) from x, from LDS
-	auto from_lds = [&](int d, int dx2, int dx3) {
-		int cc = c3 + dx3;
-		cc = cc < 0 ? cc + L3 : (cc >= L3 ? cc - L3 : cc);
-		m3<R> m;
+	// the fused kernel gets the direction as a compile-time constant (no branch
+	// between staples); the separate momentum step keeps it a wave-uniform
+	// value (with constants the compiler spends all registers on it)
+	auto compute = [&](auto mu_c) {
+		const int mu = (int)mu_c;
+		const long x = lane_site(g, base + (threadIdx.x & 63));
+		const int r = (int)(x - base) / L3, c3 = (int)(x - base) % L3;
+		int c[4];
+		coords<R>(g, x, c);
+		// matrix of direction d at the site displaced by (dx2, dx3) from x, from LDS
+		auto from_lds = [&](int d, int dx2, int dx3) {
+			int cc = c3 + dx3;
+			cc = cc < 0 ? cc + L3 : (cc >= L3 ? cc - L3 : cc);
+			m3<R> m;
+#pragma unroll
+			for (int k = 0; k < 9; ++k)
+			{
+				vec2_t<R> v = lds[r + 1 + dx2][d][k][cc];
+				m.e[k] = {v.x, v.y};
+			}
+			return m;
+		};
+		// displacement of +/-1 along direction a as (dx2, dx3); a < 2 is non-local
+		auto d2 = [](int a, int s) { return a == 2 ? s : 0; };
+		auto d3 = [](int a, int s) { return a == 3 ? s : 0; };
+
+		m3<R> S = zero_m3<R>();
+		const bool mloc = mu >= 2;
+		long xpm = shift_c(g, x, mu, c[mu], 1);
+#pragma unroll
+		for (int nu = 0; nu < 4; ++nu)
+		{
+			if (nu == mu)
+				continue;
+			const bool nloc = nu >= 2;
+			long xpn = shift_c(g, x, nu, c[nu], 1);
+			long xmn = shift_c(g, x, nu, c[nu], -1);
+			{
+				m3<R> a = mloc ? from_lds(nu, d2(mu, 1), d3(mu, 1)) : load(U, nu, g, xpm);
+				m3<R> b = nloc ? from_lds(mu, d2(nu, 1), d3(nu, 1)) : load(U, mu, g, xpn);
+				m3<R> T1 = mul_nd(a, b);
+				madd_nd(S, T1, from_lds(nu, 0, 0));
+			}
+			{
+				m3<R> a = (mloc && nloc) ? from_lds(nu, d2(mu, 1) + d2(nu, -1), d3(mu, 1) + d3(nu, -1))
+				                         : load(U, nu, g, shift_c(g, xpm, nu, c[nu], -1));
+				m3<R> b = nloc ? from_lds(mu, d2(nu, -1), d3(nu, -1)) : load(U, mu, g, xmn);
+				m3<R> T2 = mul_dd(a, b);
+				madd_nn(S, T2, nloc ? from_lds(nu, d2(nu, -1), d3(nu, -1)) : load(U, nu, g, xmn));
+			}
+		}
+		m3<R> u = from_lds(mu, 0, 0);
+		m3<R> A = ta(mul_nn(u, S));
+		m3<R> p = load_once(P, mu, g, x);
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
 		{
-			vec2_t<R> v = lds[r + 1 + dx2][d][k][cc];
-			m.e[k] = {v.x, v.y};
+			p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
+			p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 		}
-		return m;
+		store_once(P, mu, g, x, p);
+		if constexpr (FUSE)
+			fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 	};
-	// displacement of +/-1 along direction a as (dx2, dx3); a < 2 is non-local
-	auto d2 = [](int a, int s) { return a == 2 ? s : 0; };
-	auto d3 = [](int a, int s) { return a == 3 ? s : 0; };
-
-	m3<R> S = zero_m3<R>();
-	const bool mloc = mu >= 2;
-	long xpm = shift_c(g, x, mu, c[mu], 1);
-#pragma unroll
-	for (int nu = 0; nu < 4; ++nu)
+	if constexpr (FUSE && MESH_STATIC_MU)
 	{
-		if (nu == mu)
-			continue;
-		const bool nloc = nu >= 2;
-		long xpn = shift_c(g, x, nu, c[nu], 1);
-		long xmn = shift_c(g, x, nu, c[nu], -1);
+		switch (threadIdx.x >> 6)
 		{
-			m3<R> a = mloc ? from_lds(nu, d2(mu, 1), d3(mu, 1)) : load(U, nu, g, xpm);
-			m3<R> b = nloc ? from_lds(mu, d2(nu, 1), d3(nu, 1)) : load(U, mu, g, xpn);
-			m3<R> T1 = mul_nd(a, b);
-			madd_nd(S, T1, from_lds(nu, 0, 0));
-		}
-		{
-			m3<R> a = (mloc && nloc) ? from_lds(nu, d2(mu, 1) + d2(nu, -1), d3(mu, 1) + d3(nu, -1))
-			                         : load(U, nu, g, shift_c(g, xpm, nu, c[nu], -1));
-			m3<R> b = nloc ? from_lds(mu, d2(nu, -1), d3(nu, -1)) : load(U, mu, g, xmn);
-			m3<R> T2 = mul_dd(a, b);
-			madd_nn(S, T2, nloc ? from_lds(nu, d2(nu, -1), d3(nu, -1)) : load(U, nu, g, xmn));
+		case 0:
+			compute(std::integral_constant<int, 0>{});
+			break;
+		case 1:
+			compute(std::integral_constant<int, 1>{});
+			break;
+		case 2:
+			compute(std::integral_constant<int, 2>{});
+			break;
+		default:
+			compute(std::integral_constant<int, 3>{});
+			break;
 		}
 	}
-	m3<R> u = from_lds(mu, 0, 0);
-	m3<R> A = ta(mul_nn(u, S));
-	m3<R> p = load_once(P, mu, g, x);
-#pragma unroll
-	for (int k = 0; k < 9; ++k)
-	{
-		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
-		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
-	}
-	store_once(P, mu, g, x, p);
-	if constexpr (FUSE)
-		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
+	else
+		compute((int)(threadIdx.x >> 6));
 }
 
 // variant: one thread per site looping over the directions
