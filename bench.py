@@ -42,12 +42,13 @@ HBM_PEAK_GBS = 8000.0
 #   link_update:  read U, P; write U'           216 R
 #   delta_h:      read U, U', P, K(8 B)         216 R + 8
 #   momenta:      write P, K                     72 R + 8
+#   momenta_link: write P, K; read U; write U'  216 R + 8
 def kernel_bytes(name, R):
     return {"force_update": 216 * R, "force_link": 288 * R, "link_update": 216 * R,
-            "delta_h": 216 * R + 8, "momenta": 72 * R + 8}[name]
+            "delta_h": 216 * R + 8, "momenta": 72 * R + 8, "momenta_link": 216 * R + 8}[name]
 
 
-KERNELS = ("force_link", "force_update", "link_update", "delta_h", "momenta")
+KERNELS = ("force_link", "force_update", "link_update", "delta_h", "momenta", "momenta_link")
 
 
 def trajectory_bytes_per_site(kern, R, steps):

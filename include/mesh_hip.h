@@ -105,8 +105,9 @@ void *mesh_stream(mesh_ctx *ctx);
  *       tile;
  *   "fuse_md" = 1 (default): in mesh_su3_hmc_trajectory each momentum step
  *       and the link step after it run as one kernel (the thread that
- *       updates P_mu(x) writes exp(t P_mu(x)) U_mu(x) into a second field);
- *       same arithmetic, so the same bits. */
+ *       updates P_mu(x) writes exp(t P_mu(x)) U_mu(x) into a second field),
+ *       and so do the momenta refresh and the first link step; same
+ *       arithmetic, so the same bits. */
 int mesh_set_option(mesh_ctx *ctx, const char *name, long value);
 /* Per-kernel timing (replaces the util::Stopwatch globals swStaples, swExp,
  * swPlaquette ... of src/gauge/utils.h:23, src/gauge/hmc.h:16, printed at
@@ -114,7 +115,8 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value);
  * kernels are bracketed by HIP events on the context stream and resolved at
  * the next synchronisation.  Kernel names: "force_update", "link_update",
  * "delta_h", "momenta", "plaquette", "landau_or", "force_link" (fused
- * momentum + link step). */
+ * momentum + link step), "momenta_link" (momenta refresh fused with the
+ * first link step of a trajectory). */
 int mesh_profile_enable(mesh_ctx *ctx, int on);
 int mesh_profile_reset(mesh_ctx *ctx);
 int mesh_profile_read(mesh_ctx *ctx, const char *kernel, double *total_ms, long long *count);

@@ -143,7 +143,8 @@ int main(int argc, char **argv)
 		ctx().synchronize();
 		double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_total).count();
 		std::printf("========== performance summary ==========\n");
-		for (char const *k : {"force_update", "link_update", "delta_h", "momenta", "landau_or"})
+		for (char const *k : {"force_link", "momenta_link", "force_update", "link_update", "delta_h", "momenta",
+		                      "landau_or"})
 		{
 			double ms;
 			long long n;

@@ -16,6 +16,7 @@ enum mesh_prof_kind
 	PROF_PLAQ,
 	PROF_LANDAU_OR,
 	PROF_FORCE_LINK,
+	PROF_MOMENTA_LINK,
 	PROF_NKINDS
 };
 

@@ -126,7 +126,8 @@ template <int NQ> __global__ __launch_bounds__(1024) void k_reduce_partials(cons
 }
 
 static const char *const PROF_NAMES[PROF_NKINDS] = {"force_update", "link_update", "delta_h",   "momenta",
-                                                     "plaquette",    "landau_or",   "force_link"};
+                                                     "plaquette",    "landau_or",   "force_link",
+                                                     "momenta_link"};
 
 int mesh_prof_begin(mesh_ctx *ctx, int kind)
 {

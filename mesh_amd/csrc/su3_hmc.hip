@@ -199,6 +199,40 @@ template <class R> __global__ __launch_bounds__(256) void k_momenta(Geo g, R *P,
 		K[x] = k;
 }
 
+// momenta refresh fused with the first link step of the trajectory:
+// P_mu(x) drawn as in k_momenta, then Uout_mu(x) = exp(t TA(P_mu(x))) Uin_mu(x)
+// as in k_link_update (the same operations, so the same bits)
+template <class R>
+__global__ __launch_bounds__(256) void k_momenta_link(Geo g, R *__restrict__ P, double *__restrict__ K, uint64_t seed,
+                                                      uint32_t traj, R *__restrict__ Uout, const R *__restrict__ Uin,
+                                                      R t, int reunit_flag)
+{
+	long x = lane_site(g, (long)blockIdx.x * blockDim.x + threadIdx.x);
+	if (x >= g.V)
+		return;
+	double k = 0.0;
+	for (int mu = 0; mu < g.nd; ++mu)
+	{
+		double z[8];
+		link_normals<8>(seed, TAG_MOM, traj, (uint32_t)x, (uint32_t)mu, z);
+		m3<R> p = from_f64<R>(algebra_from_normals(z));
+		store_once(P, mu, g, x, p);
+		k += norm2_m3(to_f64(p));
+		m3<R> X = ta(p);
+#pragma unroll
+		for (int e = 0; e < 9; ++e)
+		{
+			X.e[e].re *= t;
+			X.e[e].im *= t;
+		}
+		m3<R> u = mul_nn(expsu3(X), load_once(Uin, mu, g, x));
+		if (reunit_flag)
+			u = reunit(u);
+		store_once(Uout, mu, g, x, u);
+	}
+	K[x] = k;
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void k_plaq(Geo g, const R *__restrict__ U, double *partials, int nb, int remap)
 {
@@ -676,6 +710,22 @@ int launch_momenta(mesh_ctx *ctx, Geo const &geo, int prec, void *P, double *K, 
 	return MESH_OK;
 }
 
+int launch_momenta_link(mesh_ctx *ctx, Geo const &geo, int prec, void *P, double *K, uint64_t seed, uint32_t traj,
+                        void *Uout, const void *Uin, double t, int reunit_flag)
+{
+	int nb = nblk(geo.V, 256);
+	int ph = mesh_prof_begin(ctx, PROF_MOMENTA_LINK);
+	if (prec == MESH_F64)
+		k_momenta_link<double><<<nb, 256, 0, ctx->stream>>>(geo, (double *)P, K, seed, traj, (double *)Uout,
+		                                                    (const double *)Uin, t, reunit_flag);
+	else
+		k_momenta_link<float><<<nb, 256, 0, ctx->stream>>>(geo, (float *)P, K, seed, traj, (float *)Uout,
+		                                                   (const float *)Uin, (float)t, reunit_flag);
+	mesh_prof_end(ctx, ph);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
 int delta_h(mesh_ctx *ctx, Geo const &geo, int prec, const void *Uo, const void *Un, const void *P, const double *Ko,
             double beta, double *out)
 {
@@ -962,9 +1012,9 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 	if (*U == *U_work)
 		return mesh_fail(ctx, MESH_E_ARG, "U and U_work must differ");
 	int rc;
-	if ((rc = launch_momenta(ctx, s.geo, prec, P, K, seed, traj)))
-		return rc;
 	const int nfused = (nd_ - 1) / 2;
+	if (!(ctx->fuse_md && nfused > 0) && (rc = launch_momenta(ctx, s.geo, prec, P, K, seed, traj)))
+		return rc;
 	if (ctx->fuse_md && nfused > 0)
 	{
 		// momentum step i fused with link step i+1, ping-pong between the
@@ -986,7 +1036,7 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 		}
 		void *cur = (nfused % 2) ? ctx->md_spare : *U_work;
 		void *other = (nfused % 2) ? *U_work : ctx->md_spare;
-		if ((rc = launch_link_update(ctx, s.geo, s.geo.nd, prec, cur, *U, P, deltas[0], 0)))
+		if ((rc = launch_momenta_link(ctx, s.geo, prec, P, K, seed, traj, cur, *U, deltas[0], 0)))
 			return rc;
 		for (int i = 1; i + 1 < nd_; i += 2)
 		{
