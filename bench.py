@@ -313,7 +313,7 @@ def main():
     ap.add_argument("--u1-L", type=int, default=32)
     ap.add_argument("--u1-beta", type=float, default=6.0)
     ap.add_argument("--u1-batch", type=int, default=256)
-    ap.add_argument("--u1-train", type=int, default=2000)
+    ap.add_argument("--u1-train", type=int, default=4000)
     ap.add_argument("--u1-hmc-steps", type=int, default=4000)
     ap.add_argument("--u1-flow-steps", type=int, default=1000)
     ap.add_argument("--u1-flows", default="pflow,coupling", help="flows to measure (pflow, coupling)")
