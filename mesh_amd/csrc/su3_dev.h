@@ -470,7 +470,16 @@ template <class R> MD void store_stream(R *__restrict__ base, int f, Geo const &
 	}
 }
 
+// site indices are < 2^26 (mesh_make_geo), so 32-bit division suffices (the
+// 64-bit one is a long emulated sequence)
+#ifndef MESH_COORD32
+#define MESH_COORD32 1
+#endif
+#if MESH_COORD32
+MD int coord(Geo const &g, long x, int mu) { return (int)(((unsigned)x / (unsigned)g.st[mu]) % (unsigned)g.L[mu]); }
+#else
 MD int coord(Geo const &g, long x, int mu) { return (int)((x / g.st[mu]) % g.L[mu]); }
+#endif
 MD long shift_c(Geo const &g, long x, int mu, int c, int s)
 {
 	int n = c + s;

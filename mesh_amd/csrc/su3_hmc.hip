@@ -32,6 +32,16 @@ using namespace mesh_dev;
 #ifndef MESH_STREAM_HINTS
 #define MESH_STREAM_HINTS 1
 #endif
+// waves per SIMD the Delta H kernel is limited to (two fields with neighbours
+// in flight: more waves thrash the L2; 0 = the compiler's choice)
+#ifndef MESH_DH_WAVES
+#define MESH_DH_WAVES 2
+#endif
+#if MESH_DH_WAVES > 0
+#define DH_OCC __attribute__((amdgpu_waves_per_eu(1, MESH_DH_WAVES)))
+#else
+#define DH_OCC
+#endif
 // staples with the direction as a compile-time constant (4-d lattices)
 #ifndef MESH_STATIC_MU
 #define MESH_STATIC_MU 1
@@ -57,11 +67,29 @@ __device__ __forceinline__ void store_once(R *__restrict__ b, int f, Geo const &
 
 constexpr int SB = 64; // sites per workgroup in the stencil kernels (one wavefront)
 
+// all coordinates of site x: three 32-bit divisions, fastest dimension first
 template <class R> __device__ __forceinline__ void coords(Geo const &g, long x, int *c)
 {
+#if !MESH_COORD32
 #pragma unroll
 	for (int mu = 0; mu < 4; ++mu)
 		c[mu] = mu < g.nd ? coord(g, x, mu) : 0;
+	return;
+#endif
+	unsigned t = (unsigned)x;
+#pragma unroll
+	for (int mu = 3; mu >= 1; --mu)
+	{
+		if (mu < g.nd)
+		{
+			unsigned q = t / (unsigned)g.L[mu];
+			c[mu] = (int)(t - q * (unsigned)g.L[mu]);
+			t = q;
+		}
+		else
+			c[mu] = 0;
+	}
+	c[0] = (int)t;
 }
 
 // staple sum of direction mu at site x with coordinates c
@@ -252,7 +280,7 @@ __global__ __launch_bounds__(256) void k_plaq(Geo g, const R *__restrict__ U, do
 
 // partial sums of (d(x), ps_old(x), ps_new(x), k_new(x), k_old(x))
 template <class R>
-__global__ __launch_bounds__(256) void k_delta_h(Geo g, const R *__restrict__ Uo, const R *__restrict__ Un,
+__global__ __launch_bounds__(256) DH_OCC void k_delta_h(Geo g, const R *__restrict__ Uo, const R *__restrict__ Un,
                                                  const R *__restrict__ P, const double *__restrict__ Ko,
                                                  double c3, double *partials, int nb, int remap)
 {
