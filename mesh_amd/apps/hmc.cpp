@@ -94,6 +94,10 @@ int main(int argc, char **argv)
 			therm = std::atoi(next().c_str());
 		else if (a == "--landau")
 			landau = true;
+		else if (a == "--plot")
+			std::fprintf(stderr, "mesh_hmc: --plot (gnuplot of the plaquette history) is not supported; ignored\n");
+		else if (a == "--filename" || a == "--force")
+			usage((a + ": HDF5 output of configurations is not supported by this build").c_str());
 		else if (a == "--help" || a == "-h")
 			usage("help requested");
 		else
