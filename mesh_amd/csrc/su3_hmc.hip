@@ -42,6 +42,10 @@ using namespace mesh_dev;
 #else
 #define DH_OCC
 #endif
+// variant 0 stages the workgroup's own-site links in LDS (4-d, V % 64 == 0)
+#ifndef MESH_OWN_LDS
+#define MESH_OWN_LDS 1
+#endif
 // staples with the direction as a compile-time constant (4-d lattices)
 #ifndef MESH_STATIC_MU
 #define MESH_STATIC_MU 1
@@ -138,6 +142,44 @@ __device__ __forceinline__ m3<R> staple_4d(Geo const &g, const R *__restrict__ U
 		{
 			m3<R> T1 = mul_nd(load(U, nu, g, xpm), load(U, MU, g, xpn));
 			madd_nd(S, T1, load(U, nu, g, x));
+		}
+		{
+			m3<R> T2 = mul_dd(load(U, nu, g, xpmmn), load(U, MU, g, xmn));
+			madd_nn(S, T2, load(U, nu, g, xmn));
+		}
+	}
+	return S;
+}
+
+// The same with the four links of site x read from LDS (own[d][k][lane],
+// staged once by the workgroup instead of once per direction's wavefront)
+template <int MU, class R>
+__device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict__ U, long x, int const *c,
+                                               vec2_t<R> (*own)[9][64], int lane)
+{
+	auto own_link = [&](int d) {
+		m3<R> m;
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+		{
+			vec2_t<R> v = own[d][k][lane];
+			m.e[k] = {v.x, v.y};
+		}
+		return m;
+	};
+	m3<R> S = zero_m3<R>();
+	long xpm = shift_c(g, x, MU, c[MU], 1);
+#pragma unroll
+	for (int nu = 0; nu < 4; ++nu)
+	{
+		if (nu == MU)
+			continue;
+		long xpn = shift_c(g, x, nu, c[nu], 1);
+		long xmn = shift_c(g, x, nu, c[nu], -1);
+		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
+		{
+			m3<R> T1 = mul_nd(load(U, nu, g, xpm), load(U, MU, g, xpn));
+			madd_nd(S, T1, own_link(nu));
 		}
 		{
 			m3<R> T2 = mul_dd(load(U, nu, g, xpmmn), load(U, MU, g, xmn));
@@ -369,12 +411,53 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 		return;
 	int mu = threadIdx.x >> 6;
 	long x = lane_site(g, (long)lb * SB + (threadIdx.x & 63));
-	if (mu >= g.nd || x >= g.V)
-		return;
-	int c[4];
-	coords<R>(g, x, c);
-	m3<R> S = MESH_STATIC_MU ? staple_any(g, U, mu, x, c) : staple_at(g, U, mu, x, c);
-	m3<R> u = load(U, mu, g, x);
+	m3<R> S, u;
+	if (MESH_OWN_LDS && g.nd == 4 && (g.V & 63) == 0)
+	{
+		// every thread of the workgroup is a valid (site, direction): stage the
+		// block's own links, then the staples read them from LDS
+		__shared__ vec2_t<R> own[4][9][64];
+		const int lane = threadIdx.x & 63;
+		{
+			const vec2_t<R> *p = reinterpret_cast<const vec2_t<R> *>(U) + site_off(g, mu, x);
+#pragma unroll
+			for (int k = 0; k < 9; ++k)
+				own[mu][k][lane] = p[k * plane_step(g)];
+		}
+		__syncthreads();
+		int c[4];
+		coords<R>(g, x, c);
+		switch (mu)
+		{
+		case 0:
+			S = staple_4d_own<0>(g, U, x, c, own, lane);
+			break;
+		case 1:
+			S = staple_4d_own<1>(g, U, x, c, own, lane);
+			break;
+		case 2:
+			S = staple_4d_own<2>(g, U, x, c, own, lane);
+			break;
+		default:
+			S = staple_4d_own<3>(g, U, x, c, own, lane);
+			break;
+		}
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+		{
+			vec2_t<R> v = own[mu][k][lane];
+			u.e[k] = {v.x, v.y};
+		}
+	}
+	else
+	{
+		if (mu >= g.nd || x >= g.V)
+			return;
+		int c[4];
+		coords<R>(g, x, c);
+		S = MESH_STATIC_MU ? staple_any(g, U, mu, x, c) : staple_at(g, U, mu, x, c);
+		u = load(U, mu, g, x);
+	}
 	m3<R> A = ta(mul_nn(u, S));
 	m3<R> p = load_once(P, mu, g, x);
 #pragma unroll
