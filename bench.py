@@ -13,8 +13,9 @@ beside it.  All inputs are resident in HBM when the timed region starts.
 
 Roofline: the dominant kernel is the fused molecular-dynamics step
 "force_link" (staples + TA + momentum step, then exp(t P) U into the second
-field).  Its algorithmic traffic is 288 reals per site (read U and P, write P
-and U': 2304 B FP64, 1152 B FP32); its average duration comes from HIP events
+field).  Its algorithmic traffic is 216 reals per site (read U and P, write P
+and U', with the momenta in the trajectory's compact antihermitian layout of 9
+reals per link: 1728 B FP64, 864 B FP32); its average duration comes from HIP events
 recorded around every launch on the library's stream during the timed region.
 Per-site algorithmic bytes of every kernel are in kernel_bytes(); the
 trajectory's total is the sum over the launches actually made.
@@ -36,16 +37,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-# algorithmic bytes per site of each timed kernel, R = bytes per real:
-#   force_update: read U, P; write P            216 R
-#   force_link:   read U, P; write P, U'        288 R
-#   link_update:  read U, P; write U'           216 R
-#   delta_h:      read U, U', P, K(8 B)         216 R + 8
-#   momenta:      write P, K                     72 R + 8
-#   momenta_link: write P, K; read U; write U'  216 R + 8
+# algorithmic bytes per site of each kernel as a trajectory runs it, R = bytes
+# per real; a link is 18 reals, a momentum 18 in the matrix layout (separate
+# steps) and 9 in the compact antihermitian one (fused path, the default):
+#   force_update: read U, P; write P            216 R  (matrix P)
+#   force_link:   read U, P; write P, U'        216 R  (compact P)
+#   link_update:  read U, P; write U'           216 R  (matrix P)
+#   delta_h:      read U, U', P, K(8 B)         180 R + 8  (compact P)
+#   momenta:      write P, K                     72 R + 8  (matrix P)
+#   momenta_link: write P, K; read U; write U'  180 R + 8  (compact P)
 def kernel_bytes(name, R):
-    return {"force_update": 216 * R, "force_link": 288 * R, "link_update": 216 * R,
-            "delta_h": 216 * R + 8, "momenta": 72 * R + 8, "momenta_link": 216 * R + 8}[name]
+    return {"force_update": 216 * R, "force_link": 216 * R, "link_update": 216 * R,
+            "delta_h": 180 * R + 8, "momenta": 72 * R + 8, "momenta_link": 180 * R + 8}[name]
 
 
 KERNELS = ("force_link", "force_update", "link_update", "delta_h", "momenta", "momenta_link")

@@ -225,7 +225,10 @@ int mesh_make_deltas(const char *scheme, double epsilon, int substeps, double *o
 /* One HMC trajectory: runHmcUpdate (src/gauge/hmc.cpp:75-107).
  *   *U      current field (device), *U_work a second field of the same size;
  *           on acceptance the two pointers are swapped (std::swap, hmc.cpp:96).
- *   P       momentum field, K_sites V doubles (device), both workspace.
+ *   P       momentum field, K_sites V doubles (device), both workspace
+ *           (P needs mesh_su3_field_bytes(g, Nd, prec); the fused path keeps
+ *           it in a compact antihermitian layout of 9 reals per link, so its
+ *           contents on return are not a field in the layout above).
  *   metropolis == 0 accepts unconditionally (thermalisation).
  * out[6] = {Delta H, accepted, plaquette of the resulting U, H_old, H_new, u}.
  * The Metropolis uniform u is Philox(seed, traj); accept iff u < exp(-Delta H). */

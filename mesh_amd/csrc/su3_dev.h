@@ -470,6 +470,87 @@ template <class R> MD void store_stream(R *__restrict__ base, int f, Geo const &
 	}
 }
 
+// Compact field of exactly antihermitian matrices (the momenta inside one HMC
+// trajectory): 9 reals per matrix instead of 18.  Per block of B sites, in
+// the slot order of site_off: planes 0-2 hold the complex upper entries
+// (0,1), (0,2), (1,2), plane 3 the pair (Im m00, Im m11), then a half plane
+// of Im m22; direction f starts at 9 * f * S reals.  Unpacking sets the lower
+// entries to -conj of the upper ones and the diagonal real parts to zero, so a
+// matrix with these symmetries in every bit (momenta from
+// algebra_from_normals, updated by fma with ta-projected forces, which keep
+// them) comes back bit for bit, up to the sign of zero entries.
+MD long alg_block(Geo const &g, int f, long x)
+{
+	long o = (long)(f * 9) * g.stride;
+	if (MESH_SITE_LB)
+		o += ((x >> MESH_SITE_LB) * 9) << MESH_SITE_LB;
+	return o;
+}
+MD long alg_slot(long x)
+{
+	if (!MESH_SITE_LB)
+		return x;
+	long l = x & ((1L << MESH_SITE_LB) - 1);
+	return MESH_SITE_PAIRS ? (((l & 1) << (MESH_SITE_LB - 1)) | (l >> 1)) : l;
+}
+template <class R, bool NT> MD m3<R> load_alg(const R *__restrict__ base, int f, Geo const &g, long x)
+{
+	const R *b = base + alg_block(g, f, x);
+	const long l = alg_slot(x), ks = plane_step(g);
+	const nt2_t<R> *p = reinterpret_cast<const nt2_t<R> *>(b) + l;
+	nt2_t<R> v[4];
+	R d2;
+	if constexpr (NT)
+	{
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			v[k] = __builtin_nontemporal_load(p + k * ks);
+		d2 = __builtin_nontemporal_load(b + 8 * ks + l);
+	}
+	else
+	{
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			v[k] = p[k * ks];
+		d2 = b[8 * ks + l];
+	}
+	m3<R> m;
+	m.e[1] = {v[0].x, v[0].y};
+	m.e[3] = {-v[0].x, v[0].y};
+	m.e[2] = {v[1].x, v[1].y};
+	m.e[6] = {-v[1].x, v[1].y};
+	m.e[5] = {v[2].x, v[2].y};
+	m.e[7] = {-v[2].x, v[2].y};
+	m.e[0] = {R(0), v[3].x};
+	m.e[4] = {R(0), v[3].y};
+	m.e[8] = {R(0), d2};
+	return m;
+}
+template <class R, bool NT> MD void store_alg(R *__restrict__ base, int f, Geo const &g, long x, m3<R> const &m)
+{
+	R *b = base + alg_block(g, f, x);
+	const long l = alg_slot(x), ks = plane_step(g);
+	nt2_t<R> *p = reinterpret_cast<nt2_t<R> *>(b) + l;
+	const nt2_t<R> v[4] = {{m.e[1].re, m.e[1].im},
+	                       {m.e[2].re, m.e[2].im},
+	                       {m.e[5].re, m.e[5].im},
+	                       {m.e[0].im, m.e[4].im}};
+	if constexpr (NT)
+	{
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			__builtin_nontemporal_store(v[k], p + k * ks);
+		__builtin_nontemporal_store(m.e[8].im, b + 8 * ks + l);
+	}
+	else
+	{
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			p[k * ks] = v[k];
+		b[8 * ks + l] = m.e[8].im;
+	}
+}
+
 // site indices are < 2^26 (mesh_make_geo), so 32-bit division suffices (the
 // 64-bit one is a long emulated sequence)
 #ifndef MESH_COORD32

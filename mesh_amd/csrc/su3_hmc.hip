@@ -69,6 +69,25 @@ __device__ __forceinline__ void store_once(R *__restrict__ b, int f, Geo const &
 		store(b, f, g, x, m);
 }
 
+// momenta of the trajectory's fused path: compact antihermitian layout
+// (load_alg / store_alg) when PC, else the full matrix layout; streamed
+template <bool PC, class R>
+__device__ __forceinline__ m3<R> load_mom(const R *__restrict__ P, int f, Geo const &g, long x)
+{
+	if constexpr (PC)
+		return load_alg<R, MESH_STREAM_HINTS != 0>(P, f, g, x);
+	else
+		return load_once(P, f, g, x);
+}
+template <bool PC, class R>
+__device__ __forceinline__ void store_mom(R *__restrict__ P, int f, Geo const &g, long x, m3<R> const &m)
+{
+	if constexpr (PC)
+		store_alg<R, MESH_STREAM_HINTS != 0>(P, f, g, x, m);
+	else
+		store_once(P, f, g, x, m);
+}
+
 constexpr int SB = 64; // sites per workgroup in the stencil kernels (one wavefront)
 
 // all coordinates of site x: three 32-bit divisions, fastest dimension first
@@ -227,11 +246,11 @@ template <class R> __device__ __forceinline__ double plaq_site(Geo const &g, con
 	return s;
 }
 
-template <class R> __device__ __forceinline__ double kin_site(Geo const &g, const R *__restrict__ P, long x)
+template <bool PC, class R> __device__ __forceinline__ double kin_site(Geo const &g, const R *__restrict__ P, long x)
 {
 	double k = 0.0;
 	for (int mu = 0; mu < g.nd; ++mu)
-		k += norm2_m3(to_f64(load(P, mu, g, x)));
+		k += norm2_m3(to_f64(PC ? load_alg<R, false>(P, mu, g, x) : load(P, mu, g, x)));
 	return k;
 }
 
@@ -271,8 +290,9 @@ template <class R> __global__ __launch_bounds__(256) void k_momenta(Geo g, R *P,
 
 // momenta refresh fused with the first link step of the trajectory:
 // P_mu(x) drawn as in k_momenta, then Uout_mu(x) = exp(t TA(P_mu(x))) Uin_mu(x)
-// as in k_link_update (the same operations, so the same bits)
-template <class R>
+// as in k_link_update (the same operations, so the same bits); PC: P in the
+// compact antihermitian layout
+template <class R, bool PC>
 __global__ __launch_bounds__(256) void k_momenta_link(Geo g, R *__restrict__ P, double *__restrict__ K, uint64_t seed,
                                                       uint32_t traj, R *__restrict__ Uout, const R *__restrict__ Uin,
                                                       R t, int reunit_flag)
@@ -286,7 +306,7 @@ __global__ __launch_bounds__(256) void k_momenta_link(Geo g, R *__restrict__ P, 
 		double z[8];
 		link_normals<8>(seed, TAG_MOM, traj, (uint32_t)x, (uint32_t)mu, z);
 		m3<R> p = from_f64<R>(algebra_from_normals(z));
-		store_once(P, mu, g, x, p);
+		store_mom<PC>(P, mu, g, x, p);
 		k += norm2_m3(to_f64(p));
 		m3<R> X = ta(p);
 #pragma unroll
@@ -320,8 +340,9 @@ __global__ __launch_bounds__(256) void k_plaq(Geo g, const R *__restrict__ U, do
 	block_sum_store<1>(v, partials, nb, lb);
 }
 
-// partial sums of (d(x), ps_old(x), ps_new(x), k_new(x), k_old(x))
-template <class R>
+// partial sums of (d(x), ps_old(x), ps_new(x), k_new(x), k_old(x)); PC: P in
+// the compact antihermitian layout
+template <class R, bool PC>
 __global__ __launch_bounds__(256) DH_OCC void k_delta_h(Geo g, const R *__restrict__ Uo, const R *__restrict__ Un,
                                                  const R *__restrict__ P, const double *__restrict__ Ko,
                                                  double c3, double *partials, int nb, int remap)
@@ -337,7 +358,7 @@ __global__ __launch_bounds__(256) DH_OCC void k_delta_h(Geo g, const R *__restri
 		coords<R>(g, x, c);
 		double pso = plaq_site(g, Uo, x, c);
 		double psn = plaq_site(g, Un, x, c);
-		double kn = kin_site(g, P, x);
+		double kn = kin_site<PC>(g, P, x);
 		double dps = psn - pso;
 		double dk = kn - Ko[x];
 		v[0] = fma(-c3, dps, dk);
@@ -400,7 +421,8 @@ __device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Un
 // fused momentum step: workgroup = 64 sites x 4 directions, wavefront w
 // handles direction w of the workgroup's sites.  With Unext != nullptr the
 // following link step is fused in: Unext = exp(tl TA(P_new)) U (Unext != U,
-// since neighbouring threads still read U).
+// since neighbouring threads still read U), and P is in the compact
+// antihermitian layout (the fused kernels run only inside a trajectory).
 template <class R, bool FUSE>
 __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
                                                                 int nb, int remap, R *__restrict__ Unext, R tl,
@@ -459,14 +481,14 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 		u = load(U, mu, g, x);
 	}
 	m3<R> A = ta(mul_nn(u, S));
-	m3<R> p = load_once(P, mu, g, x);
+	m3<R> p = load_mom<FUSE>(P, mu, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
-	store_once(P, mu, g, x, p);
+	store_mom<FUSE>(P, mu, g, x, p);
 	if constexpr (FUSE)
 		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 }
@@ -477,7 +499,7 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 // all four link directions of those rows plus one halo row on each side in
 // LDS; every staple link displaced only along x2/x3 is then read from LDS and
 // only the x0/x1-displaced ones from global memory.  Wavefront w handles
-// direction w; the arithmetic is exactly that of k_force_update.
+// direction w; the arithmetic (and the layout of P) is that of k_force_update.
 template <class R, int L3, bool FUSE>
 __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, const R *__restrict__ U, R *__restrict__ P,
                                                                      R kc, int nb, int remap, R *__restrict__ Unext,
@@ -561,14 +583,14 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 		}
 		m3<R> u = from_lds(mu, 0, 0);
 		m3<R> A = ta(mul_nn(u, S));
-		m3<R> p = load_once(P, mu, g, x);
+		m3<R> p = load_mom<FUSE>(P, mu, g, x);
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
 		{
 			p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
 			p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 		}
-		store_once(P, mu, g, x, p);
+		store_mom<FUSE>(P, mu, g, x, p);
 		if constexpr (FUSE)
 			fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 	};
@@ -721,9 +743,17 @@ double n_plaquettes(Geo const &geo) { return (double)geo.V * geo.nd * (geo.nd - 
 int launch_link_update(mesh_ctx *ctx, Geo const &geo, int nf, int prec, void *Uout, const void *Uin, const void *P,
                        double t, int reunit_flag);
 
+// force kernel variant in use: 0 wavefront per direction, 1 thread per site,
+// 2 LDS tile (falls back to 0 where the tile does not fit)
+int force_variant_of(mesh_ctx *ctx, int prec)
+{
+	return ctx->force_variant >= 0 ? ctx->force_variant : (prec == MESH_F32 ? 2 : 0);
+}
+
 // momentum step P -= delta beta/6 TA(U Sigma); with Unext != nullptr the
 // link step Unext = exp(tl TA(P)) U (+ reunit) follows, fused into the same
-// kernel where the variant supports it (Unext must differ from U)
+// kernel where the variant supports it (Unext must differ from U; P is then
+// in the compact antihermitian layout, see force_variant_of != 1)
 int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, void *P, double beta, double delta,
                         void *Unext = nullptr, double tl = 0.0, int reunit_flag = 0)
 {
@@ -734,7 +764,7 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 	bool fused = Unext != nullptr;
 	// default: FP64 is bound by HBM traffic and register pressure (wavefront
 	// per direction); FP32 by issue, where the LDS tile's fewer loads pay
-	const int variant = ctx->force_variant >= 0 ? ctx->force_variant : (prec == MESH_F32 ? 2 : 0);
+	const int variant = force_variant_of(ctx, prec);
 	if (variant == 2 && tile_ok)
 	{
 		int nb = (int)(geo.V / 64);
@@ -822,23 +852,28 @@ int launch_momenta(mesh_ctx *ctx, Geo const &geo, int prec, void *P, double *K, 
 }
 
 int launch_momenta_link(mesh_ctx *ctx, Geo const &geo, int prec, void *P, double *K, uint64_t seed, uint32_t traj,
-                        void *Uout, const void *Uin, double t, int reunit_flag)
+                        void *Uout, const void *Uin, double t, int reunit_flag, bool compact)
 {
 	int nb = nblk(geo.V, 256);
 	int ph = mesh_prof_begin(ctx, PROF_MOMENTA_LINK);
+	auto launch = [&](auto tag, auto pc) {
+		using Rt = decltype(tag);
+		k_momenta_link<Rt, decltype(pc)::value><<<nb, 256, 0, ctx->stream>>>(
+		    geo, (Rt *)P, K, seed, traj, (Rt *)Uout, (const Rt *)Uin, (Rt)t, reunit_flag);
+	};
+	using PC1 = std::true_type;
+	using PC0 = std::false_type;
 	if (prec == MESH_F64)
-		k_momenta_link<double><<<nb, 256, 0, ctx->stream>>>(geo, (double *)P, K, seed, traj, (double *)Uout,
-		                                                    (const double *)Uin, t, reunit_flag);
+		compact ? launch(0.0, PC1{}) : launch(0.0, PC0{});
 	else
-		k_momenta_link<float><<<nb, 256, 0, ctx->stream>>>(geo, (float *)P, K, seed, traj, (float *)Uout,
-		                                                   (const float *)Uin, (float)t, reunit_flag);
+		compact ? launch(0.0f, PC1{}) : launch(0.0f, PC0{});
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
 
 int delta_h(mesh_ctx *ctx, Geo const &geo, int prec, const void *Uo, const void *Un, const void *P, const double *Ko,
-            double beta, double *out)
+            double beta, double *out, bool compact = false)
 {
 	int nb = nblk(geo.V, 256);
 	if (int rc = mesh_reserve_partials(ctx, 5 * (size_t)nb))
@@ -846,14 +881,17 @@ int delta_h(mesh_ctx *ctx, Geo const &geo, int prec, const void *Uo, const void 
 	int gb = mesh_grid_blocks(ctx, nb);
 	double c3 = beta / 3.0;
 	int ph = mesh_prof_begin(ctx, PROF_DELTA_H);
+	auto launch = [&](auto tag, auto pc) {
+		using Rt = decltype(tag);
+		k_delta_h<Rt, decltype(pc)::value><<<gb, 256, 0, ctx->stream>>>(
+		    geo, (const Rt *)Uo, (const Rt *)Un, (const Rt *)P, Ko, c3, ctx->partials, nb, mesh_order(ctx));
+	};
+	using PC1 = std::true_type;
+	using PC0 = std::false_type;
 	if (prec == MESH_F64)
-		k_delta_h<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)Uo, (const double *)Un,
-		                                               (const double *)P, Ko, c3, ctx->partials, nb,
-		                                               mesh_order(ctx));
+		compact ? launch(0.0, PC1{}) : launch(0.0, PC0{});
 	else
-		k_delta_h<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)Uo, (const float *)Un,
-		                                              (const float *)P, Ko, c3, ctx->partials, nb,
-		                                              mesh_order(ctx));
+		compact ? launch(0.0f, PC1{}) : launch(0.0f, PC0{});
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	double r[5];
@@ -1124,6 +1162,9 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 		return mesh_fail(ctx, MESH_E_ARG, "U and U_work must differ");
 	int rc;
 	const int nfused = (nd_ - 1) / 2;
+	// the fused kernels keep P in the compact antihermitian layout (half the
+	// bytes of a matrix field, the same bits)
+	const bool compact = ctx->fuse_md && nfused > 0 && force_variant_of(ctx, prec) != 1;
 	if (!(ctx->fuse_md && nfused > 0) && (rc = launch_momenta(ctx, s.geo, prec, P, K, seed, traj)))
 		return rc;
 	if (ctx->fuse_md && nfused > 0)
@@ -1147,7 +1188,7 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 		}
 		void *cur = (nfused % 2) ? ctx->md_spare : *U_work;
 		void *other = (nfused % 2) ? *U_work : ctx->md_spare;
-		if ((rc = launch_momenta_link(ctx, s.geo, prec, P, K, seed, traj, cur, *U, deltas[0], 0)))
+		if ((rc = launch_momenta_link(ctx, s.geo, prec, P, K, seed, traj, cur, *U, deltas[0], 0, compact)))
 			return rc;
 		for (int i = 1; i + 1 < nd_; i += 2)
 		{
@@ -1171,7 +1212,7 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 				return rc;
 		}
 	double dh[5];
-	if ((rc = delta_h(ctx, s.geo, prec, *U, *U_work, P, K, beta, dh)))
+	if ((rc = delta_h(ctx, s.geo, prec, *U, *U_work, P, K, beta, dh, compact)))
 		return rc;
 	double u = metropolis_uniform(seed, traj);
 	int acc = metropolis ? (u < std::exp(-dh[0])) : 1;
