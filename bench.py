@@ -39,24 +39,28 @@ HBM_PEAK_GBS = 8000.0
 
 # algorithmic bytes per site of each kernel as a trajectory runs it, R = bytes
 # per real; a link is 18 reals, a momentum 18 in the matrix layout (separate
-# steps) and 9 in the compact antihermitian one (fused path, the default):
-#   force_update: read U, P; write P            216 R  (matrix P)
-#   force_link:   read U, P; write P, U'        216 R  (compact P)
-#   link_update:  read U, P; write U'           216 R  (matrix P)
-#   delta_h:      read U, U', P, K(8 B)         180 R + 8  (compact P)
-#   momenta:      write P, K                     72 R + 8  (matrix P)
-#   momenta_link: write P, K; read U; write U'  180 R + 8  (compact P)
-def kernel_bytes(name, R):
-    return {"force_update": 216 * R, "force_link": 216 * R, "link_update": 216 * R,
-            "delta_h": 180 * R + 8, "momenta": 72 * R + 8, "momenta_link": 180 * R + 8}[name]
+# steps, or --matrix-momenta) and 9 in the compact antihermitian one (fused
+# path, the default):
+#                                                compact P   matrix P
+#   force_update: read U, P; write P                  -       216 R
+#   force_link:   read U, P; write P, U'            216 R     288 R
+#   link_update:  read U, P; write U'                 -       216 R
+#   delta_h:      read U, U', P, K(8 B)           180 R + 8  216 R + 8
+#   momenta:      write P, K                          -       72 R + 8
+#   momenta_link: write P, K; read U; write U'    180 R + 8  216 R + 8
+def kernel_bytes(name, R, compact=True):
+    if compact and name in ("force_link", "delta_h", "momenta_link"):
+        return {"force_link": 216 * R, "delta_h": 180 * R + 8, "momenta_link": 180 * R + 8}[name]
+    return {"force_update": 216 * R, "force_link": 288 * R, "link_update": 216 * R,
+            "delta_h": 216 * R + 8, "momenta": 72 * R + 8, "momenta_link": 216 * R + 8}[name]
 
 
 KERNELS = ("force_link", "force_update", "link_update", "delta_h", "momenta", "momenta_link")
 
 
-def trajectory_bytes_per_site(kern, R, steps):
+def trajectory_bytes_per_site(kern, R, steps, compact=True):
     """algorithmic bytes per site of one trajectory, from the launches made"""
-    return sum(kernel_bytes(k, R) * kern[k]["launches"] for k in KERNELS) / steps
+    return sum(kernel_bytes(k, R, compact) * kern[k]["launches"] for k in KERNELS) / steps
 
 
 def dominant(kern):
@@ -112,7 +116,10 @@ def run_gpu(prec, L, steps, warmup, therm, seed, scheme, eps, substeps):
 
 
 def same_config_compare(U64, seed, scheme, eps, substeps):
-    """One trajectory from the same FP64 config and momenta in both precisions."""
+    """One trajectory from the same FP64 config and momenta in both precisions:
+    plaquette and Delta H errors of FP32-mixed, with the error of Delta H split
+    into that of H at the start (the FP32-rounded field and momenta) and at the
+    end of the trajectory."""
     import mesh_amd
     from mesh_amd import F32, F64, gauge
 
@@ -122,9 +129,10 @@ def same_config_compare(U64, seed, scheme, eps, substeps):
         h = gauge.Hmc(U64.grid, prec, seed=seed + 1000, ctx=U64.ctx)
         h.U = U64.convert(prec)
         r = h.runHmcUpdate(6.0, deltas, metropolis=False)
-        out[prec] = (r["dH"], gauge.plaquette(h.U))
+        out[prec] = (r["dH"], gauge.plaquette(h.U), r["H_old"], r["H_new"])
     return dict(dplaq=abs(out[F32][1] - out[F64][1]), ddH=abs(out[F32][0] - out[F64][0]),
-                dH_f64=out[F64][0], plaq_f64=out[F64][1])
+                dH_f64=out[F64][0], plaq_f64=out[F64][1],
+                H_old_f32_minus_f64=out[F32][2] - out[F64][2], H_new_f32_minus_f64=out[F32][3] - out[F64][3])
 
 
 def cpu_baseline(L, seed, scheme, eps, substeps):
@@ -308,6 +316,9 @@ def main():
     ap.add_argument("--substeps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32-mixed run")
+    ap.add_argument("--matrix-momenta", action="store_true",
+                    help="keep the trajectory's momenta as 18-real matrices (option compact_momenta = 0; "
+                         "same bits, more traffic)")
     ap.add_argument("--landau", action="store_true",
                     help="instead: Landau gauge fixing of a thermalised 32^4 configuration (one JSON line)")
     ap.add_argument("--landau-sweeps", type=int, default=30000)
@@ -339,13 +350,19 @@ def main():
         kernel_only(a.kernel_only, F64 if a.prec == "f64" else F32, a.L, a.reps, a.force_variant)
         return
 
+    import mesh_amd
+
+    compact = not a.matrix_momenta
+    mesh_amd.default_context().set_option("compact_momenta", int(compact))
+    momenta = ("compact antihermitian, 9 reals per link (bit-identical to 18-real matrices)" if compact
+               else "18-real matrices")
     r64 = run_gpu(F64, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
     V = r64["V"]
     value = V * a.steps / r64["seconds"]
     dom = dominant(r64["kernels"])
     fk = r64["kernels"][dom]
     R = 8
-    alg = kernel_bytes(dom, R) * V
+    alg = kernel_bytes(dom, R, compact) * V
     achieved = alg / (fk["avg_ms"] * 1e-3) / 1e9
     line = {
         "metric": "SU(3) HMC site-updates/s at 32^4 (FP64)",
@@ -362,13 +379,13 @@ def main():
         "data": "synthetic: Philox-seeded Haar-random SU(3) start, seed 2, thermalised",
         "config": {"workload": f"SU(3) Wilson HMC {a.L}^4 beta=6.0 {a.scheme} eps={a.epsilon} "
                                f"substeps={a.substeps} (tau=1), FP64",
-                   "L": a.L, "beta": 6.0, "thermalisation": a.therm},
+                   "L": a.L, "beta": 6.0, "thermalisation": a.therm, "momenta": momenta},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("f64", a.L) if dom == "force_link" else None,
+                     "traffic": measured_traffic("f64", a.L) if dom == "force_link" and compact else None,
                      "kernel": dom, "avg_ms": fk["avg_ms"],
                      "algorithmic_bytes_per_launch": alg},
-        "trajectory_roofline_frac": value * trajectory_bytes_per_site(r64["kernels"], 8, a.steps) / 8e12,
+        "trajectory_roofline_frac": value * trajectory_bytes_per_site(r64["kernels"], 8, a.steps, compact) / 8e12,
         "acceptance": r64["acceptance"],
         "plaquette": r64["plaquette"],
         "exp_minus_dH": r64["exp_mdH"],
@@ -379,14 +396,14 @@ def main():
         v32 = V * a.steps / r32["seconds"]
         dom32 = dominant(r32["kernels"])
         fk32 = r32["kernels"][dom32]
-        ach32 = kernel_bytes(dom32, 4) * V / (fk32["avg_ms"] * 1e-3) / 1e9
+        ach32 = kernel_bytes(dom32, 4, compact) * V / (fk32["avg_ms"] * 1e-3) / 1e9
         cmp = same_config_compare(r64["hmc"].U, a.seed, a.scheme, a.epsilon, a.substeps)
         line["f32_mixed"] = {
             "value": v32, "ms_per_step": 1e3 * r32["seconds"] / a.steps, "ratio_vs_f64": v32 / value,
             "acceptance": r32["acceptance"], "plaquette": r32["plaquette"],
             "dominant_kernel": dom32, "dominant_frac_of_hbm": ach32 / HBM_PEAK_GBS,
-            "dominant_traffic": measured_traffic("f32", a.L) if dom32 == "force_link" else None,
-            "trajectory_roofline_frac": v32 * trajectory_bytes_per_site(r32["kernels"], 4, a.steps) / 8e12,
+            "dominant_traffic": measured_traffic("f32", a.L) if dom32 == "force_link" and compact else None,
+            "trajectory_roofline_frac": v32 * trajectory_bytes_per_site(r32["kernels"], 4, a.steps, compact) / 8e12,
             "same_config_vs_f64": cmp, "kernels": r32["kernels"],
         }
     if not a.no_cpu:

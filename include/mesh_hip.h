@@ -107,7 +107,12 @@ void *mesh_stream(mesh_ctx *ctx);
  *       and the link step after it run as one kernel (the thread that
  *       updates P_mu(x) writes exp(t P_mu(x)) U_mu(x) into a second field),
  *       and so do the momenta refresh and the first link step; same
- *       arithmetic, so the same bits. */
+ *       arithmetic, so the same bits;
+ *   "compact_momenta" = 1 (default): the fused trajectory keeps its momenta
+ *       in a compact antihermitian layout of 9 reals per link (upper
+ *       off-diagonal entries and the imaginary diagonal); 0 keeps the
+ *       18-real matrix layout.  The momenta are exactly antihermitian, so
+ *       both give the same bits. */
 int mesh_set_option(mesh_ctx *ctx, const char *name, long value);
 /* Per-kernel timing (replaces the util::Stopwatch globals swStaples, swExp,
  * swPlaquette ... of src/gauge/utils.h:23, src/gauge/hmc.h:16, printed at

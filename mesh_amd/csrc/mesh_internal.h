@@ -45,6 +45,7 @@ struct mesh_ctx
 	int rb_tile = 4;       // row-blocks per traversal tile of order 3
 	int force_variant = -1; // -1: per precision (FP64 wavefront, FP32 LDS tile)
 	int fuse_md = 1;        // fuse each momentum step with the next link step
+	int compact_p = 1;      // fused trajectory: momenta in 9 reals per link
 	void *md_spare = nullptr; // third gauge field for the fused trajectory
 	size_t md_spare_bytes = 0;
 	// profiler: events around launches, resolved after a stream sync

@@ -321,6 +321,8 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value)
 		ctx->force_variant = (int)value;
 	else if (!std::strcmp(name, "fuse_md"))
 		ctx->fuse_md = value != 0;
+	else if (!std::strcmp(name, "compact_momenta"))
+		ctx->compact_p = value != 0;
 	else
 		return mesh_fail(ctx, MESH_E_ARG, std::string("unknown option ") + name);
 	return MESH_OK;

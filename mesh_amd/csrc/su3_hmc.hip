@@ -423,7 +423,7 @@ __device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Un
 // following link step is fused in: Unext = exp(tl TA(P_new)) U (Unext != U,
 // since neighbouring threads still read U), and P is in the compact
 // antihermitian layout (the fused kernels run only inside a trajectory).
-template <class R, bool FUSE>
+template <class R, bool FUSE, bool PC = FUSE>
 __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
                                                                 int nb, int remap, R *__restrict__ Unext, R tl,
                                                                 int reunit_flag)
@@ -481,14 +481,14 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 		u = load(U, mu, g, x);
 	}
 	m3<R> A = ta(mul_nn(u, S));
-	m3<R> p = load_mom<FUSE>(P, mu, g, x);
+	m3<R> p = load_mom<PC>(P, mu, g, x);
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
 		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
 		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 	}
-	store_mom<FUSE>(P, mu, g, x, p);
+	store_mom<PC>(P, mu, g, x, p);
 	if constexpr (FUSE)
 		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 }
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 // LDS; every staple link displaced only along x2/x3 is then read from LDS and
 // only the x0/x1-displaced ones from global memory.  Wavefront w handles
 // direction w; the arithmetic (and the layout of P) is that of k_force_update.
-template <class R, int L3, bool FUSE>
+template <class R, int L3, bool FUSE, bool PC = FUSE>
 __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, const R *__restrict__ U, R *__restrict__ P,
                                                                      R kc, int nb, int remap, R *__restrict__ Unext,
                                                                      R tl, int reunit_flag)
@@ -583,14 +583,14 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 		}
 		m3<R> u = from_lds(mu, 0, 0);
 		m3<R> A = ta(mul_nn(u, S));
-		m3<R> p = load_mom<FUSE>(P, mu, g, x);
+		m3<R> p = load_mom<PC>(P, mu, g, x);
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
 		{
 			p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
 			p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
 		}
-		store_mom<FUSE>(P, mu, g, x, p);
+		store_mom<PC>(P, mu, g, x, p);
 		if constexpr (FUSE)
 			fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
 	};
@@ -752,10 +752,11 @@ int force_variant_of(mesh_ctx *ctx, int prec)
 
 // momentum step P -= delta beta/6 TA(U Sigma); with Unext != nullptr the
 // link step Unext = exp(tl TA(P)) U (+ reunit) follows, fused into the same
-// kernel where the variant supports it (Unext must differ from U; P is then
-// in the compact antihermitian layout, see force_variant_of != 1)
+// kernel where the variant supports it (Unext must differ from U; with
+// `compact` P is in the compact antihermitian layout, which only the fused
+// kernels read: force_variant_of != 1)
 int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, void *P, double beta, double delta,
-                        void *Unext = nullptr, double tl = 0.0, int reunit_flag = 0)
+                        void *Unext = nullptr, double tl = 0.0, int reunit_flag = 0, bool compact = false)
 {
 	double k = delta * (beta / 6.0);
 	int ph = mesh_prof_begin(ctx, Unext ? PROF_FORCE_LINK : PROF_FORCE_UPDATE);
@@ -772,8 +773,11 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 		auto launch = [&](auto tag, auto l3) {
 			using Rt = decltype(tag);
 			constexpr int l3v = decltype(l3)::value;
-			if (fused)
-				k_force_update_tile<Rt, l3v, true><<<gb, 256, 0, ctx->stream>>>(
+			if (fused && compact)
+				k_force_update_tile<Rt, l3v, true, true><<<gb, 256, 0, ctx->stream>>>(
+				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
+			else if (fused)
+				k_force_update_tile<Rt, l3v, true, false><<<gb, 256, 0, ctx->stream>>>(
 				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else
 				k_force_update_tile<Rt, l3v, false><<<gb, 256, 0, ctx->stream>>>(
@@ -805,10 +809,12 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 		int gb = mesh_grid_blocks(ctx, nb);
 		auto launch = [&](auto tag) {
 			using Rt = decltype(tag);
-			if (fused)
-				k_force_update<Rt, true><<<gb, 256, 0, ctx->stream>>>(geo, (const Rt *)U, (Rt *)P, (Rt)k, nb,
-				                                                      mesh_order(ctx), (Rt *)Unext, (Rt)tl,
-				                                                      reunit_flag);
+			if (fused && compact)
+				k_force_update<Rt, true, true><<<gb, 256, 0, ctx->stream>>>(
+				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
+			else if (fused)
+				k_force_update<Rt, true, false><<<gb, 256, 0, ctx->stream>>>(
+				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else
 				k_force_update<Rt, false><<<gb, 256, 0, ctx->stream>>>(geo, (const Rt *)U, (Rt *)P, (Rt)k, nb,
 				                                                       mesh_order(ctx), nullptr, (Rt)0, 0);
@@ -818,7 +824,11 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	if (Unext && !fused)
+	{
+		if (compact)
+			return mesh_fail(ctx, MESH_E_ARG, "compact momenta need a fused force variant");
 		return launch_link_update(ctx, geo, geo.nd, prec, Unext, U, P, tl, reunit_flag);
+	}
 	return MESH_OK;
 }
 
@@ -1163,8 +1173,9 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 	int rc;
 	const int nfused = (nd_ - 1) / 2;
 	// the fused kernels keep P in the compact antihermitian layout (half the
-	// bytes of a matrix field, the same bits)
-	const bool compact = ctx->fuse_md && nfused > 0 && force_variant_of(ctx, prec) != 1;
+	// bytes of a matrix field, the same bits; option "compact_momenta" = 0
+	// keeps the 18-real matrix layout)
+	const bool compact = ctx->compact_p && ctx->fuse_md && nfused > 0 && force_variant_of(ctx, prec) != 1;
 	if (!(ctx->fuse_md && nfused > 0) && (rc = launch_momenta(ctx, s.geo, prec, P, K, seed, traj)))
 		return rc;
 	if (ctx->fuse_md && nfused > 0)
@@ -1193,7 +1204,7 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 		for (int i = 1; i + 1 < nd_; i += 2)
 		{
 			if ((rc = launch_force_update(ctx, s.geo, prec, cur, P, beta, deltas[i], other, deltas[i + 1],
-			                              i + 1 == nd_ - 1)))
+			                              i + 1 == nd_ - 1, compact)))
 				return rc;
 			void *t = cur;
 			cur = other;

@@ -231,16 +231,18 @@ def test_hmc_trajectories_match_oracle(ctx, prec):
 
 
 @pytest.mark.parametrize("variant", [-1, 0, 2])
-@pytest.mark.parametrize("fuse", [0, 1])
+@pytest.mark.parametrize("fuse,compact", [(0, 1), (1, 1), (1, 0)])
 @pytest.mark.parametrize("scheme,n", [("2lf", 3), ("2mn", 2), ("4mn", 1), ("2lf", 4)])
 @pytest.mark.parametrize("prec", [F64, F32])
-def test_hmc_fused_md_bitexact(ctx, prec, scheme, n, fuse, variant):
+def test_hmc_fused_md_bitexact(ctx, prec, scheme, n, fuse, compact, variant):
     """The fused momentum+link step and the separate kernels give the oracle's
     bits for every integrator; odd and even numbers of fused pairs exercise
-    both parities of the field ping-pong, with both force kernels."""
+    both parities of the field ping-pong, with both force kernels and with the
+    momenta in the compact (9 reals) and the matrix (18 reals) layout."""
     shape = [4, 4, 6, 4]
     g = mesh_amd.Grid(shape)
     ctx.set_option("fuse_md", fuse)
+    ctx.set_option("compact_momenta", compact)
     ctx.set_option("force_variant", variant)
     try:
         hmc = gauge.Hmc(g, prec, seed=5, ctx=ctx)
@@ -254,7 +256,27 @@ def test_hmc_fused_md_bitexact(ctx, prec, scheme, n, fuse, variant):
             np.testing.assert_array_equal(hmc.U.download(), U)
     finally:
         ctx.set_option("fuse_md", 1)
+        ctx.set_option("compact_momenta", 1)
         ctx.set_option("force_variant", -1)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_compact_momenta_same_bits_16(ctx, prec):
+    """At 16^4 (2mn, 12 substeps, three trajectories from a Haar start) the
+    trajectory with momenta in 9 reals per link and the one with the 18-real
+    matrix layout give the same fields, Delta H and decisions bit for bit."""
+    shape = [16, 16, 16, 16]
+    g = mesh_amd.Grid(shape)
+    d = gauge.makeDeltas("2mn", 0.5, 12)
+    runs = {}
+    for compact in (1, 0):
+        with options(ctx, compact_momenta=compact):
+            h = gauge.Hmc(g, prec, seed=4, ctx=ctx)
+            h.randomizeGaugeField()
+            rs = [h.runHmcUpdate(6.0, d, metropolis=t > 0) for t in range(3)]
+            runs[compact] = ([(r["dH"], r["accepted"], r["plaq"]) for r in rs], h.U.download())
+    assert runs[1][0] == runs[0][0]
+    np.testing.assert_array_equal(runs[1][1], runs[0][1])
 
 
 def test_f32_vs_f64_same_config(ctx):
