@@ -507,7 +507,13 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 {
 	constexpr int RW = 64 / L3; // rows per tile
 	constexpr int NR = RW + 2;  // with halo rows
-	__shared__ vec2_t<R> lds[NR][4][9][L3];
+	// a staged row holds its 4 x 9 planes of L3 entries in the slot order of
+	// the field (even x3 first), so the 16-lane groups of a wavefront read
+	// consecutive entries; rows are padded by 128 B so that successive rows,
+	// read in the same cycle by different lane groups, start on other banks
+	constexpr int RS = 36 * L3 + 128 / (int)sizeof(vec2_t<R>);
+	__shared__ vec2_t<R> lds[NR * RS];
+	auto slot3 = [](int c3) { return MESH_SITE_PAIRS ? ((c3 & 1) * (L3 / 2) + (c3 >> 1)) : c3; };
 	int lb = ordered_block(g, nb, 64, remap);
 	if (lb < 0)
 		return;
@@ -517,16 +523,16 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 	// stage rows x2_0 - 1 ... x2_0 + RW (periodic in x2)
 	for (int e = threadIdx.x; e < NR * 4 * 9 * L3; e += 256)
 	{
-		int c = e % L3;
-		if (MESH_SITE_PAIRS) // consecutive threads on consecutive plane slots
-			c = c < L3 / 2 ? 2 * c : 2 * (c - L3 / 2) + 1;
+		const int j = e % L3; // slot within the row
+		const int c = MESH_SITE_PAIRS ? (j < L3 / 2 ? 2 * j : 2 * (j - L3 / 2) + 1) : j;
 		int k = (e / L3) % 9;
 		int d = (e / (9 * L3)) % 4;
 		int lr = e / (36 * L3);
 		int x2 = x2_0 - 1 + lr;
 		x2 = x2 < 0 ? x2 + g.L[2] : (x2 >= g.L[2] ? x2 - g.L[2] : x2);
 		long xs = row0 + (long)x2 * L3 + c;
-		lds[lr][d][k][c] = reinterpret_cast<const vec2_t<R> *>(U)[site_off(g, d, xs) + k * plane_step(g)];
+		lds[lr * RS + (d * 9 + k) * L3 + j] =
+		    reinterpret_cast<const vec2_t<R> *>(U)[site_off(g, d, xs) + k * plane_step(g)];
 	}
 	__syncthreads();
 
@@ -543,11 +549,12 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 		auto from_lds = [&](int d, int dx2, int dx3) {
 			int cc = c3 + dx3;
 			cc = cc < 0 ? cc + L3 : (cc >= L3 ? cc - L3 : cc);
+			const vec2_t<R> *row = lds + (r + 1 + dx2) * RS + d * 9 * L3 + slot3(cc);
 			m3<R> m;
 #pragma unroll
 			for (int k = 0; k < 9; ++k)
 			{
-				vec2_t<R> v = lds[r + 1 + dx2][d][k][cc];
+				vec2_t<R> v = row[k * L3];
 				m.e[k] = {v.x, v.y};
 			}
 			return m;
