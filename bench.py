@@ -303,7 +303,7 @@ def main():
                          "landau_or)")
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--force-variant", type=int, default=-1,
-                    help="kernel-only mode: force kernel variant (-1 per precision, 0, 1, 2)")
+                    help="kernel-only mode: force kernel variant (-1 library default, 0, 1, 2)")
     ap.add_argument("--prec", default="f64")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)

@@ -100,9 +100,9 @@ void *mesh_stream(mesh_ctx *ctx);
  *       otherwise order 2 or 1 applies);
  *   "rb_tile" = row-blocks per tile of order 3 (default 4; used when it
  *       divides the row-blocks of an x2/x3 plane);
- *   "force_variant" = -1 (default) per precision: 0 for FP64, 2 for FP32;
- *       0 wavefront per direction, 1 thread per site, 2 LDS-staged x2/x3
- *       tile;
+ *   "force_variant" = -1 (default: 0); 0 wavefront per direction with the
+ *       block's links staged in LDS, 1 thread per site, 2 LDS-staged x2/x3
+ *       tile with halo rows;
  *   "fuse_md" = 1 (default): in mesh_su3_hmc_trajectory each momentum step
  *       and the link step after it run as one kernel (the thread that
  *       updates P_mu(x) writes exp(t P_mu(x)) U_mu(x) into a second field),

@@ -27,6 +27,15 @@ using namespace mesh_dev;
 #else
 #define FORCE_OCC
 #endif
+// ... for the FP32 wavefront-per-direction kernel only (0: as above)
+#ifndef MESH_FORCE_WAVES_F32
+#define MESH_FORCE_WAVES_F32 0
+#endif
+template <class R> constexpr int force_min_waves()
+{
+	return (std::is_same<R, float>::value && MESH_FORCE_WAVES_F32 > 0) ? MESH_FORCE_WAVES_F32
+	                                                                  : (MESH_FORCE_WAVES > 0 ? MESH_FORCE_WAVES : 1);
+}
 
 // non-temporal hints on the once-touched fields of the force kernels
 #ifndef MESH_STREAM_HINTS
@@ -45,6 +54,10 @@ using namespace mesh_dev;
 // variant 0 stages the workgroup's own-site links in LDS (4-d, V % 64 == 0)
 #ifndef MESH_OWN_LDS
 #define MESH_OWN_LDS 1
+#endif
+// ... and, when the block holds whole x3-rows, the links of the x3-neighbours
+#ifndef MESH_OWN_ROWS
+#define MESH_OWN_ROWS 1
 #endif
 // staples with the direction as a compile-time constant (4-d lattices)
 #ifndef MESH_STATIC_MU
@@ -170,22 +183,34 @@ __device__ __forceinline__ m3<R> staple_4d(Geo const &g, const R *__restrict__ U
 	return S;
 }
 
-// The same with the four links of site x read from LDS (own[d][k][lane],
-// staged once by the workgroup instead of once per direction's wavefront)
-template <int MU, class R>
+// The same with links read from the workgroup's 64-site block staged in LDS
+// (own[d][k][slot], staged once by the workgroup instead of once per
+// direction's wavefront): the four links of site x, and with ROWS (the block
+// holds whole x3-rows, L3 | 64) also those of its x3-neighbours.
+__device__ __forceinline__ int block_slot(int o) { return MESH_SITE_PAIRS ? (((o & 1) << 5) | (o >> 1)) : o; }
+
+template <int MU, bool ROWS, class R>
 __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict__ U, long x, int const *c,
                                                vec2_t<R> (*own)[9][64], int lane)
 {
-	auto own_link = [&](int d) {
+	auto own_at = [&](int d, int slot) {
 		m3<R> m;
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
 		{
-			vec2_t<R> v = own[d][k][lane];
+			vec2_t<R> v = own[d][k][slot];
 			m.e[k] = {v.x, v.y};
 		}
 		return m;
 	};
+	// slots of the x3-neighbours x +/- e3 (used with ROWS only)
+	int sp = 0, sm = 0;
+	if constexpr (ROWS)
+	{
+		const int o = (int)(x & 63), c3 = c[3], L3 = g.L[3], o0 = o - c3;
+		sp = block_slot(o0 + (c3 + 1 == L3 ? 0 : c3 + 1));
+		sm = block_slot(o0 + (c3 == 0 ? L3 - 1 : c3 - 1));
+	}
 	m3<R> S = zero_m3<R>();
 	long xpm = shift_c(g, x, MU, c[MU], 1);
 #pragma unroll
@@ -193,16 +218,22 @@ __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict
 	{
 		if (nu == MU)
 			continue;
-		long xpn = shift_c(g, x, nu, c[nu], 1);
-		long xmn = shift_c(g, x, nu, c[nu], -1);
+		const bool fwd3 = ROWS && MU == 3; // x + mu is an x3-neighbour
+		const bool nu3 = ROWS && nu == 3;  // x +/- nu are x3-neighbours
+		long xpn = nu3 ? 0 : shift_c(g, x, nu, c[nu], 1);
+		long xmn = nu3 ? 0 : shift_c(g, x, nu, c[nu], -1);
 		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
 		{
-			m3<R> T1 = mul_nd(load(U, nu, g, xpm), load(U, MU, g, xpn));
-			madd_nd(S, T1, own_link(nu));
+			m3<R> a = fwd3 ? own_at(nu, sp) : load(U, nu, g, xpm);
+			m3<R> b = nu3 ? own_at(MU, sp) : load(U, MU, g, xpn);
+			m3<R> T1 = mul_nd(a, b);
+			madd_nd(S, T1, own_at(nu, lane));
 		}
 		{
-			m3<R> T2 = mul_dd(load(U, nu, g, xpmmn), load(U, MU, g, xmn));
-			madd_nn(S, T2, load(U, nu, g, xmn));
+			m3<R> a = load(U, nu, g, xpmmn);
+			m3<R> b = nu3 ? own_at(MU, sm) : load(U, MU, g, xmn);
+			m3<R> T2 = mul_dd(a, b);
+			madd_nn(S, T2, nu3 ? own_at(nu, sm) : load(U, nu, g, xmn));
 		}
 	}
 	return S;
@@ -424,7 +455,7 @@ __device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Un
 // since neighbouring threads still read U), and P is in the compact
 // antihermitian layout (the fused kernels run only inside a trajectory).
 template <class R, bool FUSE, bool PC = FUSE>
-__global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(force_min_waves<R>()))) void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
                                                                 int nb, int remap, R *__restrict__ Unext, R tl,
                                                                 int reunit_flag)
 {
@@ -449,21 +480,22 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update(Geo g, const R *
 		__syncthreads();
 		int c[4];
 		coords<R>(g, x, c);
-		switch (mu)
-		{
-		case 0:
-			S = staple_4d_own<0>(g, U, x, c, own, lane);
-			break;
-		case 1:
-			S = staple_4d_own<1>(g, U, x, c, own, lane);
-			break;
-		case 2:
-			S = staple_4d_own<2>(g, U, x, c, own, lane);
-			break;
-		default:
-			S = staple_4d_own<3>(g, U, x, c, own, lane);
-			break;
-		}
+		auto staple = [&](auto rows) {
+			constexpr bool RW = decltype(rows)::value;
+			switch (mu)
+			{
+			case 0:
+				return staple_4d_own<0, RW>(g, U, x, c, own, lane);
+			case 1:
+				return staple_4d_own<1, RW>(g, U, x, c, own, lane);
+			case 2:
+				return staple_4d_own<2, RW>(g, U, x, c, own, lane);
+			default:
+				return staple_4d_own<3, RW>(g, U, x, c, own, lane);
+			}
+		};
+		// the block holds whole x3-rows when L3 divides 64
+		S = (MESH_OWN_ROWS && 64 % g.L[3] == 0) ? staple(std::true_type{}) : staple(std::false_type{});
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
 		{
@@ -750,11 +782,12 @@ double n_plaquettes(Geo const &geo) { return (double)geo.V * geo.nd * (geo.nd - 
 int launch_link_update(mesh_ctx *ctx, Geo const &geo, int nf, int prec, void *Uout, const void *Uin, const void *P,
                        double t, int reunit_flag);
 
-// force kernel variant in use: 0 wavefront per direction, 1 thread per site,
-// 2 LDS tile (falls back to 0 where the tile does not fit)
-int force_variant_of(mesh_ctx *ctx, int prec)
+// force kernel variant in use: 0 wavefront per direction (the default in both
+// precisions), 1 thread per site, 2 LDS tile (falls back to 0 where the tile
+// does not fit)
+int force_variant_of(mesh_ctx *ctx, int)
 {
-	return ctx->force_variant >= 0 ? ctx->force_variant : (prec == MESH_F32 ? 2 : 0);
+	return ctx->force_variant >= 0 ? ctx->force_variant : 0;
 }
 
 // momentum step P -= delta beta/6 TA(U Sigma); with Unext != nullptr the
@@ -770,8 +803,6 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 	const int L3 = geo.L[3], RW = L3 > 0 ? 64 / L3 : 0;
 	const bool tile_ok = geo.nd == 4 && (L3 == 16 || L3 == 32 || L3 == 64) && geo.L[2] % RW == 0;
 	bool fused = Unext != nullptr;
-	// default: FP64 is bound by HBM traffic and register pressure (wavefront
-	// per direction); FP32 by issue, where the LDS tile's fewer loads pay
 	const int variant = force_variant_of(ctx, prec);
 	if (variant == 2 && tile_ok)
 	{
