@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--substeps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32-mixed run")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="library tuning option (mesh_set_option) for the run, e.g. xcd_remap=4; repeatable")
     ap.add_argument("--matrix-momenta", action="store_true",
                     help="keep the trajectory's momenta as 18-real matrices (option compact_momenta = 0; "
                          "same bits, more traffic)")
@@ -337,8 +339,14 @@ def main():
               "(the lattice is not domain-decomposed)", file=sys.stderr)
         sys.exit(2)
 
+    import mesh_amd
     from mesh_amd import F32, F64
 
+    options = {}
+    for o in a.option:
+        name, _, v = o.partition("=")
+        options[name] = int(v)
+        mesh_amd.default_context().set_option(name, int(v))
     if a.landau:
         print(json.dumps(landau_bench(a.L, a.seed, a.therm, a.landau_sweeps)), flush=True)
         return
@@ -349,8 +357,6 @@ def main():
     if a.kernel_only:
         kernel_only(a.kernel_only, F64 if a.prec == "f64" else F32, a.L, a.reps, a.force_variant)
         return
-
-    import mesh_amd
 
     compact = not a.matrix_momenta
     mesh_amd.default_context().set_option("compact_momenta", int(compact))
@@ -379,7 +385,8 @@ def main():
         "data": "synthetic: Philox-seeded Haar-random SU(3) start, seed 2, thermalised",
         "config": {"workload": f"SU(3) Wilson HMC {a.L}^4 beta=6.0 {a.scheme} eps={a.epsilon} "
                                f"substeps={a.substeps} (tau=1), FP64",
-                   "L": a.L, "beta": 6.0, "thermalisation": a.therm, "momenta": momenta},
+                   "L": a.L, "beta": 6.0, "thermalisation": a.therm, "momenta": momenta,
+                   **({"options": options} if options else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("f64", a.L) if dom == "force_link" and compact else None,

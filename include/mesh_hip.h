@@ -97,6 +97,8 @@ void *mesh_stream(mesh_ctx *ctx);
  *       row-blocks and walks x0 fastest, then x1; 3 (default) each XCD takes
  *       a slab of L0/8 x0-slices and walks it x0 fastest, then "rb_tile"
  *       row-blocks, then x1, then the next tile (4-d lattices with 8 | L0;
+ *       otherwise order 4, 2 or 1 applies); 4 as 3 with XCD k owning a
+ *       quarter of x0 (k % 4) and a half of x1 (k / 4) (4 | L0, 2 | L1;
  *       otherwise order 2 or 1 applies);
  *   "rb_tile" = row-blocks per tile of order 3 (default 4; used when it
  *       divides the row-blocks of an x2/x3 plane);

@@ -42,8 +42,8 @@ struct mesh_ctx
 	size_t aux_cap = 0;    // doubles
 	std::vector<double> aux_host; // host copy of what aux holds
 	int xcd_remap = 3;     // block order of the stencil kernels (ordered_block)
-	int rb_tile = 4;       // row-blocks per traversal tile of order 3
-	int force_variant = -1; // -1: per precision (FP64 wavefront, FP32 LDS tile)
+	int rb_tile = 4;       // row-blocks per traversal tile of orders 3 and 4
+	int force_variant = -1; // -1: the default kernel (variant 0)
 	int fuse_md = 1;        // fuse each momentum step with the next link step
 	int compact_p = 1;      // fused trajectory: momenta in 9 reals per link
 	void *md_spare = nullptr; // third gauge field for the fused trajectory
@@ -136,7 +136,10 @@ __device__ __forceinline__ int logical_block(int nb, int remap)
 //      owns the slab x0 in [k L0/8, (k+1) L0/8) and walks it x0 fastest, then
 //      T row-blocks, then x1, then the next tile of T row-blocks, so the x1-
 //      and rb-neighbours of a workgroup are at most (L0/8) T workgroups back
-//      and only the two x0 faces of the slab come from another XCD.
+//      and only the two x0 faces of the slab come from another XCD;
+//   4: as 3 with XCD k owning x0 in a quarter (k % 4) and x1 in a half (k / 4)
+//      of the lattice (4 | L0, 2 | L1); also order 3's fallback where 8 does
+//      not divide L0.
 // Returns the memory-order block index (-1: padding workgroup).  The order
 // only permutes workgroups; every site's arithmetic is unchanged.
 __device__ __forceinline__ int ordered_block_of(Geo const &g, int nb, int sbk, int arg, int b)
@@ -158,6 +161,24 @@ __device__ __forceinline__ int ordered_block_of(Geo const &g, int nb, int sbk, i
 		t /= T;
 		int x1 = t % g.L[1];
 		rb += (t / g.L[1]) * T;
+		return (x0 * g.L[1] + x1) * nb2 + rb;
+	}
+	if ((order == 4 || order == 3) && planes && (g.L[0] & 3) == 0 && (g.L[1] & 1) == 0 && T > 0 &&
+	    nb2 % T == 0)
+	{
+		// XCD k owns x0 in a quarter (k % 4) and x1 in a half (k / 4) of the
+		// lattice, walked as in order 3 (and order 3's fallback where 8 does
+		// not divide L0)
+		const int X0 = g.L[0] >> 2, X1 = g.L[1] >> 1, k = b & 7;
+		const int j = b >> 3;
+		if (j >= nb >> 3)
+			return -1;
+		int x0 = (k & 3) * X0 + j % X0;
+		int t = j / X0;
+		int rb = t % T;
+		t /= T;
+		int x1 = (k >> 2) * X1 + t % X1;
+		rb += (t / X1) * T;
 		return (x0 * g.L[1] + x1) * nb2 + rb;
 	}
 	int chunk = (nb + 7) >> 3;

@@ -313,7 +313,7 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value)
 	MESH_CHECK_CTX(ctx);
 	if (!name)
 		return mesh_fail(ctx, MESH_E_ARG, "option name is NULL");
-	if (!std::strcmp(name, "xcd_remap") && value >= 0 && value <= 3)
+	if (!std::strcmp(name, "xcd_remap") && value >= 0 && value <= 4)
 		ctx->xcd_remap = (int)value;
 	else if (!std::strcmp(name, "rb_tile") && value >= 1 && value <= 64)
 		ctx->rb_tile = (int)value;

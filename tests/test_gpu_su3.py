@@ -113,7 +113,7 @@ def test_force_update_bitexact(ctx, shape, prec):
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("prec", [F64, F32])
 @pytest.mark.parametrize("shape", [(4, 4, 2, 32), (2, 3, 4, 16), (2, 2, 2, 64), (3, 2, 6, 32), (4, 4, 4, 4)])
-@pytest.mark.parametrize("remap", [0, 1, 2, 3])
+@pytest.mark.parametrize("remap", [0, 1, 2, 3, 4])
 def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
     """Every force-kernel variant (wavefront per direction, thread per site,
     LDS tile; tile-eligible shapes included) gives the oracle's bits."""
@@ -126,18 +126,20 @@ def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
     np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
 
 
+@pytest.mark.parametrize("order", [3, 4])
 @pytest.mark.parametrize("tile", [1, 2, 4])
 @pytest.mark.parametrize("variant", [0, 2])
 @pytest.mark.parametrize("prec", [F64, F32])
-@pytest.mark.parametrize("shape", [(8, 2, 4, 32), (16, 2, 8, 16), (8, 3, 8, 32)])
-def test_slab_order_bitexact(ctx, shape, prec, variant, tile):
-    """Block order 3 (x0 slabs per XCD, row-block tiles) with every tile that
-    divides the row-blocks: force update bit-exact, plaquette to 1e-12."""
+@pytest.mark.parametrize("shape", [(8, 2, 4, 32), (16, 2, 8, 16), (8, 3, 8, 32), (4, 4, 4, 32)])
+def test_slab_order_bitexact(ctx, shape, prec, variant, tile, order):
+    """Block orders 3 (x0 slabs per XCD) and 4 (x0 quarters x x1 halves per
+    XCD), with row-block tiles of every size that divides the row-blocks:
+    force update bit-exact, plaquette to 1e-12."""
     U = O.random_gauge(list(shape), 23, _pname(prec))
     P, _ = O.momenta(list(shape), 23, 0, _pname(prec))
     g, fu = _field(ctx, shape, prec, U)
     _, fp = _field(ctx, shape, prec, P)
-    with options(ctx, force_variant=variant, xcd_remap=3, rb_tile=tile):
+    with options(ctx, force_variant=variant, xcd_remap=order, rb_tile=tile):
         gauge.momentumStep(fu, fp, 6.0, 0.0833)
         pl = gauge.plaquette(fu)
     np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
@@ -145,10 +147,11 @@ def test_slab_order_bitexact(ctx, shape, prec, variant, tile):
     assert abs(pl - ref) <= 1e-12 * max(abs(ref), 1e-3)
 
 
-def test_slab_order_trajectory(ctx):
-    """A fused trajectory in block order 3 gives the oracle's field bits."""
+@pytest.mark.parametrize("order", [3, 4])
+def test_slab_order_trajectory(ctx, order):
+    """A fused trajectory in block orders 3 and 4 gives the oracle's field bits."""
     shape = [8, 2, 4, 32]
-    with options(ctx, xcd_remap=3, rb_tile=2):
+    with options(ctx, xcd_remap=order, rb_tile=2):
         hmc = gauge.Hmc(mesh_amd.Grid(shape), F64, seed=7, ctx=ctx)
         hmc.randomizeGaugeField()
         U = O.random_gauge(shape, 7, "f64")
