@@ -12,6 +12,11 @@
 // per half-sweep.
 #include "mesh_internal.h"
 
+// FP64 half-sweep: forward links with non-temporal loads and stores
+#ifndef MESH_OR_STREAM
+#define MESH_OR_STREAM 1
+#endif
+
 using namespace mesh_dev;
 
 namespace {
@@ -200,7 +205,9 @@ template <class R> __device__ __forceinline__ void or_site(Geo const &g, R *U, R
 		if (mu >= g.nd)
 			break;
 		xm[mu] = shift_c(g, x, mu, coord(g, x, mu), -1);
-		m3<R> a = load(U, mu, g, x);
+		// FP64: the forward links stream past the L2 so that the backward
+		// ones, read again after the transformation, are still there
+		m3<R> a = (!keep && MESH_OR_STREAM) ? load_stream(U, mu, g, x) : load(U, mu, g, x);
 		m3<R> b = load(U, mu, g, xm[mu]);
 #pragma unroll
 		for (int i = 0; i < 3; ++i)
@@ -226,6 +233,11 @@ template <class R> __device__ __forceinline__ void or_site(Geo const &g, R *U, R
 		{
 			store(U, mu, g, x, mul_nn(G, la[mu]));
 			store(U, mu, g, xm[mu], mul_nd(lb[mu], G));
+		}
+		else if (MESH_OR_STREAM)
+		{
+			store_stream(U, mu, g, x, mul_nn(G, la[mu]));
+			store(U, mu, g, xm[mu], mul_nd(load(U, mu, g, xm[mu]), G));
 		}
 		else
 		{
