@@ -31,10 +31,17 @@ using namespace mesh_dev;
 #ifndef MESH_FORCE_WAVES_F32
 #define MESH_FORCE_WAVES_F32 0
 #endif
-template <class R> constexpr int force_min_waves()
+// ... and for the FP64 kernel with the row-staged staple path (0: as above;
+// 3 fits in 168 VGPRs but measured slower than the compiler's two waves)
+#ifndef MESH_FORCE_WAVES_ROWS
+#define MESH_FORCE_WAVES_ROWS 0
+#endif
+template <class R, int PATH = 0> constexpr int force_min_waves()
 {
 	return (std::is_same<R, float>::value && MESH_FORCE_WAVES_F32 > 0) ? MESH_FORCE_WAVES_F32
-	                                                                  : (MESH_FORCE_WAVES > 0 ? MESH_FORCE_WAVES : 1);
+	       : (std::is_same<R, double>::value && PATH == 2 && MESH_FORCE_WAVES_ROWS > 0)
+	           ? MESH_FORCE_WAVES_ROWS
+	           : (MESH_FORCE_WAVES > 0 ? MESH_FORCE_WAVES : 1);
 }
 
 // non-temporal hints on the once-touched fields of the force kernels
@@ -454,10 +461,29 @@ __device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Un
 // following link step is fused in: Unext = exp(tl TA(P_new)) U (Unext != U,
 // since neighbouring threads still read U), and P is in the compact
 // antihermitian layout (the fused kernels run only inside a trajectory).
-template <class R, bool FUSE, bool PC = FUSE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(force_min_waves<R>()))) void k_force_update(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
-                                                                int nb, int remap, R *__restrict__ Unext, R tl,
-                                                                int reunit_flag)
+// Staple paths of k_force_update, one per kernel (a kernel is compiled for
+// the registers of its largest path, and the general ones need more):
+//   STAPLE_ROWS  4-d, V % 64 == 0, L3 | 64: the block's links staged in LDS,
+//                own and x3-neighbour links read there;
+//   STAPLE_OWN   4-d, V % 64 == 0: the block's links staged, own links read there;
+//   STAPLE_ANY   any geometry: every link from global memory.
+enum
+{
+	STAPLE_ANY = 0,
+	STAPLE_OWN = 1,
+	STAPLE_ROWS = 2
+};
+inline int staple_path(Geo const &g)
+{
+	if (!MESH_OWN_LDS || g.nd != 4 || (g.V & 63) != 0)
+		return STAPLE_ANY;
+	return (MESH_OWN_ROWS && 64 % g.L[3] == 0) ? STAPLE_ROWS : STAPLE_OWN;
+}
+
+template <class R, bool FUSE, bool PC, int PATH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>()))) void k_force_update(
+    Geo g, const R *__restrict__ U, R *__restrict__ P, R kc, int nb, int remap, R *__restrict__ Unext, R tl,
+    int reunit_flag)
 {
 	int lb = ordered_block(g, nb, SB, remap);
 	if (lb < 0)
@@ -465,7 +491,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(force_min_w
 	int mu = threadIdx.x >> 6;
 	long x = lane_site(g, (long)lb * SB + (threadIdx.x & 63));
 	m3<R> S, u;
-	if (MESH_OWN_LDS && g.nd == 4 && (g.V & 63) == 0)
+	if constexpr (PATH != STAPLE_ANY)
 	{
 		// every thread of the workgroup is a valid (site, direction): stage the
 		// block's own links, then the staples read them from LDS
@@ -480,22 +506,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(force_min_w
 		__syncthreads();
 		int c[4];
 		coords<R>(g, x, c);
-		auto staple = [&](auto rows) {
-			constexpr bool RW = decltype(rows)::value;
-			switch (mu)
-			{
-			case 0:
-				return staple_4d_own<0, RW>(g, U, x, c, own, lane);
-			case 1:
-				return staple_4d_own<1, RW>(g, U, x, c, own, lane);
-			case 2:
-				return staple_4d_own<2, RW>(g, U, x, c, own, lane);
-			default:
-				return staple_4d_own<3, RW>(g, U, x, c, own, lane);
-			}
-		};
-		// the block holds whole x3-rows when L3 divides 64
-		S = (MESH_OWN_ROWS && 64 % g.L[3] == 0) ? staple(std::true_type{}) : staple(std::false_type{});
+		constexpr bool RW = PATH == STAPLE_ROWS; // the block holds whole x3-rows
+		switch (mu)
+		{
+		case 0:
+			S = staple_4d_own<0, RW>(g, U, x, c, own, lane);
+			break;
+		case 1:
+			S = staple_4d_own<1, RW>(g, U, x, c, own, lane);
+			break;
+		case 2:
+			S = staple_4d_own<2, RW>(g, U, x, c, own, lane);
+			break;
+		default:
+			S = staple_4d_own<3, RW>(g, U, x, c, own, lane);
+			break;
+		}
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
 		{
@@ -845,19 +871,34 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 	{
 		int nb = nblk(geo.V, SB);
 		int gb = mesh_grid_blocks(ctx, nb);
-		auto launch = [&](auto tag) {
+		auto launch = [&](auto tag, auto path) {
 			using Rt = decltype(tag);
+			constexpr int PT = decltype(path)::value;
 			if (fused && compact)
-				k_force_update<Rt, true, true><<<gb, 256, 0, ctx->stream>>>(
+				k_force_update<Rt, true, true, PT><<<gb, 256, 0, ctx->stream>>>(
 				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else if (fused)
-				k_force_update<Rt, true, false><<<gb, 256, 0, ctx->stream>>>(
+				k_force_update<Rt, true, false, PT><<<gb, 256, 0, ctx->stream>>>(
 				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else
-				k_force_update<Rt, false><<<gb, 256, 0, ctx->stream>>>(geo, (const Rt *)U, (Rt *)P, (Rt)k, nb,
-				                                                       mesh_order(ctx), nullptr, (Rt)0, 0);
+				k_force_update<Rt, false, false, PT><<<gb, 256, 0, ctx->stream>>>(
+				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), nullptr, (Rt)0, 0);
 		};
-		prec == MESH_F64 ? launch(0.0) : launch(0.0f);
+		auto by_path = [&](auto tag) {
+			switch (staple_path(geo))
+			{
+			case STAPLE_ROWS:
+				launch(tag, std::integral_constant<int, STAPLE_ROWS>{});
+				break;
+			case STAPLE_OWN:
+				launch(tag, std::integral_constant<int, STAPLE_OWN>{});
+				break;
+			default:
+				launch(tag, std::integral_constant<int, STAPLE_ANY>{});
+				break;
+			}
+		};
+		prec == MESH_F64 ? by_path(0.0) : by_path(0.0f);
 	}
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
