@@ -249,10 +249,13 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
     return out
 
 
-def landau_bench(L, seed, therm, max_sweeps, omega=1.7, tol=1e-12):
+def landau_bench(L, seed, therm, max_sweeps, omega=1.7, tol=1e-12, ascent_iters=0):
     """Landau gauge fixing of a thermalised SU(3) L^4 configuration (HMC as in
     the main workload, `therm` trajectories) by even/odd overrelaxation, in
-    FP32 links (the measured path) and FP64; theta checked every 10 sweeps."""
+    FP32 links (the measured path) and FP64; theta checked every 10 sweeps.
+    With ascent_iters > 0 also the reference's line-search solver (FP64, at
+    most that many iterations) and FP64 overrelaxation, both to the
+    reference's stopping condition (condition() < 1e-10, theta < 4e-10)."""
     import mesh_amd
     from mesh_amd import F32, F64, gauge
 
@@ -289,6 +292,25 @@ def landau_bench(L, seed, therm, max_sweeps, omega=1.7, tol=1e-12):
                      "seconds": dt, "site_sweeps_per_s": V * sweeps / dt,
                      "half_sweep_ms": ms / max(n, 1),
                      "half_sweep_frac_of_hbm": alg / (ms / max(n, 1) * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+    if ascent_iters > 0:
+        cmp = {}
+        U = hmc.U.clone()
+        lan = gauge.Landau(U)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        it, cond = lan.run_ascent(ascent_iters, tol=1e-10)
+        ctx.synchronize()
+        cmp["reference_ascent"] = {"iterations": it, "condition": cond, "converged": bool(cond < 1e-10),
+                                   "functional": lan(), "seconds": time.perf_counter() - t0}
+        U = hmc.U.clone()
+        lan = gauge.Landau(U)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sweeps, theta = lan.run(max_sweeps, omega=omega, tol=4e-10, check_every=10)
+        ctx.synchronize()
+        cmp["overrelaxation"] = {"sweeps": sweeps, "condition": theta / 4, "functional": lan(),
+                                 "seconds": time.perf_counter() - t0}
+        out["f64_to_reference_condition"] = cmp
     out["plaquette_before"] = gauge.plaquette(hmc.U)
     out["value"] = out["f32"]["site_sweeps_per_s"]
     out["unit"] = "site-sweeps/s"
@@ -324,6 +346,9 @@ def main():
     ap.add_argument("--landau", action="store_true",
                     help="instead: Landau gauge fixing of a thermalised 32^4 configuration (one JSON line)")
     ap.add_argument("--landau-sweeps", type=int, default=30000)
+    ap.add_argument("--landau-ascent", type=int, default=0, metavar="N",
+                    help="with --landau: also the reference's line-search solver (at most N iterations) "
+                         "against overrelaxation, both to condition() < 1e-10")
     ap.add_argument("--u1", action="store_true",
                     help="instead: U(1) 2-d ESS/s of Q, flow-assisted vs plain HMC (one JSON line)")
     ap.add_argument("--u1-L", type=int, default=32)
@@ -348,7 +373,8 @@ def main():
         options[name] = int(v)
         mesh_amd.default_context().set_option(name, int(v))
     if a.landau:
-        print(json.dumps(landau_bench(a.L, a.seed, a.therm, a.landau_sweeps)), flush=True)
+        print(json.dumps(landau_bench(a.L, a.seed, a.therm, a.landau_sweeps, ascent_iters=a.landau_ascent)),
+              flush=True)
         return
     if a.u1:
         print(json.dumps(u1_ess(a.u1_L, a.u1_beta, a.u1_batch, a.u1_train, a.u1_hmc_steps, a.u1_flow_steps,

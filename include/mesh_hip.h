@@ -216,6 +216,10 @@ int mesh_su3_reunit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U
 /* U = exp(t TA(F)) U for nf = 1 fields (gauge transformation update,
  * src/gauge/landau.h:131). */
 int mesh_su3_exp_mul(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U, const void *F, double t);
+/* Y += a X over nf matrices per site, the product rounded before the sum:
+ * add_mul / `lhs += rhs * a` on lattices (src/lattice/lattice.h:302-309,
+ * 349-363), e.g. the conjugate-gradient update of landau.h:128. */
+int mesh_su3_axpy(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *Y, double a, const void *X);
 
 /* out[5] = {Delta H, sum_x ps_old(x), sum_x ps_new(x), K_old, K_new} with
  * per-site differencing d(x) = -(beta/3)(ps_new - ps_old)(x) + (k_new - k_old)(x),

@@ -6,7 +6,9 @@
 //   --start random|cold --seed S --count 100 --scheme 4mn|2mn|2lf
 //   --epsilon 1 --substeps 8 --precision 2 (1 = FP32-mixed, 2 = FP64)
 // Additions: --therm K (first K trajectories without Metropolis),
-//   --landau (Landau gauge fixing of the final configuration).
+//   --landau (Landau gauge fixing of the final configuration by
+//   overrelaxation), --landau-ascent (the same by the reference's line-search
+//   ascent, src/gauge/landau.h:100-136).
 #include "mesh/gauge.h"
 
 #include <cstdio>
@@ -25,7 +27,7 @@ namespace {
 	std::fprintf(stderr,
 	             "error: %s\nusage: mesh_hmc [--group su3|u1] [--geom L0 L1 [L2 [L3]]] [--beta B] [--start random|cold]\n"
 	             "                [--seed S] [--count N] [--scheme 2lf|2mn|4mn] [--epsilon E] [--substeps N]\n"
-	             "                [--precision 1|2] [--therm K] [--landau]\n",
+	             "                [--precision 1|2] [--therm K] [--landau] [--landau-ascent]\n",
 	             msg);
 	std::exit(2);
 }
@@ -56,7 +58,7 @@ int main(int argc, char **argv)
 	std::vector<int32_t> geom = {8, 8, 8, 8};
 	double beta = 6.0, epsilon = 1.0;
 	int substeps = 8, count = 100, precision = 2, therm = 0;
-	bool landau = false;
+	bool landau = false, ascent = false;
 
 	for (int i = 1; i < argc; ++i)
 	{
@@ -94,6 +96,8 @@ int main(int argc, char **argv)
 			therm = std::atoi(next().c_str());
 		else if (a == "--landau")
 			landau = true;
+		else if (a == "--landau-ascent")
+			landau = ascent = true;
 		else if (a == "--plot")
 			std::fprintf(stderr, "mesh_hmc: --plot (gnuplot of the plaquette history) is not supported; ignored\n");
 		else if (a == "--filename" || a == "--force")
@@ -138,9 +142,18 @@ int main(int argc, char **argv)
 			    {
 				    Landau<G> lan(hmc.U);
 				    lan.verbose = true;
-				    int it = lan.run(100000);
-				    std::printf("landau: %d sweeps, theta = %.3e, functional = %.12f\n", it,
-				                lan.condition() * g.ndim(), lan());
+				    if (ascent)
+				    {
+					    int it = lan.run_ascent(100000);
+					    std::printf("landau ascent: %d iterations, condition = %.3e, functional = %.12f\n", it,
+					                lan.condition(), lan());
+				    }
+				    else
+				    {
+					    int it = lan.run(100000);
+					    std::printf("landau: %d sweeps, theta = %.3e, functional = %.12f\n", it,
+					                lan.condition() * g.ndim(), lan());
+				    }
 			    }
 		    },
 		    group, precision);

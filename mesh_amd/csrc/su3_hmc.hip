@@ -732,6 +732,28 @@ __global__ __launch_bounds__(256) void k_link_update(Geo g, int nf, R *Uout, con
 	store(Uout, f, g, x, u);
 }
 
+// Y = Y + (a X) per real, the multiply and the add rounded separately as in
+// the reference's lattice expression `Y += X * a`
+template <class R>
+__global__ __launch_bounds__(256) void k_axpy(Geo g, int nf, R *__restrict__ Y, R a, const R *__restrict__ X)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= g.V * nf)
+		return;
+	int f = (int)(i / g.V);
+	long x = lane_site(g, i - (long)f * g.V);
+	m3<R> y = load(Y, f, g, x);
+	m3<R> b = load(X, f, g, x);
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		R pr = b.e[k].re * a, pi = b.e[k].im * a;
+		y.e[k].re = y.e[k].re + pr;
+		y.e[k].im = y.e[k].im + pi;
+	}
+	store(Y, f, g, x, y);
+}
+
 template <class R> __global__ __launch_bounds__(256) void k_reunit(Geo g, int nf, R *U)
 {
 	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1168,6 +1190,23 @@ int mesh_su3_exp_mul(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *
 	if (!U || !F || nf < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad exp_mul arguments");
 	return launch_link_update(ctx, s.geo, nf, prec, U, U, F, t, 0);
+}
+
+int mesh_su3_axpy(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *Y, double a, const void *X)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!Y || !X || nf < 1)
+		return mesh_fail(ctx, MESH_E_ARG, "bad axpy arguments");
+	int nb = nblk(s.geo.V * nf, 256);
+	if (prec == MESH_F64)
+		k_axpy<double><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (double *)Y, a, (const double *)X);
+	else
+		k_axpy<float><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (float *)Y, (float)a, (const float *)X);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
 }
 
 int mesh_su3_reunit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U)

@@ -86,6 +86,15 @@ def evolve(U, P, t):
     return U
 
 
+def add_mul(lhs, a, rhs):
+    """lhs += a * rhs, the product rounded before the sum (add_mul and
+    `lhs += rhs * a`, src/lattice/lattice.h:302-309, 349-363)."""
+    if lhs.nf != rhs.nf:
+        raise ValueError("fields differ in the number of matrices per site")
+    lhs.ctx.call("mesh_su3_axpy", lhs.grid.ref, lhs.nf, lhs.prec, lhs.ptr, _d(a), rhs.ptr)
+    return lhs
+
+
 def momentumStep(U, P, beta, delta):
     """P[mu] -= wilsonDeriv(U, mu, beta) * delta for every mu, fused (hmc.h:54-58)."""
     U.ctx.call("mesh_su3_force_update", U.grid.ref, U.prec, U.ptr, P.ptr, _d(beta), _d(delta))
@@ -265,3 +274,62 @@ class Landau:
                 break
         self.Ug = Ug
         return it, theta
+
+    @staticmethod
+    def solve_quadratic(x1, x2, x3, y1, y2, y3):
+        """abscissa of the vertex of the parabola through three points (landau.h:68-75)"""
+        return 0.5 * ((x2 * x2 - x3 * x3) * y1 - (x1 * x1 - x3 * x3) * y2 + (x1 * x1 - x2 * x2) * y3) / \
+            ((x2 - x3) * y1 - (x1 - x3) * y2 + (x1 - x2) * y3)
+
+    def line_search(self, F, eps):
+        """step length along F (landau.h:77-98): halve eps until the functional
+        of exp(eps F) g exceeds that of g, try 1.2 eps, else the vertex of the
+        parabola through eps, 1.2 eps and 0.8 eps."""
+        base = self()
+        h = self.g.clone()
+
+        def trial(e):
+            h.copy_from(self.g)
+            return self(evolve(h, F, e))
+
+        y1 = trial(eps)
+        while y1 <= base:
+            eps *= 0.5
+            y1 = trial(eps)
+            if eps < 1e-10:
+                return 0.0
+        y2 = trial(eps * 1.2)
+        if y2 > y1:
+            return 1.2 * eps
+        y3 = trial(eps * 0.8)
+        return self.solve_quadratic(eps, 1.2 * eps, 0.8 * eps, y1, y2, y3)
+
+    def run_ascent(self, max_iter, tol=1e-10):
+        """The reference's solver (landau.h:100-136): ascent along deriv() with
+        the line search, Polak-Ribiere steps after iteration 20 (as there,
+        `last = F` precedes `F += last * beta`, so the step is (1 + beta) F),
+        then g = reunitize(exp(eps F) g), until condition() < tol.  Returns
+        (iterations, condition); ``self.history`` holds (condition,
+        functional, eps) per iteration."""
+        eps, last_norm, last = 3000.0, 0.0, None
+        self.history = []
+        cond = math.inf
+        for it in range(max_iter):
+            cond = self.condition()
+            self.history.append((cond, self(), eps))
+            if self.verbose:
+                print(f"iter = {it + 1}, cond = {cond}, term = {self.history[-1][1]}, eps = {eps}")
+            if cond < tol:
+                return it, cond
+            F = self.deriv()
+            norm = sum_real_trace_prod(F, F)
+            if it > 20:
+                beta = (norm - sum_real_trace_prod(F, last)) / last_norm
+            last = F.clone()
+            if it > 20 and beta > 0:
+                add_mul(F, beta, last)
+            eps = self.line_search(F, eps)
+            evolve(self.g, F, eps)
+            reunitize(self.g)
+            last_norm = norm
+        return max_iter, cond

@@ -67,6 +67,15 @@ template <class G> Lattice<G> wilsonDeriv(GaugeField<G> const &U, int mu, double
 	return F;
 }
 
+// lhs += a * rhs, the product rounded before the sum (add_mul,
+// src/lattice/lattice.h:358-363; `lhs += rhs * a`, lattice.h:302-309, 349-355)
+template <class G> void add_mul(Field<G> &lhs, double a, Field<G> const &rhs)
+{
+	if (lhs.nfields() != rhs.nfields())
+		throw std::runtime_error("add_mul: fields differ in the number of matrices per site");
+	ctx().check(mesh_su3_axpy(ctx().get(), lhs.grid().c(), lhs.nfields(), G::prec, lhs.data(), a, rhs.data()));
+}
+
 // project_on_group_fast on every link (src/gauge/utils.h:80-91)
 template <class G> void reunitize(Field<G> &U)
 {
@@ -368,6 +377,67 @@ template <class G> struct Landau
 				break;
 		}
 		return it;
+	}
+
+	// abscissa of the vertex of the parabola through three points (landau.h:68-75)
+	static double solve_quadratic(double x1, double x2, double x3, double y1, double y2, double y3)
+	{
+		return 0.5 * ((x2 * x2 - x3 * x3) * y1 - (x1 * x1 - x3 * x3) * y2 + (x1 * x1 - x2 * x2) * y3) /
+		       ((x2 - x3) * y1 - (x1 - x3) * y2 + (x1 - x2) * y3);
+	}
+
+	// step length along F (landau.h:77-98)
+	double line_search(Lattice<G> const &F, double eps) const
+	{
+		double base = (*this)();
+		Lattice<G> h(g);
+		auto trial = [&](double e) {
+			h = g;
+			evolve(h, F, e);
+			return (*this)(h);
+		};
+		double y1 = trial(eps);
+		while (y1 <= base)
+		{
+			eps *= 0.5;
+			y1 = trial(eps);
+			if (eps < 1e-10)
+				return 0.0;
+		}
+		double y2 = trial(eps * 1.2);
+		if (y2 > y1)
+			return 1.2 * eps;
+		double y3 = trial(eps * 0.8);
+		return solve_quadratic(eps, 1.2 * eps, 0.8 * eps, y1, y2, y3);
+	}
+
+	// The reference's solver (landau.h:100-136): ascent along deriv() with the
+	// line search, Polak-Ribiere steps after iteration 20 (as written there,
+	// `last = F` precedes `F += last * beta`, so the step is (1 + beta) F), then
+	// g = reunitize(exp(eps F) g), until condition() < tol; returns iterations.
+	int run_ascent(int max_iter, double tol = 1e-10)
+	{
+		double eps = 3000.0, lastNorm = 0.0;
+		Lattice<G> last(U.grid());
+		for (int iter = 0; iter < max_iter; ++iter)
+		{
+			double cond = condition();
+			if (verbose)
+				std::printf("iter = %d, cond = %.6e, term = %.12f, eps = %.6e\n", iter + 1, cond, (*this)(), eps);
+			if (cond < tol)
+				return iter;
+			Lattice<G> F = deriv();
+			double norm = sum_real_trace_prod(F, F);
+			double beta = iter > 20 ? (norm - sum_real_trace_prod(F, last)) / lastNorm : 0.0;
+			last = F;
+			if (iter > 20 && beta > 0)
+				add_mul(F, beta, last);
+			eps = line_search(F, eps);
+			evolve(g, F, eps);
+			reunitize(g);
+			lastNorm = norm;
+		}
+		return max_iter;
 	}
 };
 

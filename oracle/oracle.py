@@ -79,6 +79,8 @@ def _declare(L):
         g("orc_landau_or_halfsweep").argtypes = [rp, C.c_void_p, _ip, C.c_int, C.c_int, C.c_double]
     L.orc_su3_force_f64.argtypes = [_dp, _ip, C.c_int, C.c_int, C.c_double, _dp]
     L.orc_landau_deriv_f64.argtypes = [_dp, _dp, _ip, C.c_int, _dp]
+    L.orc_su3_exp_mul_field_f64.argtypes = [_dp, _dp, C.c_int64, C.c_double]
+    L.orc_axpy_f64.argtypes = [_dp, C.c_double, _dp, C.c_int64]
     L.orc_su3_random_transform_f64.argtypes = [_dp, _ip, C.c_int, C.c_uint64]
     L.orc_su3_gauge_transform_f64.argtypes = [_dp, _dp, _ip, C.c_int]
 
@@ -295,6 +297,84 @@ def landau_or_halfsweep(U, dims, parity, omega, h=None):
     assert U.flags.c_contiguous and (h is None or h.flags.c_contiguous)
     getattr(lib(), "orc_landau_or_halfsweep_" + _sfx(U))(_real(U), _hptr(h), _dims(dims), len(dims),
                                                          parity, omega)
+
+
+def exp_mul_field(h, F, t):
+    """exp(t TA(F)) h per matrix (new array; F, h complex128 of equal shape)."""
+    h = np.ascontiguousarray(h, np.complex128).copy()
+    F = np.ascontiguousarray(F, np.complex128)
+    assert F.shape == h.shape
+    lib().orc_su3_exp_mul_field_f64(_real(h), _real(F), h.size // 9, t)
+    return h
+
+
+def axpy(y, a, x):
+    """y + (x * a), product and sum rounded separately (new array)."""
+    y = np.ascontiguousarray(y, np.complex128).copy()
+    x = np.ascontiguousarray(x, np.complex128)
+    assert x.shape == y.shape
+    lib().orc_axpy_f64(_real(y), a, _real(x), 2 * y.size)
+    return y
+
+
+def sum_real_trace_prod(A, B):
+    """sum_x Re tr(A_x B_x) (src/gauge/utils.h:29)."""
+    return float(np.einsum("xij,xji->", A.reshape(-1, 3, 3), B.reshape(-1, 3, 3)).real)
+
+
+def landau_solve_quadratic(x1, x2, x3, y1, y2, y3):
+    """Abscissa of the vertex of the parabola through three points
+    (src/gauge/landau.h:68-75)."""
+    return 0.5 * ((x2 * x2 - x3 * x3) * y1 - (x1 * x1 - x3 * x3) * y2 + (x1 * x1 - x2 * x2) * y3) / \
+        ((x2 - x3) * y1 - (x1 - x3) * y2 + (x1 - x2) * y3)
+
+
+def landau_line_search(U, dims, h, F, eps):
+    """Step length along F from the transformation h (src/gauge/landau.h:77-98):
+    halve eps until F[exp(eps F) h] exceeds F[h], try 1.2 eps, else the vertex
+    of the parabola through eps, 1.2 eps and 0.8 eps."""
+    base = landau_functional(U, dims, h)
+    trial = lambda e: landau_functional(U, dims, exp_mul_field(h, F, e))
+    y1 = trial(eps)
+    while y1 <= base:
+        eps *= 0.5
+        y1 = trial(eps)
+        if eps < 1e-10:
+            return 0.0
+    y2 = trial(eps * 1.2)
+    if y2 > y1:
+        return 1.2 * eps
+    y3 = trial(eps * 0.8)
+    return landau_solve_quadratic(eps, 1.2 * eps, 0.8 * eps, y1, y2, y3)
+
+
+def landau_ascent(U, dims, max_iter, tol=1e-10, h=None):
+    """The reference's Landau solver (src/gauge/landau.h:100-136): steepest
+    ascent with the line search, Polak-Ribiere steps after iteration 20 (as
+    written there, `last = F` precedes `F += last * beta`, so the step is
+    (1 + beta) F), then g = reunitize(exp(eps F) g); stops when condition() <
+    tol.  Returns (g, history of (condition, functional, eps) per iteration)."""
+    nd, V = len(dims), volume(dims)
+    g = np.tile(np.eye(3, dtype=np.complex128), (1, V, 1, 1)) if h is None else \
+        np.ascontiguousarray(h, np.complex128).reshape(1, V, 3, 3).copy()
+    eps, last_norm, hist = 3000.0, 0.0, []
+    last = None
+    for it in range(max_iter):
+        cond = landau_divsum(U, dims, g) / (nd * V)
+        hist.append((cond, landau_functional(U, dims, g), eps))
+        if cond < tol:
+            break
+        F = landau_deriv(U, g, dims).reshape(1, V, 3, 3)
+        norm = sum_real_trace_prod(F, F)
+        if it > 20:
+            beta = (norm - sum_real_trace_prod(F, last)) / last_norm
+        last = F.copy()
+        if it > 20 and beta > 0:
+            F = axpy(F, beta, last)
+        eps = landau_line_search(U, dims, g, F, eps)
+        g = reunit_field(exp_mul_field(g, F, eps))
+        last_norm = norm
+    return g, hist
 
 
 def random_transform(dims, seed):

@@ -707,6 +707,31 @@ void orc_su3_reunit_field_f32(float *U, int64_t nlinks)
 		std::memcpy(U + 18 * i, &a, sizeof a);
 	}
 }
+// h = exp(t TA(F)) h for nlinks matrices (the gauge-transformation update
+// exp(F * eps) * g of src/gauge/landau.h:131, with the same link evolution as
+// the molecular dynamics)
+void orc_su3_exp_mul_field_f64(double *h, double const *F, int64_t nlinks, double t)
+{
+#pragma omp parallel for schedule(static)
+	for (int64_t i = 0; i < nlinks; ++i)
+	{
+		mat3<double> a, f;
+		std::memcpy(&a, h + 18 * i, sizeof a);
+		std::memcpy(&f, F + 18 * i, sizeof f);
+		a = evolve_link(a, f, t);
+		std::memcpy(h + 18 * i, &a, sizeof a);
+	}
+}
+// y = y + (a x) per real, product and sum rounded separately as the
+// reference's `F += last * beta` (src/lattice/lattice.h:302-309, 349-355)
+void orc_axpy_f64(double *y, double a, double const *x, int64_t n)
+{
+	for (int64_t i = 0; i < n; ++i)
+	{
+		double p = x[i] * a;
+		y[i] = y[i] + p;
+	}
+}
 void orc_su3_kinetic_sites_f64(double const *P, int const *dims, int nd, double *K)
 {
 	Geo g(dims, nd);

@@ -30,6 +30,20 @@ def test_cpp_hmc_driver(precision):
     assert "HMC summary" in r.stdout and "landau:" in r.stdout and "time in force_update" in r.stdout
 
 
+def test_cpp_hmc_driver_landau_ascent():
+    """--landau-ascent: the reference's line-search solver (Landau<G>::run_ascent)
+    on the final configuration reaches the reference's stopping condition."""
+    r = subprocess.run([os.path.join(BIN, "mesh_hmc"), "--geom", "4", "4", "4", "4", "--count", "4", "--therm", "2",
+                        "--scheme", "2mn", "--epsilon", "0.5", "--substeps", "6", "--seed", "3", "--precision", "2",
+                        "--landau-ascent"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    m = re.search(r"landau ascent: (\d+) iterations, condition = ([0-9.e+-]+), functional = ([0-9.]+)", r.stdout)
+    assert m, r.stdout[-2000:]
+    assert int(m.group(1)) < 100000 and float(m.group(2)) < 1e-10
+
+
 @pytest.mark.parametrize("precision", ["2", "1"])
 def test_cpp_hmc_driver_u1(precision):
     """--group u1 on a 16x16 lattice (Hmc<U1<T>>): the mean plaquette of one

@@ -131,3 +131,56 @@ def test_landau_finite_difference_gpu(ctx):
     gauge.evolve(lan.g, f, eps)
     new = lan()
     assert abs((new - old) / expected - 1) < 1e-4
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_add_mul_bitexact(ctx, prec):
+    """lhs += a * rhs (mesh_su3_axpy) against the oracle's y + (x a)."""
+    sh = [4, 2, 6, 2]
+    x = O.random_gauge(sh, 31, _pname(prec))
+    y = O.random_gauge(sh, 32, _pname(prec))
+    _, fx = _up(ctx, sh, prec, x)
+    _, fy = _up(ctx, sh, prec, y)
+    a = 0.73125
+    gauge.add_mul(fy, a, fx)
+    if prec == F64:
+        np.testing.assert_array_equal(fy.download(), O.axpy(y, a, x))
+    else:
+        ref = y + (x * np.float32(a)).astype(np.complex64)
+        np.testing.assert_array_equal(fy.download(), ref.astype(np.complex64))
+
+
+@pytest.mark.parametrize("case", ["pure_gauge", "thermalised"])
+def test_ascent_against_oracle(ctx, case):
+    """The reference's solver on the GPU (Landau.run_ascent over the
+    functional, derivative, exp(eps F) g, add_mul and reunitize entry points)
+    against its CPU restatement on the same field: the same iterates to
+    rounding level for the first iterations, and on a gauge copy of the unit
+    field both reach the closed-form maximum F = 1."""
+    sh = [4, 4, 4, 4]
+    V = O.volume(sh)
+    if case == "pure_gauge":
+        h = O.random_transform(sh, 10)
+        Uu = np.zeros((4, V, 3, 3), np.complex128)
+        Uu[:, :] = np.eye(3)
+        Uh = O.gauge_transform(Uu, h, sh)
+        n_iter = 200
+    else:
+        Uh = O.random_gauge(sh, 33)
+        for t in range(3):
+            O.hmc_trajectory(Uh, sh, 6.0, O.make_deltas("2lf", 0.5, 10), 33, t, metropolis=False)
+        n_iter = 30
+    _, U = _up(ctx, sh, F64, Uh)
+    lan = gauge.Landau(U)
+    it, cond = lan.run_ascent(n_iter)
+    _, hist = O.landau_ascent(Uh, sh, n_iter)
+    k = min(len(hist), len(lan.history), 25)
+    for i in range(k):
+        c_g, f_g, e_g = lan.history[i]
+        c_o, f_o, e_o = hist[i]
+        assert abs(f_g - f_o) <= 1e-9 * abs(f_o), (i, f_g, f_o)
+        assert abs(c_g - c_o) <= 1e-6 * abs(c_o) + 1e-14, (i, c_g, c_o)
+        assert abs(e_g - e_o) <= 1e-6 * abs(e_o), (i, e_g, e_o)
+    if case == "pure_gauge":
+        assert cond < 1e-10 and hist[-1][0] < 1e-10, (cond, hist[-1])
+        assert lan() > 1 - 1e-10

@@ -280,3 +280,39 @@ def test_landau_tracking_transform():
         O.landau_or_halfsweep(Ug, dims, 1, 1.7, h)
     np.testing.assert_allclose(O.gauge_transform(U, h, dims), Ug, rtol=0, atol=1e-13)
     assert abs(O.landau_functional(U, dims, h) - O.landau_functional(Ug, dims)) < 1e-14
+
+
+def test_landau_solve_quadratic_known_answer():
+    """Vertex of the parabola through three points (src/gauge/landau.h:68-75)."""
+    f = lambda x: -3.0 * (x - 1.7) ** 2 + 2.0
+    xs = (0.5, 0.6, 0.4)
+    assert abs(O.landau_solve_quadratic(*xs, *(f(x) for x in xs)) - 1.7) < 1e-12
+
+
+def test_landau_ascent_pure_gauge():
+    """The reference's solver (line-search ascent, Polak-Ribiere steps after
+    iteration 20) restated on the CPU: on a gauge copy of the unit field it
+    reaches the closed-form maximum F = 1 with a functional that never
+    decreases, and the transformation it finds maps the field back to the
+    unit field (U^g = 1 up to rounding, since g = c h^+ for a constant c)."""
+    dims = [4, 4, 4, 4]
+    V = O.volume(dims)
+    h = O.random_transform(dims, 10)
+    Uu = np.zeros((4, V, 3, 3), np.complex128)
+    Uu[:, :] = np.eye(3)
+    U = O.gauge_transform(Uu, h, dims)
+    g, hist = O.landau_ascent(U, dims, 200)
+    cond, F, _ = zip(*hist)
+    assert len(hist) > 22 and cond[-1] < 1e-10, (len(hist), cond[-1])
+    assert F[-1] > 1 - 1e-10
+    assert all(b >= a for a, b in zip(F, F[1:]))
+    np.testing.assert_allclose(O.gauge_transform(U, g, dims), Uu, rtol=0, atol=1e-5)
+
+
+def test_axpy_rounds_product_then_sum():
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(2, 5, 3, 3)) + 1j * rng.normal(size=(2, 5, 3, 3))
+    y = rng.normal(size=(2, 5, 3, 3)) + 1j * rng.normal(size=(2, 5, 3, 3))
+    a = 0.3712
+    r = O.axpy(y, a, x)
+    assert np.array_equal(r.real, y.real + x.real * a) and np.array_equal(r.imag, y.imag + x.imag * a)
