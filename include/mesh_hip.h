@@ -110,6 +110,8 @@ void *mesh_stream(mesh_ctx *ctx);
  *       updates P_mu(x) writes exp(t P_mu(x)) U_mu(x) into a second field),
  *       and so do the momenta refresh and the first link step; same
  *       arithmetic, so the same bits;
+ *   "u1_general" = 0 (default): U(1) lattices that fit use the LDS-resident
+ *       kernels; 1 sends every U(1) lattice through the site-parallel ones;
  *   "compact_momenta" = 1 (default): the fused trajectory keeps its momenta
  *       in a compact antihermitian layout of 9 reals per link (upper
  *       off-diagonal entries and the imaginary diagonal); 0 keeps the
@@ -271,23 +273,29 @@ int mesh_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, int prec, void *
 int mesh_su3_gauge_transform(mesh_ctx *ctx, const mesh_grid *g, int prec, void *Uout, const void *U,
                              const void *h);
 
-/* ---- U(1) on 2-d lattices (src/gauge/groups.h:29-108, hmc.cpp:109-110) -----
+/* ---- U(1) (src/gauge/groups.h:29-108, hmc.cpp:109-110) ----------------------
  *
- * A batch of `batch` independent configurations ("chains") of an L0 x L1
- * periodic lattice (grid nd = 2), FP32 angles theta[c][mu][x] in [-pi, pi),
- * x = x0 * L1 + x1.  theta_P(x) = theta_0(x) + theta_1(x+0) - theta_0(x+1)
- * - theta_1(x); S = beta sum_x (1 - cos theta_P); the topological charge is
- * Q = sum_x wrap(theta_P(x)) / 2 pi.  Device buffers: theta 2 V floats per
- * configuration; per-configuration outputs are device doubles. */
+ * A batch of `batch` independent configurations ("chains") of a periodic
+ * lattice of 2 to 4 dimensions, FP32 angles theta[c][mu][x] in [-pi, pi), sites
+ * row-major (dimension 0 slowest; x = x0 * L1 + x1 in 2-d).  theta_{mu nu}(x) =
+ * theta_mu(x) + theta_nu(x+mu) - theta_mu(x+nu) - theta_nu(x);
+ * S = beta sum_x sum_{mu<nu} (1 - cos theta_{mu nu}); on 2-d lattices the
+ * topological charge is Q = sum_x wrap(theta_01(x)) / 2 pi.  Device buffers:
+ * theta Nd V floats per configuration; per-configuration outputs are device
+ * doubles.  2-d lattices of up to 4096 sites (HMC) / 8192 sites (random start,
+ * observables) run in LDS-resident kernels, one workgroup per configuration;
+ * everything else in site-parallel kernels with the same arithmetic per link
+ * (option "u1_general" = 1 selects these for every lattice). */
 
 /* uniform (Haar) random angles: random_gauge_field for U1 (utils.h:43-52) */
 int mesh_u1_random(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, uint64_t seed);
-/* out_dev[4c + {0,1,2,3}] = {sum_x (1 - cos theta_P), Q, sum_x cos theta_P, sum_x wrap(theta_P)} */
+/* out_dev[4c + {0,1,2,3}] = {sum (1 - cos theta_P), Q, sum cos theta_P, sum wrap(theta_P)} over
+ * sites and planes; Q = 0 on lattices of more than 2 dimensions */
 int mesh_u1_observables(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *theta, double *out_dev);
 /* One HMC trajectory per configuration (runHmcUpdate, hmc.cpp:75-107): momenta
  * q ~ N(0, 1/2) per link, K = sum q^2, link step theta += t q, momentum step
  * q -= t (beta/2) Im(U S), accept on u < exp(-Delta H) (metropolis != 0).
- * deltas: host step table (mesh_make_deltas), odd length; V <= 4096.
+ * deltas: host step table (mesh_make_deltas), odd length.
  * out_dev[4c + {0,1,2,3}] = {Delta H, accepted, u, S_new / beta}. */
 int mesh_u1_hmc(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, const double *deltas, int ndeltas,
                 double beta, uint64_t seed, uint32_t traj, int metropolis, double *out_dev);

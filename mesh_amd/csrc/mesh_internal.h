@@ -48,6 +48,9 @@ struct mesh_ctx
 	int compact_p = 1;      // fused trajectory: momenta in 9 reals per link
 	void *md_spare = nullptr; // third gauge field for the fused trajectory
 	size_t md_spare_bytes = 0;
+	int u1_general = 0;       // U(1): site-parallel kernels also for small 2-d lattices
+	void *u1_ws = nullptr;    // workspace of the site-parallel U(1) kernels
+	size_t u1_ws_bytes = 0;
 	// profiler: events around launches, resolved after a stream sync
 	int prof_on = 0;
 	std::vector<hipEvent_t> prof_pool;

@@ -248,6 +248,8 @@ int mesh_ctx_destroy(mesh_ctx *ctx)
 		(void)hipFree(ctx->aux);
 	if (ctx->md_spare)
 		(void)hipFree(ctx->md_spare);
+	if (ctx->u1_ws)
+		(void)hipFree(ctx->u1_ws);
 	(void)hipFree(ctx->red_dev);
 	(void)hipHostFree(ctx->red_host);
 	(void)hipStreamDestroy(ctx->stream);
@@ -323,6 +325,8 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value)
 		ctx->fuse_md = value != 0;
 	else if (!std::strcmp(name, "compact_momenta"))
 		ctx->compact_p = value != 0;
+	else if (!std::strcmp(name, "u1_general"))
+		ctx->u1_general = value != 0;
 	else
 		return mesh_fail(ctx, MESH_E_ARG, std::string("unknown option ") + name);
 	return MESH_OK;

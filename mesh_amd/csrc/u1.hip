@@ -80,10 +80,18 @@ template <int NQ> __device__ __forceinline__ void bsum(double (&v)[NQ], double (
 	__syncthreads();
 }
 
+// Philox counter of link (c, mu, x) of a stream: (x, c << 1 | mu bit 0,
+// step << 5 | mu bit 1 << 4, tag) -- on 2-d lattices (mu < 2) simply
+// (x, c << 1 | mu, step << 5, tag)
+__device__ __forceinline__ u4 u1_ctr(uint32_t tag, uint32_t step, int c, int mu, int x)
+{
+	return {(uint32_t)x, ((uint32_t)c << 1) | (uint32_t)(mu & 1), (step << 5) | ((uint32_t)(mu >> 1) << 4), tag};
+}
+
 // one normal for link (c, mu, x) of a stream
 __device__ __forceinline__ double u1_normal(uint64_t seed, uint32_t tag, uint32_t step, int c, int mu, int x)
 {
-	u4 ctr = {(uint32_t)x, ((uint32_t)c << 1) | (uint32_t)mu, step << 5, tag};
+	u4 ctr = u1_ctr(tag, step, c, mu, x);
 	u4 r = philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
 	double z0, z1;
 	normal_pair(r, &z0, &z1);
@@ -93,7 +101,7 @@ __device__ __forceinline__ double u1_normal(uint64_t seed, uint32_t tag, uint32_
 // uniform in [0,1) for (c, mu, x) of a stream
 __device__ __forceinline__ double u1_uniform(uint64_t seed, uint32_t tag, uint32_t step, int c, int mu, int x)
 {
-	u4 ctr = {(uint32_t)x, ((uint32_t)c << 1) | (uint32_t)mu, step << 5, tag};
+	u4 ctr = u1_ctr(tag, step, c, mu, x);
 	u4 r = philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
 	return u01_from_bits(r.x, r.y) - 0x1p-53;
 }
@@ -226,6 +234,343 @@ __global__ __launch_bounds__(U1_THREADS) void k_u1_hmc(U1Geo g, float *th, const
 			tg[i] = t[i];
 }
 
+// ---- any dimension 2 <= nd <= 4, one thread per (configuration, site) -------
+//
+// Same conventions and arithmetic per link as the LDS kernels above: theta and
+// momenta in FP32, theta_{mu nu}(x) = theta_mu(x) + theta_nu(x+mu) -
+// theta_mu(x+nu) - theta_nu(x) summed left to right, link step theta = wrap(theta
+// + t q), momentum step q_mu(x) -= t (beta/2) sum_{nu != mu} [sin theta_{mu nu}(x)
+// - sin theta_{mu nu}(x - nu)] (nu ascending; theta_{nu mu} = -theta_{mu nu}),
+// energies per site in FP64, differenced per site for Delta H.  On a 2-d lattice
+// every link therefore gets the same bits as from the LDS kernels.
+
+struct U1NGeo
+{
+	int nd, V, np;   // dimensions, sites, planes nd (nd - 1) / 2
+	int L[4], st[4]; // extents, row-major strides (dimension 0 slowest)
+};
+
+__device__ __forceinline__ int n_fwd(U1NGeo const &g, int x, int mu)
+{
+	int c = (x / g.st[mu]) % g.L[mu];
+	return c + 1 == g.L[mu] ? x - (g.L[mu] - 1) * g.st[mu] : x + g.st[mu];
+}
+__device__ __forceinline__ int n_bwd(U1NGeo const &g, int x, int mu)
+{
+	int c = (x / g.st[mu]) % g.L[mu];
+	return c == 0 ? x + (g.L[mu] - 1) * g.st[mu] : x - g.st[mu];
+}
+// plane index of (mu, nu), mu < nu, in the order (0,1), (0,2), ..., (nd-2, nd-1)
+__device__ __forceinline__ int plane_of(U1NGeo const &g, int mu, int nu)
+{
+	return mu * (2 * g.nd - mu - 1) / 2 + (nu - mu - 1);
+}
+// plaquette angle of plane (mu, nu) at x of one configuration t[nd V] (unwrapped)
+__device__ __forceinline__ float plaq_n(U1NGeo const &g, const float *t, int mu, int nu, int x)
+{
+	return t[mu * g.V + x] + t[nu * g.V + n_fwd(g, x, mu)] - t[mu * g.V + n_fwd(g, x, nu)] - t[nu * g.V + x];
+}
+
+__global__ __launch_bounds__(256) void k_u1n_random(U1NGeo g, long n, float *th, uint64_t seed)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const long per = (long)g.nd * g.V;
+	int c = (int)(i / per);
+	int r = (int)(i - c * per);
+	int mu = r / g.V, x = r - mu * g.V;
+	double u = u1_uniform(seed, TAG_U1RAND, 0u, c, mu, x);
+	th[i] = (float)(6.283185307179586477 * u - 3.141592653589793238);
+}
+
+// momenta, a copy of the start configuration, per-site energy before
+__global__ __launch_bounds__(256) void k_u1n_start(U1NGeo g, int batch, const float *__restrict__ th,
+                                                   float *__restrict__ th_old, float *__restrict__ q,
+                                                   double *__restrict__ e_old, double beta, uint64_t seed,
+                                                   uint32_t traj)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (long)batch * g.V)
+		return;
+	int c = (int)(i / g.V), x = (int)(i - (long)c * g.V);
+	const long o = (long)c * g.nd * g.V;
+	const float *t = th + o;
+	double e = 0.0;
+	for (int mu = 0; mu < g.nd; ++mu)
+		for (int nu = mu + 1; nu < g.nd; ++nu)
+			e += beta * (1.0 - cos((double)plaq_n(g, t, mu, nu, x)));
+	for (int mu = 0; mu < g.nd; ++mu)
+	{
+		float qq = (float)(u1_normal(seed, TAG_U1MOM, traj, c, mu, x) * 0.70710678118654752440);
+		q[o + mu * g.V + x] = qq;
+		e += (double)qq * qq;
+		th_old[o + mu * g.V + x] = t[mu * g.V + x];
+	}
+	e_old[i] = e;
+}
+
+__global__ __launch_bounds__(256) void k_u1n_link(long n, float *__restrict__ th, const float *__restrict__ q, float d)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		th[i] = wrap_pi(fmaf(d, q[i], th[i]));
+}
+
+// sin theta_P of every plane at every site
+__global__ __launch_bounds__(256) void k_u1n_sinp(U1NGeo g, int batch, const float *__restrict__ th,
+                                                  float *__restrict__ sp)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (long)batch * g.V)
+		return;
+	int c = (int)(i / g.V), x = (int)(i - (long)c * g.V);
+	const float *t = th + (long)c * g.nd * g.V;
+	float *s = sp + (long)c * g.np * g.V;
+	for (int mu = 0; mu < g.nd; ++mu)
+		for (int nu = mu + 1; nu < g.nd; ++nu)
+			s[plane_of(g, mu, nu) * g.V + x] = sinf(plaq_n(g, t, mu, nu, x));
+}
+
+__global__ __launch_bounds__(256) void k_u1n_mom(U1NGeo g, int batch, const float *__restrict__ sp,
+                                                 float *__restrict__ q, float dh)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (long)batch * g.V)
+		return;
+	int c = (int)(i / g.V), x = (int)(i - (long)c * g.V);
+	const float *s = sp + (long)c * g.np * g.V;
+	float *qc = q + (long)c * g.nd * g.V;
+	for (int mu = 0; mu < g.nd; ++mu)
+	{
+		float f = 0.0f;
+		bool first = true;
+		for (int nu = 0; nu < g.nd; ++nu)
+		{
+			if (nu == mu)
+				continue;
+			const int xb = n_bwd(g, x, nu);
+			float a, b;
+			if (mu < nu)
+			{
+				const float *sv = s + plane_of(g, mu, nu) * g.V;
+				a = sv[x];
+				b = sv[xb];
+			}
+			else
+			{
+				const float *sv = s + plane_of(g, nu, mu) * g.V;
+				a = -sv[x];
+				b = -sv[xb];
+			}
+			const float term = a - b;
+			f = first ? term : f + term;
+			first = false;
+		}
+		qc[mu * g.V + x] = fmaf(-dh, f, qc[mu * g.V + x]);
+	}
+}
+
+// block partial sums per configuration of (e_new - e_old, S_new / beta);
+// grid = batch * nbk blocks
+__global__ __launch_bounds__(256) void k_u1n_energy(U1NGeo g, int nbk, const float *__restrict__ th,
+                                                    const float *__restrict__ q, const double *__restrict__ e_old,
+                                                    double beta, double *__restrict__ partials)
+{
+	__shared__ double red[4][4];
+	const int c = blockIdx.x / nbk, b = blockIdx.x - c * nbk;
+	const int x = b * 256 + threadIdx.x;
+	double v[2] = {0.0, 0.0};
+	if (x < g.V)
+	{
+		const long o = (long)c * g.nd * g.V;
+		const float *t = th + o;
+		double sn = 0.0, k = 0.0;
+		for (int mu = 0; mu < g.nd; ++mu)
+			for (int nu = mu + 1; nu < g.nd; ++nu)
+				sn += 1.0 - cos((double)plaq_n(g, t, mu, nu, x));
+		for (int mu = 0; mu < g.nd; ++mu)
+		{
+			double qq = q[o + mu * g.V + x];
+			k += qq * qq;
+		}
+		v[0] = (beta * sn + k) - e_old[(long)c * g.V + x];
+		v[1] = sn;
+	}
+	bsum<2>(v, red);
+	if (threadIdx.x == 0)
+	{
+		partials[2 * (long)blockIdx.x + 0] = v[0];
+		partials[2 * (long)blockIdx.x + 1] = v[1];
+	}
+}
+
+// per configuration: Delta H, Metropolis (the uniform of the LDS kernel)
+__global__ __launch_bounds__(256) void k_u1n_accept(int batch, int nbk, const double *__restrict__ partials,
+                                                    uint64_t seed, uint32_t traj, int metropolis, double *out)
+{
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= batch)
+		return;
+	double dH = 0.0, sn = 0.0;
+	for (int b = 0; b < nbk; ++b)
+	{
+		dH += partials[2 * ((long)c * nbk + b) + 0];
+		sn += partials[2 * ((long)c * nbk + b) + 1];
+	}
+	u4 ctr = {traj, (uint32_t)c, 0u, TAG_U1METRO};
+	u4 r = philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+	double u = u01_from_bits(r.x, r.y);
+	int a = metropolis ? (u < exp(-dH)) : 1;
+	out[4 * c + 0] = dH;
+	out[4 * c + 1] = a;
+	out[4 * c + 2] = u;
+	out[4 * c + 3] = sn;
+}
+
+// rejected configurations get their start angles back
+__global__ __launch_bounds__(256) void k_u1n_restore(long n, long per, float *__restrict__ th,
+                                                     const float *__restrict__ th_old, const double *__restrict__ out)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	if (out[4 * (i / per) + 1] == 0.0)
+		th[i] = th_old[i];
+}
+
+// block partial sums per configuration of (sum (1 - cos), sum cos, sum wrap theta_P)
+__global__ __launch_bounds__(256) void k_u1n_obs(U1NGeo g, int nbk, const float *__restrict__ th,
+                                                 double *__restrict__ partials)
+{
+	__shared__ double red[4][4];
+	const int c = blockIdx.x / nbk, b = blockIdx.x - c * nbk;
+	const int x = b * 256 + threadIdx.x;
+	double v[3] = {0.0, 0.0, 0.0};
+	if (x < g.V)
+	{
+		const float *t = th + (long)c * g.nd * g.V;
+		for (int mu = 0; mu < g.nd; ++mu)
+			for (int nu = mu + 1; nu < g.nd; ++nu)
+			{
+				float p = wrap_pi(plaq_n(g, t, mu, nu, x));
+				double cp = cos((double)p);
+				v[0] += 1.0 - cp;
+				v[1] += cp;
+				v[2] += (double)p;
+			}
+	}
+	bsum<3>(v, red);
+	if (threadIdx.x == 0)
+		for (int k = 0; k < 3; ++k)
+			partials[3 * (long)blockIdx.x + k] = v[k];
+}
+
+// out[4c + {0,1,2,3}] as mesh_u1_observables; Q only on 2-d lattices
+__global__ __launch_bounds__(256) void k_u1n_obs_final(int batch, int nbk, int nd, const double *__restrict__ partials,
+                                                       double *out)
+{
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= batch)
+		return;
+	double v[3] = {0.0, 0.0, 0.0};
+	for (int b = 0; b < nbk; ++b)
+		for (int k = 0; k < 3; ++k)
+			v[k] += partials[3 * ((long)c * nbk + b) + k];
+	out[4 * c + 0] = v[0];
+	out[4 * c + 1] = nd == 2 ? rint(v[2] * (1.0 / 6.283185307179586477)) : 0.0;
+	out[4 * c + 2] = v[1];
+	out[4 * c + 3] = v[2];
+}
+
+int u1n_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, U1NGeo *ug)
+{
+	if (!g || g->nd < 2 || g->nd > 4)
+		return mesh_fail(ctx, MESH_E_SIZE, "U(1) lattices have 2 to 4 dimensions");
+	long V = 1;
+	for (int i = 0; i < g->nd; ++i)
+	{
+		if (g->dims[i] < 2)
+			return mesh_fail(ctx, MESH_E_SIZE, "U(1) lattice extents must be >= 2");
+		V *= g->dims[i];
+	}
+	if (batch < 1 || (long)batch * g->nd * V >= (1L << 31))
+		return mesh_fail(ctx, MESH_E_SIZE, "U(1) batch x links exceeds 2^31");
+	ug->nd = g->nd;
+	ug->V = (int)V;
+	ug->np = g->nd * (g->nd - 1) / 2;
+	int s = 1;
+	for (int i = g->nd - 1; i >= 0; --i)
+	{
+		ug->L[i] = g->dims[i];
+		ug->st[i] = s;
+		s *= g->dims[i];
+	}
+	for (int i = g->nd; i < 4; ++i)
+		ug->L[i] = 1, ug->st[i] = (int)V;
+	return MESH_OK;
+}
+
+// the LDS-resident kernels serve 2-d lattices up to `vmax` sites, unless the
+// context asks for the general kernels (option "u1_general")
+bool u1_lds(mesh_ctx *ctx, const mesh_grid *g, long vmax)
+{
+	return !ctx->u1_general && g && g->nd == 2 && (long)g->dims[0] * g->dims[1] <= vmax;
+}
+
+// context workspace of the general kernels (grown as needed)
+int u1n_workspace(mesh_ctx *ctx, size_t bytes)
+{
+	if (bytes <= ctx->u1_ws_bytes)
+		return MESH_OK;
+	if (ctx->u1_ws)
+	{
+		MESH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+		MESH_HIP(ctx, hipFree(ctx->u1_ws));
+		ctx->u1_ws = nullptr;
+		ctx->u1_ws_bytes = 0;
+	}
+	if (hipMalloc(&ctx->u1_ws, bytes) != hipSuccess)
+		return mesh_fail(ctx, MESH_E_ALLOC, "cannot allocate the U(1) workspace");
+	ctx->u1_ws_bytes = bytes;
+	return MESH_OK;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int u1n_hmc(mesh_ctx *ctx, U1NGeo const &ug, int batch, float *theta, const double *deltas, int ndeltas, double beta,
+            uint64_t seed, uint32_t traj, int metropolis, double *out)
+{
+	const long nl = (long)batch * ug.nd * ug.V, ns = (long)batch * ug.V;
+	const int nbk = (ug.V + 255) / 256;
+	const size_t b_q = align256(nl * sizeof(float)), b_sp = align256((size_t)batch * ug.np * ug.V * sizeof(float)),
+	             b_e = align256(ns * sizeof(double)), b_p = align256((size_t)batch * nbk * 2 * sizeof(double));
+	if (int rc = u1n_workspace(ctx, 2 * b_q + b_sp + b_e + b_p))
+		return rc;
+	char *w = (char *)ctx->u1_ws;
+	float *q = (float *)w, *th_old = (float *)(w + b_q), *sp = (float *)(w + 2 * b_q);
+	double *e_old = (double *)(w + 2 * b_q + b_sp), *partials = (double *)(w + 2 * b_q + b_sp + b_e);
+	const int gl = (int)((nl + 255) / 256), gs = (int)((ns + 255) / 256);
+	k_u1n_start<<<gs, 256, 0, ctx->stream>>>(ug, batch, theta, th_old, q, e_old, beta, seed, traj);
+	const float hb = (float)(0.5 * beta);
+	for (int s = 0; s < ndeltas; ++s)
+	{
+		const float d = (float)deltas[s];
+		if (s % 2 == 0)
+			k_u1n_link<<<gl, 256, 0, ctx->stream>>>(nl, theta, q, d);
+		else
+		{
+			k_u1n_sinp<<<gs, 256, 0, ctx->stream>>>(ug, batch, theta, sp);
+			k_u1n_mom<<<gs, 256, 0, ctx->stream>>>(ug, batch, sp, q, d * hb);
+		}
+	}
+	k_u1n_energy<<<batch * nbk, 256, 0, ctx->stream>>>(ug, nbk, theta, q, e_old, beta, partials);
+	k_u1n_accept<<<(batch + 255) / 256, 256, 0, ctx->stream>>>(batch, nbk, partials, seed, traj, metropolis, out);
+	k_u1n_restore<<<gl, 256, 0, ctx->stream>>>(nl, (long)ug.nd * ug.V, theta, th_old, out);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
 int u1_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, U1Geo *ug)
 {
 	if (!g || g->nd != 2 || g->dims[0] < 2 || g->dims[1] < 2)
@@ -248,6 +593,18 @@ extern "C" {
 int mesh_u1_random(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, uint64_t seed)
 {
 	MESH_CHECK_CTX(ctx);
+	if (!u1_lds(ctx, g, 8192))
+	{
+		U1NGeo ng;
+		if (int rc = u1n_setup(ctx, g, batch, &ng))
+			return rc;
+		if (!theta)
+			return mesh_fail(ctx, MESH_E_ARG, "theta is NULL");
+		long n = (long)batch * ng.nd * ng.V;
+		k_u1n_random<<<(int)((n + 255) / 256), 256, 0, ctx->stream>>>(ng, n, theta, seed);
+		MESH_LAUNCHED(ctx);
+		return MESH_OK;
+	}
 	U1Geo ug;
 	if (int rc = u1_setup(ctx, g, batch, &ug))
 		return rc;
@@ -262,6 +619,21 @@ int mesh_u1_random(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, u
 int mesh_u1_observables(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *theta, double *out_dev)
 {
 	MESH_CHECK_CTX(ctx);
+	if (!u1_lds(ctx, g, 8192))
+	{
+		U1NGeo ng;
+		if (int rc = u1n_setup(ctx, g, batch, &ng))
+			return rc;
+		if (!theta || !out_dev)
+			return mesh_fail(ctx, MESH_E_ARG, "NULL argument to u1_observables");
+		const int nbk = (ng.V + 255) / 256;
+		if (int rc = mesh_reserve_partials(ctx, 3 * (size_t)batch * nbk))
+			return rc;
+		k_u1n_obs<<<batch * nbk, 256, 0, ctx->stream>>>(ng, nbk, theta, ctx->partials);
+		k_u1n_obs_final<<<(batch + 255) / 256, 256, 0, ctx->stream>>>(batch, nbk, ng.nd, ctx->partials, out_dev);
+		MESH_LAUNCHED(ctx);
+		return MESH_OK;
+	}
 	U1Geo ug;
 	if (int rc = u1_setup(ctx, g, batch, &ug))
 		return rc;
@@ -276,6 +648,15 @@ int mesh_u1_hmc(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, cons
                 double beta, uint64_t seed, uint32_t traj, int metropolis, double *out_dev)
 {
 	MESH_CHECK_CTX(ctx);
+	if (!u1_lds(ctx, g, 4096))
+	{
+		U1NGeo ng;
+		if (int rc = u1n_setup(ctx, g, batch, &ng))
+			return rc;
+		if (!theta || !deltas || !out_dev || ndeltas < 1 || ndeltas % 2 != 1)
+			return mesh_fail(ctx, MESH_E_ARG, "bad u1_hmc arguments (deltas must have odd length)");
+		return u1n_hmc(ctx, ng, batch, theta, deltas, ndeltas, beta, seed, traj, metropolis, out_dev);
+	}
 	U1Geo ug;
 	if (int rc = u1_setup(ctx, g, batch, &ug))
 		return rc;

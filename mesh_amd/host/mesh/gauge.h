@@ -224,8 +224,8 @@ template <class G> class Hmc
 	}
 };
 
-// U(1) links as angles on a 2-d lattice: theta[mu][x], FP32 (the reference's
-// GaugeField<U1<T>> of complex<T> links exp(i theta))
+// U(1) links as angles on a 2- to 4-d lattice: theta[mu][x], FP32 (the
+// reference's GaugeField<U1<T>> of complex<T> links exp(i theta))
 class U1Field
 {
 	Grid const *g_;
@@ -234,8 +234,8 @@ class U1Field
   public:
 	explicit U1Field(Grid const &g) : g_(&g)
 	{
-		if (g.ndim() != 2)
-			throw std::runtime_error("U(1) fields are implemented on 2-d lattices");
+		if (g.ndim() < 2 || g.ndim() > 4)
+			throw std::runtime_error("U(1) fields are implemented on lattices of 2 to 4 dimensions");
 		ctx().check(mesh_malloc(ctx().get(), &p_, bytes()));
 	}
 	~U1Field()
@@ -246,20 +246,21 @@ class U1Field
 	U1Field(U1Field const &) = delete;
 	U1Field &operator=(U1Field const &) = delete;
 	Grid const &grid() const { return *g_; }
-	size_t bytes() const { return 2 * g_->size() * sizeof(float); }
+	size_t bytes() const { return g_->ndim() * g_->size() * sizeof(float); }
 	float *data() const { return (float *)p_; }
 	// theta = 0: the unit configuration
 	void fill_unit() { ctx().check(mesh_memclear(ctx().get(), p_, bytes())); }
 	std::vector<float> download() const
 	{
-		std::vector<float> h(2 * g_->size());
+		std::vector<float> h(g_->ndim() * g_->size());
 		ctx().check(mesh_memcpy_to_host(ctx().get(), h.data(), p_, bytes()));
 		return h;
 	}
 };
 
-// HMC for U(1) on a 2-d lattice (Hmc<U1<T>>, src/gauge/hmc.cpp:109-110): the
-// whole trajectory runs in one kernel (mesh_u1_hmc, batch of one)
+// HMC for U(1) (Hmc<U1<T>>, src/gauge/hmc.cpp:109-110) through mesh_u1_hmc
+// with a batch of one: on small 2-d lattices the whole trajectory is one
+// kernel, otherwise the site-parallel kernels
 template <class T> class Hmc<U1<T>>
 {
   public:
@@ -306,7 +307,8 @@ template <class T> class Hmc<U1<T>>
 		++traj;
 		double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 		accept_history.push_back(h[1]);
-		plaq_history.push_back(h[6] / (double)g.size()); // mean cos theta_P = Re tr P / Nc
+		const int np = g.ndim() * (g.ndim() - 1) / 2;
+		plaq_history.push_back(h[6] / ((double)g.size() * np)); // mean cos theta_P = Re tr P / Nc
 		deltaH_history.push_back(h[0]);
 		time_history.push_back(dt);
 	}

@@ -57,13 +57,18 @@ class Buffer:
 
 
 class U1Ensemble:
-    """`batch` independent U(1) configurations of an L0 x L1 lattice."""
+    """`batch` independent U(1) configurations of an L0 x L1 lattice, or of a
+    lattice of extents `dims` (2 to 4 dimensions) when given."""
 
-    def __init__(self, L0, L1, batch, ctx=None):
+    def __init__(self, L0, L1, batch, ctx=None, dims=None):
         self.ctx = ctx or default_context()
-        self.grid = Grid((L0, L1))
-        self.L0, self.L1, self.batch, self.V = L0, L1, batch, L0 * L1
-        self.theta = Buffer(batch * 2 * self.V, np.float32, self.ctx)
+        self.dims = tuple(dims) if dims is not None else (L0, L1)
+        self.grid = Grid(self.dims)
+        self.nd = len(self.dims)
+        self.V = int(np.prod(self.dims))
+        self.np = self.nd * (self.nd - 1) // 2
+        self.L0, self.L1, self.batch = self.dims[0], self.dims[1], batch
+        self.theta = Buffer(batch * self.nd * self.V, np.float32, self.ctx)
         self._obs = Buffer(4 * batch, np.float64, self.ctx)
         self._hmc = Buffer(4 * batch, np.float64, self.ctx)
 
@@ -74,16 +79,16 @@ class U1Ensemble:
         self.theta.zero()
 
     def angles(self):
-        return self.theta.download().reshape(self.batch, 2, self.L0, self.L1)
+        return self.theta.download().reshape(self.batch, self.nd, *self.dims)
 
     def set_angles(self, a):
         self.theta.upload(np.asarray(a, np.float32))
 
     def observables(self):
-        """dict of per-chain arrays: action/beta, Q, sum cos theta_P."""
+        """dict of per-chain arrays: action/beta, Q (2-d), mean cos theta_P."""
         self.ctx.call("mesh_u1_observables", self.grid.ref, self.batch, self.theta.ptr, self._obs.ptr)
         o = self._obs.download().reshape(self.batch, 4)
-        return dict(S=o[:, 0], Q=o[:, 1], cos=o[:, 2] / self.V)
+        return dict(S=o[:, 0], Q=o[:, 1], cos=o[:, 2] / (self.V * self.np))
 
     def hmc(self, beta, deltas, seed, traj, metropolis=True):
         """One trajectory of every chain; returns dict(dH, acc, S) arrays."""

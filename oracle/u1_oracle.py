@@ -91,6 +91,82 @@ def hmc_trajectory(th, q, beta, deltas):
     return th, q, e1 - e0
 
 
+# ------------------------------------------------- any dimension (2 <= nd <= 4)
+# th: (nd, L0, ..., L_{nd-1}); th[mu] shifted along axis mu is th[mu](x + mu).
+
+def _ctr(chain, mu, step):
+    """Philox counter words 1 and 2 of link (chain, mu): bit 0 of mu in word 1,
+    bit 1 of mu in bit 4 of word 2 (mesh_amd/csrc/u1.hip u1_ctr)."""
+    return (chain << 1) | (mu & 1), (step << 5) | ((mu >> 1) << 4)
+
+
+def nd_random_angles(seed, chain, dims):
+    nd, V = len(dims), int(np.prod(dims))
+    out = np.zeros((nd, V), np.float32)
+    for mu in range(nd):
+        c1, c2 = _ctr(chain, mu, 0)
+        for x in range(V):
+            u = _draw(seed, x, c1, c2, TAG_U1RAND)[0] - 2.0 ** -53
+            out[mu, x] = np.float32(6.283185307179586477 * u - 3.141592653589793238)
+    return out.reshape(nd, *dims)
+
+
+def nd_momenta(seed, traj, chain, dims):
+    nd, V = len(dims), int(np.prod(dims))
+    q = np.zeros((nd, V), np.float32)
+    for mu in range(nd):
+        c1, c2 = _ctr(chain, mu, traj)
+        for x in range(V):
+            q[mu, x] = np.float32(_draw(seed, x, c1, c2, TAG_U1MOM)[1] * 0.70710678118654752440)
+    return q.reshape(nd, *dims)
+
+
+def nd_plaquettes(th):
+    """theta_{mu nu}(x) for the planes mu < nu in the order (0,1), (0,2), ...:
+    theta_mu(x) + theta_nu(x+mu) - theta_mu(x+nu) - theta_nu(x)."""
+    nd = th.shape[0]
+    return [th[mu] + np.roll(th[nu], -1, axis=mu) - np.roll(th[mu], -1, axis=nu) - th[nu]
+            for mu in range(nd) for nu in range(mu + 1, nd)]
+
+
+def nd_action_over_beta(th):
+    return float(sum(np.sum(1 - np.cos(p)) for p in nd_plaquettes(th)))
+
+
+def nd_dS(th, beta):
+    """dS/dtheta_mu(x) = beta sum_{nu != mu} [sin theta_{mu nu}(x) - sin theta_{mu nu}(x - nu)]."""
+    nd = th.shape[0]
+    P = nd_plaquettes(th)
+    k = 0
+    s = {}
+    for mu in range(nd):
+        for nu in range(mu + 1, nd):
+            s[mu, nu] = np.sin(P[k])
+            s[nu, mu] = -s[mu, nu]
+            k += 1
+    g = np.zeros_like(th, dtype=np.float64)
+    for mu in range(nd):
+        for nu in range(nd):
+            if nu != mu:
+                g[mu] += s[mu, nu] - np.roll(s[mu, nu], 1, axis=nu)
+    return beta * g
+
+
+def nd_hmc_trajectory(th, q, beta, deltas):
+    """float64 trajectory over a step table (hmc.h:38-59) in any dimension;
+    returns (theta', q', Delta H)."""
+    th = np.asarray(th, np.float64).copy()
+    q = np.asarray(q, np.float64).copy()
+    e0 = beta * nd_action_over_beta(th) + np.sum(q * q)
+    for i, d in enumerate(deltas):
+        if i % 2 == 0:
+            th = wrap(th + d * q)
+        else:
+            q = q - d * 0.5 * nd_dS(th, beta)
+    e1 = beta * nd_action_over_beta(th) + np.sum(q * q)
+    return th, q, e1 - e0
+
+
 # ---------------------------------------------------------------- exact results
 
 def _c_nu(beta, nu):

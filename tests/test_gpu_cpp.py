@@ -60,3 +60,21 @@ def test_cpp_hmc_driver_u1(precision):
     assert acc > 0.7
     # the printed error ignores autocorrelation: allow a generous multiple
     assert abs(p - U.exact_mean_cos(2.0, 256)) < 10 * err + 2e-3, (p, err)
+
+
+@pytest.mark.parametrize("precision", ["2", "1"])
+def test_cpp_hmc_driver_u1_4d(precision):
+    """--group u1 on the reference's default 4-d geometry class (4^4 here) at
+    strong coupling: the mean plaquette against I1(beta)/I0(beta)."""
+    import scipy.special as sp
+
+    r = subprocess.run([os.path.join(BIN, "mesh_hmc"), "--group", "u1", "--geom", "4", "4", "4", "4", "--beta", "0.2",
+                        "--count", "300", "--therm", "50", "--scheme", "2lf", "--epsilon", "0.5", "--substeps",
+                        "8", "--seed", "6", "--precision", precision],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout[-1500:] + r.stderr
+    m = re.search(r"plaquette = ([0-9.]+) \+- ([0-9.]+)", r.stdout)
+    p, err = float(m.group(1)), float(m.group(2))
+    u = sp.iv(1, 0.2) / sp.iv(0, 0.2)
+    assert abs(p - u) < 10 * err + 2e-3, (p, err, u)

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import mesh_amd
+from conftest import options
 from mesh_amd import gauge
 from mesh_amd import u1 as U1
 from oracle import oracle as O
@@ -255,8 +256,10 @@ def test_flow_mh_rule(ctx):
 def test_flow_errors(ctx):
     with pytest.raises(mesh_amd.MeshError):
         U1.Flow(6, 8, ctx).sample(U1.U1Ensemble(6, 8, 2, ctx), 0, 0)
-    with pytest.raises(mesh_amd.MeshError):
-        U1.U1Ensemble(128, 128, 1, ctx).hmc(1.0, [0.1], 0, 0)
+    with pytest.raises((mesh_amd.MeshError, ValueError)):  # U(1) lattices have 2 to 4 dimensions
+        U1.U1Ensemble(0, 0, 1, ctx, dims=(16,)).hmc(1.0, [0.1], 0, 0)
+    with pytest.raises(mesh_amd.MeshError):  # step tables have odd length
+        U1.U1Ensemble(128, 128, 1, ctx).hmc(1.0, [0.1, 0.2], 0, 0)
 
 
 def _pparams(seed, scale):
@@ -410,3 +413,85 @@ def test_pflow_gain_over_hmc(ctx):
     assert r0["acc_flow"] < 0.01, r0["acc_flow"]
     assert runs["flow"]["acc_flow"] > 0.05, runs["flow"]["acc_flow"]
     assert t_flow < t_hmc / 5, (t_hmc, t_flow)
+
+
+# ---------------------------------------------------------------- any dimension
+
+@pytest.mark.parametrize("dims", [(4, 2, 3, 2), (3, 4, 2), (96, 100)])
+def test_u1_nd_random_bitexact(ctx, dims):
+    """Site-parallel random start (3-d, 4-d, and a 2-d lattice too large for
+    the LDS kernels) against the oracle's Philox streams."""
+    B = 2
+    ens = U1.U1Ensemble(0, 0, B, ctx, dims=dims)
+    ens.random_start(41)
+    th = ens.angles()
+    for c in range(B) if np.prod(dims) < 1000 else range(1):
+        np.testing.assert_array_equal(th[c], U.nd_random_angles(41, c, dims))
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (4, 6, 4)])
+def test_u1_nd_hmc_against_oracle(ctx, dims):
+    """One FP32 trajectory of the site-parallel kernels against the FP64
+    oracle from the same angles and momenta; observables of the result."""
+    B, beta, seed, traj = 2, 1.5, 43, 2
+    ens = U1.U1Ensemble(0, 0, B, ctx, dims=dims)
+    ens.random_start(seed)
+    th0 = ens.angles().astype(np.float64)
+    d = gauge.makeDeltas("2lf", 0.5, 6)
+    ens.hmc(beta, d, seed, traj, metropolis=False)
+    raw = ens._hmc.download().reshape(B, 4)
+    th1 = ens.angles()
+    o = ens.observables()
+    for c in range(B):
+        q = U.nd_momenta(seed, traj, c, dims)
+        t_ref, _, dh_ref = U.nd_hmc_trajectory(th0[c], q, beta, d)
+        assert _wrapdiff(th1[c], t_ref).max() < 2e-4
+        assert abs(raw[c, 0] - dh_ref) < 2e-3 * (1 + abs(dh_ref)), (c, raw[c, 0], dh_ref)
+        assert raw[c, 2] == U.metropolis_uniform(seed, traj, c)
+        S = U.nd_action_over_beta(th1[c].astype(np.float64))
+        assert raw[c, 3] == pytest.approx(S, rel=1e-5)
+        assert o["S"][c] == pytest.approx(S, rel=1e-5)
+        P = U.nd_plaquettes(th1[c].astype(np.float64))
+        assert o["cos"][c] == pytest.approx(np.mean([np.cos(p).mean() for p in P]), abs=1e-6)
+        assert o["Q"][c] == 0.0
+
+
+def test_u1_general_path_matches_lds_path_2d(ctx):
+    """On a 2-d lattice the site-parallel kernels give the LDS kernels' bits:
+    the same fields after three Metropolis trajectories, the same decisions."""
+    L0, L1, B = 8, 12, 6
+    d = gauge.makeDeltas("2mn", 0.5, 4)
+    res = {}
+    for general in (0, 1):
+        with options(ctx, u1_general=general):
+            ens = U1.U1Ensemble(L0, L1, B, ctx)
+            ens.random_start(47)
+            rs = [ens.hmc(2.0, d, 47, t) for t in range(3)]
+            res[general] = (ens.angles(), [r["acc"] for r in rs], [r["dH"] for r in rs])
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    for a, b in zip(res[0][1], res[1][1]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(res[0][2], res[1][2]):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-9)
+
+
+def test_u1_4d_strong_coupling(ctx):
+    """4-d U(1) at beta = 0.2: <cos theta_P> = I1(beta)/I0(beta) + O(u^5)
+    (u^5 = 1e-5 here) from 64 chains of HMC."""
+    import scipy.special as sp
+
+    beta, B = 0.2, 64
+    ens = U1.U1Ensemble(0, 0, B, ctx, dims=(4, 4, 4, 4))
+    ens.random_start(51)
+    d = gauge.makeDeltas("2lf", 0.5, 8)
+    means = []
+    for t in range(160):
+        r = ens.hmc(beta, d, 52, t)
+        if t >= 40:
+            means.append(ens.observables()["cos"])
+    m = np.array(means)  # (steps, chains)
+    chain_means = m.mean(axis=0)
+    est, err = chain_means.mean(), chain_means.std(ddof=1) / math.sqrt(B)
+    u = sp.iv(1, beta) / sp.iv(0, beta)
+    assert r["acc"].mean() > 0.5
+    assert abs(est - u) < 5 * err + 3e-5, (est, u, err)

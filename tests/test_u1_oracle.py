@@ -227,3 +227,45 @@ def test_pflow_loss_gradient_finite_difference():
         pm[i] -= h
         fd = (U.pflow_loss(pp, u, 2.0)[0].mean() - U.pflow_loss(pm, u, 2.0)[0].mean()).item() / (2 * h)
         assert abs(fd - g[i]) < 1e-6 * (1 + abs(g[i])), (i, fd, g[i])
+
+
+def test_u1_nd_oracle_reduces_to_2d():
+    """The any-dimension oracle on a 2-d lattice is the 2-d oracle: the same
+    random angles and momenta (the same Philox streams), plaquettes, force and
+    trajectory."""
+    L0, L1 = 4, 6
+    th = U.nd_random_angles(7, 1, (L0, L1))
+    np.testing.assert_array_equal(th, U.random_angles(7, 1, L0, L1))
+    q = U.nd_momenta(7, 3, 1, (L0, L1))
+    np.testing.assert_array_equal(q, U.momenta(7, 3, 1, L0, L1))
+    t64 = th.astype(np.float64)
+    np.testing.assert_allclose(U.nd_plaquettes(t64)[0], U.plaq(t64), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(U.nd_dS(t64, 1.7), U.dS(t64, 1.7), rtol=0, atol=1e-14)
+    d = O.make_deltas("2lf", 0.5, 5)
+    a, b = U.nd_hmc_trajectory(th, q, 2.0, d), U.hmc_trajectory(th, q, 2.0, d)
+    assert abs(a[2] - b[2]) < 1e-12
+    np.testing.assert_allclose(a[0], b[0], rtol=0, atol=1e-12)
+
+
+def test_u1_nd_force_finite_difference_and_integrator():
+    """4-d: dS/dtheta against central differences of the action; Delta H of the
+    leapfrog scales as the square of the step; the trajectory reverses."""
+    dims = (3, 4, 2, 3)
+    rng = np.random.default_rng(4)
+    th = rng.uniform(-np.pi, np.pi, size=(4,) + dims)
+    beta = 1.3
+    g = U.nd_dS(th, beta)
+    for (mu, x) in [(0, (0, 1, 1, 2)), (2, (2, 3, 0, 1)), (3, (1, 0, 1, 0))]:
+        h = 1e-6
+        tp, tm = th.copy(), th.copy()
+        tp[(mu,) + x] += h
+        tm[(mu,) + x] -= h
+        fd = beta * (U.nd_action_over_beta(tp) - U.nd_action_over_beta(tm)) / (2 * h)
+        assert abs(fd - g[(mu,) + x]) < 1e-6 * max(1.0, abs(fd))
+    q = U.nd_momenta(5, 0, 0, dims).astype(np.float64)
+    dh = [abs(U.nd_hmc_trajectory(th, q, beta, O.make_deltas("2lf", 0.5, n))[2]) for n in (10, 20)]
+    assert 3.0 < dh[0] / dh[1] < 5.0
+    d = O.make_deltas("2lf", 0.5, 10)
+    t1, q1, _ = U.nd_hmc_trajectory(th, q, beta, d)
+    t2, _, _ = U.nd_hmc_trajectory(t1, -q1, beta, d)
+    assert np.abs(U.wrap(t2 - th)).max() < 1e-12
