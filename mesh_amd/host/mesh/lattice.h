@@ -167,7 +167,7 @@ template <class T> struct SU3
 
 // U(1): the reference's complex<T> link exp(i theta) is stored as its angle,
 // in FP32 whatever T is (energies, sums and the Metropolis test are FP64);
-// 2-d lattices only
+// lattices of 2 to 4 dimensions
 template <class T> struct U1
 {
 	using real = T;
