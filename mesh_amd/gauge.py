@@ -153,6 +153,9 @@ class Hmc:
         self.seed = int(seed)
         self.U = GaugeField(grid, self.prec, self.ctx)
         self.U_new = GaugeField(grid, self.prec, self.ctx)
+        # workspace of the trajectory: after runHmcUpdate P holds the final
+        # momenta in the compact antihermitian layout, after randomizeMomenta
+        # the matrix layout
         self.P = GaugeField(grid, self.prec, self.ctx)
         self.K = DeviceArray(grid.size(), self.ctx)
         self.traj = 0

@@ -163,6 +163,9 @@ template <class G> class Hmc
 {
   public:
 	Grid const &g;
+	// P is the trajectory's workspace: after runHmcUpdate it holds the final
+	// momenta in the compact antihermitian layout (mesh_su3_hmc_trajectory),
+	// after randomizeMomenta the matrix layout
 	GaugeField<G> U, P, U_new;
 	uint64_t seed = 0; // keys the Philox streams (the reference seeds xoshiro256)
 	uint32_t traj = 0;
