@@ -387,7 +387,10 @@ __global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, const float *_
 				float target = wrapf(tpn - p.t);
 				// g is increasing on [-pi, pi] with g(+-pi) = +-pi: Newton steps
 				// inside a bracket that every evaluation narrows, bisection when a
-				// step leaves it (at most 40 evaluations; typically 3-5)
+				// step leaves it, until a step or the bracket is below
+				// 2e-7 max(1, |u|) (an FP32 ulp of u is up to 2.4e-7 here, so an
+				// absolute bound could not stop an iterate that alternates
+				// between neighbouring floats; at most 40 evaluations)
 				float lo = -PI_F, hi = PI_F, u = target, G, Gp;
 				for (int b = 0; b < 40; ++b)
 				{
@@ -399,10 +402,12 @@ __global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, const float *_
 						lo = u;
 					else
 						hi = u;
+					if (hi - lo <= 2e-7f * fmaxf(1.f, fabsf(u)))
+						break;
 					float un = u - f / Gp;
 					if (un > lo && un < hi)
 					{
-						bool done = fabsf(un - u) <= 2e-7f;
+						bool done = fabsf(un - u) <= 2e-7f * fmaxf(1.f, fabsf(un));
 						u = un;
 						if (done)
 							break;

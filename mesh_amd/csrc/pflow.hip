@@ -28,6 +28,11 @@
 #include "mesh_internal.h"
 #include "rng_dev.h"
 
+// log q: table-guided inverse of the circle map (0: Newton from the target)
+#ifndef MESH_PFLOW_TABLE
+#define MESH_PFLOW_TABLE 1
+#endif
+
 using namespace mesh_dev;
 
 namespace {
@@ -110,11 +115,18 @@ __device__ __forceinline__ void g_eval(Map const &p, float v, float &G, float &G
 	}
 }
 
-// v = g^-1(target) on [-pi, pi]: Newton inside a shrinking bracket, bisection
-// when a step leaves it (at most 40 evaluations); returns g'(v) in Gp
-__device__ __forceinline__ float g_inverse(Map const &p, float target, float &Gp)
+// v = g^-1(target) on [lo, hi] (a bracket of the root) from the start u:
+// Newton inside a shrinking bracket, bisection when a step leaves it, until a
+// step or the bracket is below 2e-7 max(1, |v|) -- an FP32 ulp of v is up to
+// 2.4e-7 on [-pi, pi], so an absolute bound would let an iterate that
+// alternates between neighbouring floats run to the cap (at most 40
+// evaluations); returns g'(v) in Gp
+__device__ __forceinline__ float g_inverse(Map const &p, float target, float &Gp, float lo = -PI_F, float hi = PI_F,
+                                           float u = 2.f * PI_F)
 {
-	float lo = -PI_F, hi = PI_F, u = target, G;
+	if (u > PI_F)
+		u = target;
+	float G;
 	for (int b = 0; b < 40; ++b)
 	{
 		g_eval(p, u, G, Gp);
@@ -125,10 +137,12 @@ __device__ __forceinline__ float g_inverse(Map const &p, float target, float &Gp
 			lo = u;
 		else
 			hi = u;
+		if (hi - lo <= 2e-7f * fmaxf(1.f, fabsf(u)))
+			break;
 		float un = u - f / Gp;
 		if (un > lo && un < hi)
 		{
-			bool done = fabsf(un - u) <= 2e-7f;
+			bool done = fabsf(un - u) <= 2e-7f * fmaxf(1.f, fabsf(un));
 			u = un;
 			if (done)
 				break;
@@ -138,6 +152,41 @@ __device__ __forceinline__ float g_inverse(Map const &p, float target, float &Gp
 	}
 	g_eval(p, u, G, Gp);
 	return u;
+}
+
+// g on the grid v_j = -pi + j 2 pi / NTAB, j = 0..NTAB, one value per thread
+// of the workgroup: G is monotone, so a table lookup brackets the root of
+// g(v) = target within three cells and interpolation starts Newton within
+// about (2 pi / NTAB)^2 of it
+constexpr int NTAB = 256;
+__device__ __forceinline__ void g_table(Map const &p, float *tab)
+{
+	for (int j = threadIdx.x; j <= NTAB; j += THREADS)
+	{
+		float G, Gp;
+		g_eval(p, fmaf((float)j, TWO_PI_F / NTAB, -PI_F), G, Gp);
+		tab[j] = G;
+	}
+}
+__device__ __forceinline__ float g_inverse_tab(Map const &p, const float *tab, float target, float &Gp)
+{
+	int a = 0, b = NTAB;
+	while (b - a > 1)
+	{
+		int m = (a + b) >> 1;
+		if (tab[m] <= target)
+			a = m;
+		else
+			b = m;
+	}
+	const float h = TWO_PI_F / NTAB;
+	const float va = fmaf((float)a, h, -PI_F);
+	const float dG = tab[a + 1] - tab[a];
+	float u = dG > 0.f ? fmaf((target - tab[a]) / dG, h, va) : va + 0.5f * h;
+	// one cell of margin either side: the table holds g rounded to FP32
+	const float lo = a > 0 ? va - h : -PI_F, hi = a + 2 <= NTAB ? va + 2.f * h : PI_F;
+	u = fminf(fmaxf(u, lo), hi);
+	return g_inverse(p, target, Gp, lo, hi, u);
 }
 
 // uniform on [-pi, pi) for counter word (a, b) of configuration c
@@ -235,10 +284,13 @@ __global__ __launch_bounds__(THREADS) void k_pflow_logq(PGeo g, const float *__r
 {
 	extern __shared__ float t[]; // 2V
 	__shared__ double red[2][4];
+	__shared__ float tab[NTAB + 1];
 	const int c = blockIdx.x;
 	const Map p = map_params(params);
 	for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
 		t[i] = theta[(long)c * 2 * g.V + i];
+	if (MESH_PFLOW_TABLE)
+		g_table(p, tab);
 	__syncthreads();
 	double v[2] = {0.0, 0.0};
 	for (int x = threadIdx.x; x < g.V; x += THREADS)
@@ -249,7 +301,10 @@ __global__ __launch_bounds__(THREADS) void k_pflow_logq(PGeo g, const float *__r
 		if (x < g.V - 1)
 		{
 			float Gp;
-			g_inverse(p, wrapf(tp - p.t), Gp);
+			if (MESH_PFLOW_TABLE)
+				g_inverse_tab(p, tab, wrapf(tp - p.t), Gp);
+			else
+				g_inverse(p, wrapf(tp - p.t), Gp);
 			v[0] -= (double)logf(Gp);
 		}
 	}
