@@ -36,6 +36,16 @@ using namespace mesh_dev;
 #ifndef MESH_FORCE_WAVES_ROWS
 #define MESH_FORCE_WAVES_ROWS 0
 #endif
+// upper bound of waves per SIMD for the FP64 kernel with the row-staged staple
+// path (8: none): it runs at two either way (170 VGPRs), and with the bound the
+// compiler may spend the registers up to 256 on loads in flight
+#ifndef MESH_FORCE_MAXW_ROWS
+#define MESH_FORCE_MAXW_ROWS 2
+#endif
+template <class R, int PATH = 0> constexpr int force_max_waves()
+{
+	return (std::is_same<R, double>::value && PATH == 2) ? MESH_FORCE_MAXW_ROWS : 8;
+}
 template <class R, int PATH = 0> constexpr int force_min_waves()
 {
 	return (std::is_same<R, float>::value && MESH_FORCE_WAVES_F32 > 0) ? MESH_FORCE_WAVES_F32
@@ -481,7 +491,8 @@ inline int staple_path(Geo const &g)
 }
 
 template <class R, bool FUSE, bool PC, int PATH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>()))) void k_force_update(
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R, PATH>()))) void k_force_update(
     Geo g, const R *__restrict__ U, R *__restrict__ P, R kc, int nb, int remap, R *__restrict__ Unext, R tl,
     int reunit_flag)
 {
