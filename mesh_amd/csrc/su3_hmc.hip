@@ -60,6 +60,10 @@ template <class R, int PATH = 0> constexpr int force_min_waves()
 #endif
 // waves per SIMD the Delta H kernel is limited to (two fields with neighbours
 // in flight: more waves thrash the L2; 0 = the compiler's choice)
+// Delta H with the two fields' plaquettes on separate halves of a workgroup
+#ifndef MESH_DH_SPLIT
+#define MESH_DH_SPLIT 1
+#endif
 #ifndef MESH_DH_WAVES
 #define MESH_DH_WAVES 2
 #endif
@@ -414,6 +418,57 @@ __global__ __launch_bounds__(256) DH_OCC void k_delta_h(Geo g, const R *__restri
 		v[2] = psn;
 		v[3] = kn;
 		v[4] = Ko[x];
+	}
+	block_sum_store<5>(v, partials, nb, lb);
+}
+
+// The same with a workgroup of 256 threads over 128 sites: threads 0-127 sum
+// the plaquettes of U_old, threads 128-255 those of U_new and the kinetic
+// energy, for the same sites; the per-site values are those of k_delta_h (the
+// block partials sum them in another order).
+template <class R, bool PC>
+__global__ __launch_bounds__(256) void k_delta_h_split(Geo g, const R *__restrict__ Uo, const R *__restrict__ Un,
+                                                       const R *__restrict__ P, const double *__restrict__ Ko,
+                                                       double c3, double *partials, int nb, int remap)
+{
+	__shared__ double pso_s[128];
+	int lb = ordered_block(g, nb, 128, remap);
+	if (lb < 0)
+		return;
+	const bool newer = threadIdx.x >= 128;
+	const int t = threadIdx.x & 127;
+	long x = lane_site(g, (long)lb * 128 + t);
+	double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+	double ps = 0.0, kn = 0.0;
+	if (x < g.V)
+	{
+		int c[4];
+		coords<R>(g, x, c);
+		if (newer)
+		{
+			ps = plaq_site(g, Un, x, c);
+			kn = kin_site<PC>(g, P, x);
+		}
+		else
+			ps = plaq_site(g, Uo, x, c);
+	}
+	if (!newer)
+		pso_s[t] = ps;
+	__syncthreads();
+	if (x < g.V)
+	{
+		if (newer)
+		{
+			double pso = pso_s[t];
+			double dps = ps - pso;
+			double dk = kn - Ko[x];
+			v[0] = fma(-c3, dps, dk);
+			v[2] = ps;
+			v[3] = kn;
+			v[4] = Ko[x];
+		}
+		else
+			v[1] = ps;
 	}
 	block_sum_store<5>(v, partials, nb, lb);
 }
@@ -997,7 +1052,8 @@ int launch_momenta_link(mesh_ctx *ctx, Geo const &geo, int prec, void *P, double
 int delta_h(mesh_ctx *ctx, Geo const &geo, int prec, const void *Uo, const void *Un, const void *P, const double *Ko,
             double beta, double *out, bool compact = false)
 {
-	int nb = nblk(geo.V, 256);
+	const int sbk = MESH_DH_SPLIT ? 128 : 256; // sites per workgroup
+	int nb = nblk(geo.V, sbk);
 	if (int rc = mesh_reserve_partials(ctx, 5 * (size_t)nb))
 		return rc;
 	int gb = mesh_grid_blocks(ctx, nb);
@@ -1005,8 +1061,12 @@ int delta_h(mesh_ctx *ctx, Geo const &geo, int prec, const void *Uo, const void 
 	int ph = mesh_prof_begin(ctx, PROF_DELTA_H);
 	auto launch = [&](auto tag, auto pc) {
 		using Rt = decltype(tag);
-		k_delta_h<Rt, decltype(pc)::value><<<gb, 256, 0, ctx->stream>>>(
-		    geo, (const Rt *)Uo, (const Rt *)Un, (const Rt *)P, Ko, c3, ctx->partials, nb, mesh_order(ctx));
+		if (MESH_DH_SPLIT)
+			k_delta_h_split<Rt, decltype(pc)::value><<<gb, 256, 0, ctx->stream>>>(
+			    geo, (const Rt *)Uo, (const Rt *)Un, (const Rt *)P, Ko, c3, ctx->partials, nb, mesh_order(ctx));
+		else
+			k_delta_h<Rt, decltype(pc)::value><<<gb, 256, 0, ctx->stream>>>(
+			    geo, (const Rt *)Uo, (const Rt *)Un, (const Rt *)P, Ko, c3, ctx->partials, nb, mesh_order(ctx));
 	};
 	using PC1 = std::true_type;
 	using PC0 = std::false_type;
