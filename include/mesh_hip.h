@@ -222,6 +222,15 @@ int mesh_su3_exp_mul(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *
  * add_mul / `lhs += rhs * a` on lattices (src/lattice/lattice.h:302-309,
  * 349-363), e.g. the conjugate-gradient update of landau.h:128. */
 int mesh_su3_axpy(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *Y, double a, const void *X);
+/* dst[x] = src[x + offset e_dir], periodic, nf matrices per site; dst != src,
+ * 0 <= dir < nd, 0 < |offset| < L[dir]: cshift (src/lattice/lattice.h:366-389,
+ * device_cshift src/lattice/device.h:55-110). */
+int mesh_su3_cshift(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *dst, const void *src, int dir,
+                    int offset);
+/* out[18] = sum over sites (and the nf matrices) of every complex entry,
+ * (re, im) row-major, summed in FP64: lattice_sum (src/lattice/lattice.h:270-273,
+ * device_sum src/lattice/device.h:238-265, which sums in the field's own type). */
+int mesh_su3_sum(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *src, double *out);
 
 /* out[5] = {Delta H, sum_x ps_old(x), sum_x ps_new(x), K_old, K_new} with
  * per-site differencing d(x) = -(beta/3)(ps_new - ps_old)(x) + (k_new - k_old)(x),

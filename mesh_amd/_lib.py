@@ -71,6 +71,8 @@ _SIGS = {
     "mesh_su3_reunit": (i32, [vp, gp, i32, i32, vp]),
     "mesh_su3_exp_mul": (i32, [vp, gp, i32, i32, vp, vp, f64]),
     "mesh_su3_axpy": (i32, [vp, gp, i32, i32, vp, f64, vp]),
+    "mesh_su3_cshift": (i32, [vp, gp, i32, i32, vp, vp, i32, i32]),
+    "mesh_su3_sum": (i32, [vp, gp, i32, i32, vp, dp]),
     "mesh_su3_delta_h": (i32, [vp, gp, i32, vp, vp, vp, vp, f64, dp]),
     "mesh_make_deltas": (i32, [C.c_char_p, f64, i32, dp, i32]),
     "mesh_su3_hmc_trajectory": (i32, [vp, gp, i32, C.POINTER(vp), C.POINTER(vp), vp, vp, f64, dp, i32, u64,

@@ -56,6 +56,22 @@ template <class G> void testImpl(Coordinate geom, uint64_t seed)
 		check(std::fabs(r3 - 1) < (G::prec == MESH_F64 ? 1e-3 : 5e-2), "derivative of Landau gauge condition", r3);
 	}
 
+	// cshift (lattice.h:366-389) and lattice_sum (lattice.h:270-273): shifting
+	// forth and back is the identity, and a shifted field sums to the same
+	{
+		auto a = cshift(f, g.ndim() - 1, 1);
+		auto b = cshift(a, g.ndim() - 1, -1);
+		auto s0 = lattice_sum(f), s1 = lattice_sum(a), s2 = lattice_sum(b);
+		double d01 = 0, d02 = 0;
+		for (int k = 0; k < 9; ++k)
+		{
+			d01 = std::max(d01, std::abs(s1[k] - s0[k]));
+			d02 = std::max(d02, std::abs(s2[k] - s0[k]));
+		}
+		check(d01 < 1e-6 * g.size(), "lattice_sum invariant under cshift", d01);
+		check(d02 == 0.0, "cshift by +1 then -1 gives the field back", d02);
+	}
+
 	GaugeField<G> one(g);
 	one.fill_unit();
 	check(std::fabs(plaquette(one) - 1) < 1e-12, "plaquette of the unit field", plaquette(one));

@@ -66,7 +66,7 @@ void mesh_prof_end(mesh_ctx *ctx, int handle);
 void mesh_prof_collect(mesh_ctx *ctx);
 
 constexpr int MESH_CTX_MAGIC = 0x4d455348;
-constexpr int RED_MAX_Q = 8;
+constexpr int RED_MAX_Q = 18;
 
 int mesh_fail(mesh_ctx *ctx, int code, std::string const &msg);
 int mesh_hip_fail(mesh_ctx *ctx, hipError_t e, char const *what);

@@ -187,6 +187,7 @@ int mesh_finish_reduction(mesh_ctx *ctx, int nb, int nq, double *out)
 	case 6: k_reduce_partials<6><<<1, 1024, 0, st>>>(p, nb, o); break;
 	case 7: k_reduce_partials<7><<<1, 1024, 0, st>>>(p, nb, o); break;
 	case 8: k_reduce_partials<8><<<1, 1024, 0, st>>>(p, nb, o); break;
+	case 18: k_reduce_partials<18><<<1, 1024, 0, st>>>(p, nb, o); break;
 	default:
 		return mesh_fail(ctx, MESH_E_ARG, "unsupported reduction width");
 	}

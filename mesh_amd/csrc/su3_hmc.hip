@@ -798,6 +798,46 @@ __global__ __launch_bounds__(256) void k_link_update(Geo g, int nf, R *Uout, con
 	store(Uout, f, g, x, u);
 }
 
+// dst[x] = src[x + offset e_dir] (periodic) for nf matrices per site
+template <class R>
+__global__ __launch_bounds__(256) void k_cshift(Geo g, int nf, R *__restrict__ dst, const R *__restrict__ src, int dir,
+                                                int offset)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= g.V * nf)
+		return;
+	int f = (int)(i / g.V);
+	long x = lane_site(g, i - (long)f * g.V);
+	const int c = coord(g, x, dir), L = g.L[dir];
+	int cn = c + offset;
+	cn = cn >= L ? cn - L : (cn < 0 ? cn + L : cn);
+	store(dst, f, g, x, load(src, f, g, x + (long)(cn - c) * g.st[dir]));
+}
+
+// block partial sums of the 18 reals of every matrix (FP64)
+template <class R>
+__global__ __launch_bounds__(256) void k_sum_m3(Geo g, int nf, const R *__restrict__ src, double *partials, int nb)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	double v[18];
+#pragma unroll
+	for (int k = 0; k < 18; ++k)
+		v[k] = 0.0;
+	if (i < g.V * nf)
+	{
+		int f = (int)(i / g.V);
+		long x = lane_site(g, i - (long)f * g.V);
+		m3<R> m = load(src, f, g, x);
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+		{
+			v[2 * k] = (double)m.e[k].re;
+			v[2 * k + 1] = (double)m.e[k].im;
+		}
+	}
+	block_sum_store<18>(v, partials, nb, blockIdx.x);
+}
+
 // Y = Y + (a X) per real, the multiply and the add rounded separately as in
 // the reference's lattice expression `Y += X * a`
 template <class R>
@@ -1261,6 +1301,47 @@ int mesh_su3_exp_mul(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *
 	if (!U || !F || nf < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad exp_mul arguments");
 	return launch_link_update(ctx, s.geo, nf, prec, U, U, F, t, 0);
+}
+
+int mesh_su3_cshift(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *dst, const void *src, int dir,
+                    int offset)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!dst || !src || nf < 1)
+		return mesh_fail(ctx, MESH_E_ARG, "bad cshift arguments");
+	if (dst == src)
+		return mesh_fail(ctx, MESH_E_ARG, "inplace cshift not supported");
+	if (dir < 0 || dir >= s.geo.nd || offset == 0 || offset <= -s.geo.L[dir] || offset >= s.geo.L[dir])
+		return mesh_fail(ctx, MESH_E_ARG, "cshift needs 0 <= dir < nd and 0 < |offset| < L[dir]");
+	int nb = nblk(s.geo.V * nf, 256);
+	if (prec == MESH_F64)
+		k_cshift<double><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (double *)dst, (const double *)src, dir, offset);
+	else
+		k_cshift<float><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (float *)dst, (const float *)src, dir, offset);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_su3_sum(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *src, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	if (!src || !out || nf < 1)
+		return mesh_fail(ctx, MESH_E_ARG, "bad sum arguments");
+	int nb = nblk(s.geo.V * nf, 256);
+	if (int rc = mesh_reserve_partials(ctx, 18 * (size_t)nb))
+		return rc;
+	if (prec == MESH_F64)
+		k_sum_m3<double><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (const double *)src, ctx->partials, nb);
+	else
+		k_sum_m3<float><<<nb, 256, 0, ctx->stream>>>(s.geo, nf, (const float *)src, ctx->partials, nb);
+	MESH_LAUNCHED(ctx);
+	return mesh_finish_reduction(ctx, nb, 18, out);
 }
 
 int mesh_su3_axpy(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *Y, double a, const void *X)

@@ -86,6 +86,22 @@ def evolve(U, P, t):
     return U
 
 
+def cshift(a, dir, offset):
+    """New field b[x] = a[x + offset e_dir], periodic (src/lattice/lattice.h:366-389)."""
+    out = Field(a.grid, a.nf, a.prec, a.ctx)
+    a.ctx.call("mesh_su3_cshift", a.grid.ref, a.nf, a.prec, out.ptr, a.ptr, int(dir), int(offset))
+    return out
+
+
+def lattice_sum(a):
+    """Sum of every matrix of the field as a complex 3x3 array, summed in FP64
+    (lattice_sum, src/lattice/lattice.h:270-273)."""
+    o = _out(18)
+    a.ctx.call("mesh_su3_sum", a.grid.ref, a.nf, a.prec, a.ptr, o)
+    v = np.array([o[k] for k in range(18)])
+    return (v[0::2] + 1j * v[1::2]).reshape(3, 3)
+
+
 def add_mul(lhs, a, rhs):
     """lhs += a * rhs, the product rounded before the sum (add_mul and
     `lhs += rhs * a`, src/lattice/lattice.h:302-309, 349-363)."""

@@ -67,6 +67,25 @@ template <class G> Lattice<G> wilsonDeriv(GaugeField<G> const &U, int mu, double
 	return F;
 }
 
+// circular shift: b[x] = a[x + offset e_dir] (src/lattice/lattice.h:366-389)
+template <class G> Lattice<G> cshift(Lattice<G> const &a, int dir, int offset)
+{
+	Lattice<G> r(a.grid());
+	ctx().check(mesh_su3_cshift(ctx().get(), a.grid().c(), 1, G::prec, r.data(), a.data(), dir, offset));
+	return r;
+}
+
+// sum of every matrix of the field, in FP64 (lattice_sum, src/lattice/lattice.h:270-273)
+template <class G> std::array<std::complex<double>, 9> lattice_sum(Field<G> const &a)
+{
+	double o[18];
+	ctx().check(mesh_su3_sum(ctx().get(), a.grid().c(), a.nfields(), G::prec, a.data(), o));
+	std::array<std::complex<double>, 9> r;
+	for (int k = 0; k < 9; ++k)
+		r[k] = {o[2 * k], o[2 * k + 1]};
+	return r;
+}
+
 // lhs += a * rhs, the product rounded before the sum (add_mul,
 // src/lattice/lattice.h:358-363; `lhs += rhs * a`, lattice.h:302-309, 349-355)
 template <class G> void add_mul(Field<G> &lhs, double a, Field<G> const &rhs)

@@ -360,3 +360,32 @@ def test_c2_force_parity_32(ctx, prec):
     gauge.momentumStep(U, P, 6.0, 1.0 / 12)
     ref = O.force_update(Uh, Ph, shape, 6.0, 1.0 / 12)
     np.testing.assert_array_equal(P.download(), ref)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", [(4, 3, 6, 2), (2, 5, 8)])
+def test_cshift_and_sum(ctx, shape, prec):
+    """cshift (src/lattice/lattice.h:366-389) against numpy roll, bit for bit,
+    for every direction and offsets of both signs; lattice_sum against numpy."""
+    sh = list(shape)
+    U = O.random_gauge(sh, 61, _pname(prec))
+    g, f = _field(ctx, sh, prec, U)
+    V = O.volume(sh)
+    host = U.reshape((len(sh),) + tuple(sh) + (3, 3))
+    for d in range(len(sh)):
+        for off in (1, -1, sh[d] - 1):
+            if off == 0 or abs(off) >= sh[d]:
+                continue
+            out = gauge.cshift(f, d, off).download()
+            ref = np.roll(host, -off, axis=1 + d).reshape(len(sh), V, 3, 3)
+            np.testing.assert_array_equal(out, ref)
+    s = gauge.lattice_sum(f)
+    ref = U.astype(np.complex128).sum(axis=(0, 1))
+    # FP64 sums of O(1) entries: a few ulp per addition at most
+    assert np.abs(s - ref).max() < 1e-12 * V * len(sh)
+    with pytest.raises(mesh_amd.MeshError):
+        gauge.cshift(f, len(sh), 1)
+    with pytest.raises(mesh_amd.MeshError):
+        gauge.cshift(f, 0, 0)
+    with pytest.raises(mesh_amd.MeshError):
+        gauge.cshift(f, 0, sh[0])
