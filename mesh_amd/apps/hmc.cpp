@@ -3,13 +3,20 @@
 // same summary; the trajectory runs in libmesh_hip.so.
 //
 // Options (reference defaults): --group su3 --geom 8 8 8 8 --beta 6
-//   --start random|cold --seed S --count 100 --scheme 4mn|2mn|2lf
-//   --epsilon 1 --substeps 8 --precision 2 (1 = FP32-mixed, 2 = FP64)
+//   --start random|cold|<ensemble>/configs/<n> --seed S --count 100
+//   --scheme 4mn|2mn|2lf --epsilon 1 --substeps 8 --precision 2 (1 =
+//   FP32-mixed, 2 = FP64) --filename <ensemble> [--force]: every
+//   configuration and the histories go to an ensemble directory (the
+//   reference's HDF5 file, src/apps/hmc.cpp:63-86, 119-124; format in
+//   mesh/io.h).  A start configuration from an ensemble sets the geometry
+//   when --geom is not given and, without --seed, continues that chain's
+//   Philox streams, so the run reproduces the uninterrupted chain bit for bit.
 // Additions: --therm K (first K trajectories without Metropolis),
 //   --landau (Landau gauge fixing of the final configuration by
 //   overrelaxation), --landau-ascent (the same by the reference's line-search
 //   ascent, src/gauge/landau.h:100-136).
 #include "mesh/gauge.h"
+#include "mesh/io.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -25,9 +32,10 @@ namespace {
 [[noreturn]] void usage(char const *msg)
 {
 	std::fprintf(stderr,
-	             "error: %s\nusage: mesh_hmc [--group su3|u1] [--geom L0 L1 [L2 [L3]]] [--beta B] [--start random|cold]\n"
-	             "                [--seed S] [--count N] [--scheme 2lf|2mn|4mn] [--epsilon E] [--substeps N]\n"
-	             "                [--precision 1|2] [--therm K] [--landau] [--landau-ascent]\n",
+	             "error: %s\nusage: mesh_hmc [--group su3|u1] [--geom L0 L1 [L2 [L3]]] [--beta B]\n"
+	             "                [--start random|cold|<ensemble>/configs/<n>] [--seed S] [--count N]\n"
+	             "                [--scheme 2lf|2mn|4mn] [--epsilon E] [--substeps N] [--precision 1|2]\n"
+	             "                [--filename <ensemble> [--force]] [--therm K] [--landau] [--landau-ascent]\n",
 	             msg);
 	std::exit(2);
 }
@@ -58,7 +66,8 @@ int main(int argc, char **argv)
 	std::vector<int32_t> geom = {8, 8, 8, 8};
 	double beta = 6.0, epsilon = 1.0;
 	int substeps = 8, count = 100, precision = 2, therm = 0;
-	bool landau = false, ascent = false;
+	bool landau = false, ascent = false, geom_given = false, allow_overwrite = false;
+	std::string filename;
 
 	for (int i = 1; i < argc; ++i)
 	{
@@ -73,6 +82,7 @@ int main(int argc, char **argv)
 		else if (a == "--geom")
 		{
 			geom.clear();
+			geom_given = true;
 			while (i + 1 < argc && is_number(argv[i + 1]))
 				geom.push_back(std::atoi(argv[++i]));
 		}
@@ -100,19 +110,25 @@ int main(int argc, char **argv)
 			landau = ascent = true;
 		else if (a == "--plot")
 			std::fprintf(stderr, "mesh_hmc: --plot (gnuplot of the plaquette history) is not supported; ignored\n");
-		else if (a == "--filename" || a == "--force")
-			usage((a + ": HDF5 output of configurations is not supported by this build").c_str());
+		else if (a == "--filename")
+			filename = next();
+		else if (a == "--force")
+			allow_overwrite = true;
 		else if (a == "--help" || a == "-h")
 			usage("help requested");
 		else
 			usage(("unknown option " + a).c_str());
 	}
+	const bool seed_given = !seed.empty();
 	if (seed.empty())
 		seed = std::to_string(std::random_device{}());
+	const bool from_file = start != "random" && start != "cold";
 
 	try
 	{
 		auto deltas = makeDeltas(scheme, epsilon, substeps);
+		if (from_file && !geom_given)
+			geom = configGeometry(start);
 		auto t_total = std::chrono::steady_clock::now();
 		mesh_profile_enable(ctx().get(), 1);
 		dispatchByGroup(
@@ -125,14 +141,53 @@ int main(int argc, char **argv)
 			    else if (start == "cold")
 				    hmc.U.fill_unit();
 			    else
-				    throw std::runtime_error("unknown start '" + start + "' (random or cold)");
+			    {
+				    readConfig(start, hmc.U);
+				    auto [edir, ename] = io::split_config_name(start);
+				    auto at = io::Ensemble::open(edir).attributes();
+				    if (!seed_given && at.has("philox_key") && is_number(ename))
+				    {
+					    hmc.seed = std::strtoull(at.get_string("philox_key").c_str(), nullptr, 10);
+					    hmc.traj = (uint32_t)(at.get_int("philox_traj0") + std::stoll(ename));
+					    std::printf("continuing the chain of %s at trajectory counter %u\n", edir.c_str(), hmc.traj);
+				    }
+			    }
+			    io::Ensemble ens;
+			    if (!filename.empty())
+			    {
+				    ens = io::Ensemble::create(filename, allow_overwrite);
+				    ens.set_attribute("group", group);
+				    ens.set_attribute("beta", beta);
+				    ens.set_attribute("geometry", std::vector<int>(geom.begin(), geom.end()));
+				    ens.set_attribute("hmc_scheme", scheme);
+				    ens.set_attribute("hmc_epsilon", epsilon);
+				    ens.set_attribute("hmc_substeps", substeps);
+				    ens.set_attribute("hmc_metropolis", 1);
+				    ens.set_attribute("markov_seed", seed);
+				    ens.set_attribute("markov_start", start);
+				    ens.set_attribute("markov_discard", therm);
+				    ens.set_attribute("markov_count", count);
+				    ens.set_attribute("markov_spacing", 1);
+				    ens.set_attribute("precision", precision);
+				    ens.set_attribute("philox_key", std::to_string(hmc.seed));
+				    ens.set_attribute("philox_traj0", (long long)hmc.traj);
+			    }
 			    for (int iter = 0; iter < count; ++iter)
 			    {
 				    hmc.runHmcUpdate(beta, deltas, iter >= therm);
 				    if (iter == therm - 1)
 					    hmc.reset_observables();
+				    if (ens)
+					    writeConfig(ens, std::to_string(iter + 1), hmc.U);
 			    }
 			    hmc.print_summary();
+			    if (ens)
+			    {
+				    ens.write_data("plaq_history", hmc.plaq_history);
+				    ens.write_data("accept_history", hmc.accept_history);
+				    ens.write_data("deltaH_history", hmc.deltaH_history);
+				    ens.write_data("time_history", hmc.time_history);
+			    }
 			    if constexpr (G::Nc() == 1)
 			    {
 				    if (landau)

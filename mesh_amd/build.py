@@ -53,7 +53,7 @@ def build(verbose=False, extra=(), out=LIB, tag="default"):
     return out
 
 
-APPS = {"mesh_hmc": "hmc.cpp", "test_lattice_gauge": "test_lattice_gauge.cpp"}
+APPS = {"mesh_hmc": "hmc.cpp", "test_lattice_gauge": "test_lattice_gauge.cpp", "test_io": "test_io.cpp"}
 
 
 def build_apps(verbose=False):
