@@ -12,6 +12,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <utility>
 
 namespace mesh_dev {
 
@@ -247,10 +248,41 @@ template <class R> MD cpx<R> det(m3<R> const &X)
 }
 
 template <class R> constexpr int exp_terms() { return sizeof(R) == 8 ? 18 : 10; }
+// for ||X||_F^2 <= EXP_SMALL_R2 the first exp_terms_small() terms suffice:
+// the remainder is below 0.2^7/7! = 2.5e-9 (FP32) or 0.2^12/12! = 8.6e-18
+// (FP64), under half an ulp of 1 as with the full count at ||X||_F = 1
+constexpr double EXP_SMALL_R2 = 0.04;
+template <class R> constexpr int exp_terms_small() { return sizeof(R) == 8 ? 11 : 6; }
+
+template <class R, int N>
+MD void exp_series_term(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2)
+{
+	const R inv = R(1) / R(N + 1);
+	cpx<R> gb = cmul(ga, b);
+	cpx<R> nal = {gb.re * inv, gb.im * inv};
+	cpx<R> nbe = {fma(ga.re, a, al.re) * inv, fma(ga.im, a, al.im) * inv};
+	cpx<R> nga = {be.re * inv, be.im * inv};
+	al = nal;
+	be = nbe;
+	ga = nga;
+	f0.re += al.re;
+	f0.im += al.im;
+	f1.re += be.re;
+	f1.im += be.im;
+	f2.re += ga.re;
+	f2.im += ga.im;
+}
+template <class R, int N0, int... N>
+MD void exp_series(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2,
+                   std::integer_sequence<int, N...>)
+{
+	(exp_series_term<R, N0 + N>(a, b, al, be, ga, f0, f1, f2), ...);
+}
 
 // exp of a traceless antihermitian matrix: Cayley-Hamilton with
 // power-series coefficients (X^3 = a X + b with a = tr X^2 / 2, b = det X),
-// argument halved until ||X||_F <= 1 and the result squared back.
+// argument halved until ||X||_F <= 1 and the result squared back; the series
+// stops after exp_terms_small() terms where ||X||_F^2 <= EXP_SMALL_R2.
 template <class R> MD m3<R> expsu3(m3<R> X)
 {
 	m3<R> X2 = mul_nn(X, X);
@@ -282,24 +314,10 @@ template <class R> MD m3<R> expsu3(m3<R> X)
 	cpx<R> b = det(X);
 	cpx<R> al = {R(1), R(0)}, be = c0<R>(), ga = c0<R>();
 	cpx<R> f0 = {R(1), R(0)}, f1 = c0<R>(), f2 = c0<R>();
-#pragma unroll
-	for (int n = 0; n < exp_terms<R>(); ++n)
-	{
-		const R inv = R(1) / R(n + 1);
-		cpx<R> gb = cmul(ga, b);
-		cpx<R> nal = {gb.re * inv, gb.im * inv};
-		cpx<R> nbe = {fma(ga.re, a, al.re) * inv, fma(ga.im, a, al.im) * inv};
-		cpx<R> nga = {be.re * inv, be.im * inv};
-		al = nal;
-		be = nbe;
-		ga = nga;
-		f0.re += al.re;
-		f0.im += al.im;
-		f1.re += be.re;
-		f1.im += be.im;
-		f2.re += ga.re;
-		f2.im += ga.im;
-	}
+	constexpr int NS = exp_terms_small<R>(), NF = exp_terms<R>();
+	exp_series<R, 0>(a, b, al, be, ga, f0, f1, f2, std::make_integer_sequence<int, NS>{});
+	if (r2 > R(EXP_SMALL_R2))
+		exp_series<R, NS>(a, b, al, be, ga, f0, f1, f2, std::make_integer_sequence<int, NF - NS>{});
 	m3<R> E;
 #pragma unroll
 	for (int k = 0; k < 9; ++k)

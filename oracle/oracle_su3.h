@@ -253,8 +253,12 @@ template <class T> inline cplx<T> det3(mat3<T> const &X)
 	return d;
 }
 
-// number of power-series terms in expsu3 (remainder < 1/(N+1)! for ||X||_F <= 1)
+// number of power-series terms in expsu3 (remainder < 1/(N+1)! for ||X||_F <= 1);
+// for ||X||_F^2 <= exp_small_r2 the first exp_terms_small() terms (remainder
+// below 0.2^7/7! = 2.5e-9 FP32, 0.2^12/12! = 8.6e-18 FP64)
 template <class T> constexpr int exp_terms() { return sizeof(T) == 8 ? 18 : 10; }
+template <class T> constexpr int exp_terms_small() { return sizeof(T) == 8 ? 11 : 6; }
+constexpr double exp_small_r2 = 0.04;
 
 // exp(X) for traceless antihermitian X.
 // Cayley-Hamilton: X^3 = a X + b 1 with a = tr(X^2)/2, b = det X, so that
@@ -262,7 +266,7 @@ template <class T> constexpr int exp_terms() { return sizeof(T) == 8 ? 18 : 10; 
 //   alpha_{n+1} = gamma_n b, beta_{n+1} = alpha_n + gamma_n a, gamma_{n+1} = beta_n
 // and exp X = f0 1 + f1 X + f2 X^2, f_k = sum_n (coefficient_n)/n!.  For
 // ||X||_F > 1 the argument is first divided by 2^s (exact) and the result
-// squared s times.
+// squared s times; the (scaled) ||X||_F^2 sets the number of terms.
 template <class T> inline mat3<T> expsu3(mat3<T> X)
 {
 	mat3<T> X2 = mul_nn(X, X);
@@ -293,7 +297,8 @@ template <class T> inline mat3<T> expsu3(mat3<T> X)
 	cplx<T> b = det3(X);
 	cplx<T> al = {T(1), T(0)}, be = {T(0), T(0)}, ga = {T(0), T(0)};
 	cplx<T> f0 = {T(1), T(0)}, f1 = {T(0), T(0)}, f2 = {T(0), T(0)};
-	for (int n = 0; n < exp_terms<T>(); ++n)
+	const int nterms = r2 > T(exp_small_r2) ? exp_terms<T>() : exp_terms_small<T>();
+	for (int n = 0; n < nterms; ++n)
 	{
 		T inv = T(1) / T(n + 1);
 		cplx<T> gb = cmul(ga, b);

@@ -80,6 +80,23 @@ def test_exp_f32():
         assert np.abs(got - E).max() < 1e-6
 
 
+def test_exp_small_argument_terms():
+    """The series stops after fewer terms where ||X||_F <= 0.2: against scipy's
+    expm on both sides of that switch, as accurate as the full series."""
+    import scipy.linalg as sl
+
+    rng = np.random.default_rng(5)
+    for r in (0.01, 0.1, 0.1999, 0.2001, 0.5, 0.99):
+        for _ in range(50):
+            M = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+            X = (M - M.conj().T) / 2
+            X -= np.trace(X) / 3 * np.eye(3)
+            X *= r / np.linalg.norm(X)
+            E = sl.expm(X)
+            assert np.abs(O.su3_exp(X) - E).max() < 5e-16
+            assert np.abs(O.su3_exp(X.astype(np.complex64)) - E).max() < 2.5e-7
+
+
 def test_ta_closed_form():
     rng = np.random.default_rng(1)
     for _ in range(20):
