@@ -40,19 +40,6 @@ namespace {
 	std::exit(2);
 }
 
-// seed string -> 64-bit key (FNV-1a; the reference hashes with sha3,
-// src/apps/hmc.cpp:93)
-uint64_t hash_seed(std::string const &s)
-{
-	uint64_t h = 1469598103934665603ull;
-	for (unsigned char c : s)
-	{
-		h ^= c;
-		h *= 1099511628211ull;
-	}
-	return h;
-}
-
 bool is_number(std::string const &s)
 {
 	return !s.empty() && s.find_first_not_of("0123456789") == std::string::npos;
@@ -135,7 +122,7 @@ int main(int argc, char **argv)
 		    [&]<typename G>() {
 			    Grid g(geom);
 			    Hmc<G> hmc(g);
-			    hmc.seed = is_number(seed) ? std::strtoull(seed.c_str(), nullptr, 10) : hash_seed(seed);
+			    hmc.seed = seed_key(seed);
 			    if (start == "random")
 				    hmc.randomizeGaugeField();
 			    else if (start == "cold")

@@ -53,7 +53,8 @@ def build(verbose=False, extra=(), out=LIB, tag="default"):
     return out
 
 
-APPS = {"mesh_hmc": "hmc.cpp", "test_lattice_gauge": "test_lattice_gauge.cpp", "test_io": "test_io.cpp"}
+APPS = {"mesh_hmc": "hmc.cpp", "mesh_landau": "landau.cpp", "mesh_sweep_beta": "sweep_beta.cpp",
+        "test_lattice_gauge": "test_lattice_gauge.cpp", "test_io": "test_io.cpp"}
 
 
 def build_apps(verbose=False):

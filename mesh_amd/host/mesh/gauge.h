@@ -8,7 +8,10 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <numeric>
 #include <string_view>
 
@@ -111,6 +114,22 @@ template <class G> void random_gauge_field(Field<G> &U, uint64_t seed, uint32_t 
 template <class G> void random_algebra_field(GaugeField<G> &P, uint64_t seed, uint32_t traj)
 {
 	ctx().check(mesh_su3_momenta(ctx().get(), P.grid().c(), G::prec, P.data(), nullptr, seed, traj));
+}
+
+// Philox key of a command-line seed: a decimal number is the key itself, any
+// other string is hashed (FNV-1a; the reference seeds xoshiro256 with
+// sha3<256>(seed), src/apps/hmc.cpp:93)
+inline uint64_t seed_key(std::string const &s)
+{
+	if (!s.empty() && s.size() <= 20 && s.find_first_not_of("0123456789") == std::string::npos)
+		return std::strtoull(s.c_str(), nullptr, 10);
+	uint64_t h = 1469598103934665603ull;
+	for (unsigned char c : s)
+	{
+		h ^= c;
+		h *= 1099511628211ull;
+	}
+	return h;
 }
 
 // ---- molecular dynamics -------------------------------------------------------
@@ -378,9 +397,23 @@ template <class G> struct Landau
 		return F;
 	}
 
+	// the gauge-transformed field U^g_mu(x) = g(x) U_mu(x) g(x+mu)^+
+	GaugeField<G> gauge_fixed() const
+	{
+		GaugeField<G> Ug(U.grid());
+		ctx().check(mesh_su3_gauge_transform(ctx().get(), U.grid().c(), G::prec, Ug.data(), U.data(), g.data()));
+		return Ug;
+	}
+
+	// theta of the transformed links at the last check of run()
+	double last_theta = 0;
+
 	// Even/odd overrelaxation until theta = Nd * condition() < tol
 	// (replaces the line-search ascent of landau.h:100-136); returns sweeps.
-	int run(int max_iter, double omega = 1.7, double tol = 1e-12, int check_every = 10)
+	// The sweeps update the transformed links U^g and g together; every
+	// reunit_every sweeps g is reunitarised and U^g rebuilt from it, which
+	// bounds the drift of g from SU(3) that FP32 products accumulate.
+	int run(int max_iter, double omega = 1.7, double tol = 1e-12, int check_every = 10, int reunit_every = 100)
 	{
 		Context &c = ctx();
 		GaugeField<G> Ug(U.grid());
@@ -388,16 +421,24 @@ template <class G> struct Landau
 		int it = 0;
 		while (it < max_iter)
 		{
-			for (int k = 0; k < check_every && it < max_iter; ++k, ++it)
+			for (int k = 0; k < check_every && it < max_iter; ++k)
+			{
 				for (int parity = 0; parity < 2; ++parity)
 					c.check(mesh_landau_or_halfsweep(c.get(), U.grid().c(), G::prec, Ug.data(), g.data(), parity,
 					                                 omega));
+				++it;
+				if (reunit_every > 0 && it % reunit_every == 0)
+				{
+					reunitize(g);
+					c.check(mesh_su3_gauge_transform(c.get(), U.grid().c(), G::prec, Ug.data(), U.data(), g.data()));
+				}
+			}
 			double d;
 			c.check(mesh_landau_divsum(c.get(), U.grid().c(), G::prec, Ug.data(), nullptr, &d));
-			double theta = d / (double)U.grid().size();
+			last_theta = d / (double)U.grid().size();
 			if (verbose)
-				std::printf("iter = %d, theta = %.3e\n", it, theta);
-			if (theta < tol)
+				std::printf("iter = %d, theta = %.3e\n", it, last_theta);
+			if (last_theta < tol)
 				break;
 		}
 		return it;

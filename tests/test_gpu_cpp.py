@@ -78,3 +78,67 @@ def test_cpp_hmc_driver_u1_4d(precision):
     p, err = float(m.group(1)), float(m.group(2))
     u = sp.iv(1, 0.2) / sp.iv(0, 0.2)
     assert abs(p - u) < 10 * err + 2e-3, (p, err, u)
+
+
+def _app(name, args, timeout=120):
+    r = subprocess.run([os.path.join(BIN, name)] + args, capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-2000:])
+    return r
+
+
+@pytest.mark.parametrize("precision", ["2", "1"])
+def test_cpp_sweep_beta_strong_coupling(precision):
+    """mesh_sweep_beta (src/apps/sweep_beta.cpp): at small beta the SU(3) mean
+    plaquette follows the strong-coupling series beta/18 + beta^2/216."""
+    r = _app("mesh_sweep_beta", ["--geom", "4", "4", "4", "4", "--beta-min", "0", "--beta-max", "1.5",
+                                 "--beta-count", "4", "--count", "40", "--seed", "3", "--precision", precision])
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = re.findall(r"beta = ([0-9.]+), acc = ([0-9.]+), plaq = (-?[0-9.]+)", r.stdout)
+    assert len(rows) == 4
+    for b, acc, p in rows:
+        b, acc, p = float(b), float(acc), float(p)
+        assert acc > 0.5
+        assert abs(p - (b / 18 + b * b / 216)) < 0.02, (b, p)
+
+
+def test_cpp_sweep_beta_u1():
+    """--group u1 on 8x8: the mean plaquette against the exact torus value."""
+    r = _app("mesh_sweep_beta", ["--group", "u1", "--geom", "8", "8", "--beta-min", "0.5", "--beta-max", "2",
+                                 "--beta-count", "2", "--count", "300", "--scheme", "2lf", "--epsilon", "0.5",
+                                 "--substeps", "10", "--seed", "8"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = re.findall(r"beta = ([0-9.]+), acc = ([0-9.]+), plaq = (-?[0-9.]+)", r.stdout)
+    assert [float(b) for b, _, _ in rows] == [0.5, 2.0]
+    for b, _, p in rows:
+        assert abs(float(p) - U.exact_mean_cos(float(b), 64)) < 0.03, (b, p)
+
+
+def test_cpp_landau_app_on_stored_config(tmp_path):
+    """mesh_landau (src/apps/landau.cpp) on a configuration that mesh_hmc
+    stored: overrelaxation reaches theta < 1e-12 and the reference's ascent its
+    own stopping condition, in both precisions; the plaquette is gauge
+    invariant."""
+    ens = str(tmp_path / "e")
+    r = _app("mesh_hmc", ["--geom", "4", "4", "4", "4", "--count", "3", "--scheme", "2mn", "--epsilon", "0.5",
+                          "--substeps", "4", "--seed", "9", "--filename", ens])
+    assert r.returncode == 0, r.stdout + r.stderr
+    cfg = os.path.join(ens, "configs", "3")
+    for precision in ("2", "1"):
+        r = _app("mesh_landau", ["--precision", precision, cfg])
+        assert r.returncode == 0, r.stdout + r.stderr
+        m = re.search(r"landau: (\d+) sweeps, theta = ([0-9.e+-]+), functional = ([0-9.]+)", r.stdout)
+        assert m and float(m.group(2)) < 1e-12, r.stdout[-500:]
+        # theta of U^g rebuilt from the reunitarised g: the FP32 rounding of g
+        # and U^g sets its floor
+        t = float(re.search(r"reunitarised g: theta = ([0-9.e+-]+)", r.stdout).group(1))
+        assert t < (1e-12 if precision == "2" else 1e-9), t
+        pb, pa = map(float, re.search(r"plaquette: before ([0-9.]+), after ([0-9.]+)", r.stdout).groups())
+        assert abs(pb - pa) < (1e-12 if precision == "2" else 1e-6)
+    r = _app("mesh_landau", ["--ascent", "--max-iter", "100000", cfg])
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"landau ascent: (\d+) iterations, condition = ([0-9.e+-]+)", r.stdout)
+    assert m and float(m.group(2)) < 1e-10
+    r = _app("mesh_landau", ["--group", "u1", cfg])
+    assert r.returncode != 0 and "SU(3)" in r.stderr
+    r = _app("mesh_landau", [])
+    assert r.returncode == 2
