@@ -170,6 +170,25 @@ template <class R> MD void madd_nd(m3<R> &S, m3<R> const &A, m3<R> const &B)
 		}
 }
 
+// A^+ (exact: a transpose and a sign)
+template <class R> MD m3<R> adj_m3(m3<R> const &A)
+{
+	m3<R> r;
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+			r.e[3 * i + j] = {A.e[3 * j + i].re, -A.e[3 * j + i].im};
+	return r;
+}
+// First factor of the upper staple of direction mu in plane nu,
+// U_nu(x+mu) U_mu(x+nu)^+ from a = U_nu(x+mu), b = U_mu(x+nu).  For nu < mu
+// it is the adjoint of b a^+, the factor of direction nu in plane mu, so the
+// two directions of a plane use one rounded product.
+template <class R> MD m3<R> upper_factor(m3<R> const &a, m3<R> const &b, bool nu_below_mu)
+{
+	return nu_below_mu ? adj_m3(mul_nd(b, a)) : mul_nd(a, b);
+}
 template <class R> MD m3<R> zero_m3()
 {
 	m3<R> r;

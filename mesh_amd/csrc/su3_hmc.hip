@@ -54,6 +54,13 @@ template <class R, int PATH = 0> constexpr int force_min_waves()
 	           : (MESH_FORCE_WAVES > 0 ? MESH_FORCE_WAVES : 1);
 }
 
+// FP32 wavefront-per-direction kernel: the first upper-staple factor of each
+// plane formed once per workgroup and shared through LDS (0: each direction
+// forms its own; the same bits either way)
+#ifndef MESH_SHARE_UPPER
+#define MESH_SHARE_UPPER 1
+#endif
+
 // non-temporal hints on the once-touched fields of the force kernels
 #ifndef MESH_STREAM_HINTS
 #define MESH_STREAM_HINTS 1
@@ -165,7 +172,8 @@ template <class R> __device__ __forceinline__ m3<R> staple_at(Geo const &g, cons
 		long xmn = shift_c(g, x, nu, c[nu], -1);
 		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
 		{
-			m3<R> T1 = mul_nd(load(U, nu, g, xpm), load(U, mu, g, xpn));
+			m3<R> a = load(U, nu, g, xpm), b = load(U, mu, g, xpn);
+			m3<R> T1 = nu < mu ? adj_m3(mul_nd(b, a)) : mul_nd(a, b);
 			madd_nd(S, T1, load(U, nu, g, x));
 		}
 		{
@@ -193,7 +201,7 @@ __device__ __forceinline__ m3<R> staple_4d(Geo const &g, const R *__restrict__ U
 		long xmn = shift_c(g, x, nu, c[nu], -1);
 		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
 		{
-			m3<R> T1 = mul_nd(load(U, nu, g, xpm), load(U, MU, g, xpn));
+			m3<R> T1 = upper_factor(load(U, nu, g, xpm), load(U, MU, g, xpn), nu < MU);
 			madd_nd(S, T1, load(U, nu, g, x));
 		}
 		{
@@ -210,28 +218,78 @@ __device__ __forceinline__ m3<R> staple_4d(Geo const &g, const R *__restrict__ U
 // holds whole x3-rows, L3 | 64) also those of its x3-neighbours.
 __device__ __forceinline__ int block_slot(int o) { return MESH_SITE_PAIRS ? (((o & 1) << 5) | (o >> 1)) : o; }
 
-template <int MU, bool ROWS, class R>
-__device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict__ U, long x, int const *c,
-                                               vec2_t<R> (*own)[9][64], int lane)
+template <class R> __device__ __forceinline__ m3<R> lds_m3(vec2_t<R> (*a)[9][64], int d, int slot)
 {
-	auto own_at = [&](int d, int slot) {
-		m3<R> m;
+	m3<R> m;
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+	{
+		vec2_t<R> v = a[d][k][slot];
+		m.e[k] = {v.x, v.y};
+	}
+	return m;
+}
+
+// slots of the x3-neighbours x +/- e3 in the staged block (whole x3-rows)
+__device__ __forceinline__ void x3_slots(Geo const &g, long x, int const *c, int &sp, int &sm)
+{
+	const int o = (int)(x & 63), c3 = c[3], L3 = g.L[3], o0 = o - c3;
+	sp = block_slot(o0 + (c3 + 1 == L3 ? 0 : c3 + 1));
+	sm = block_slot(o0 + (c3 == 0 ? L3 - 1 : c3 - 1));
+}
+
+// index of plane (p, q), p < q, among the six of a 4-d lattice
+__host__ __device__ constexpr int plane_index(int p, int q) { return p == 0 ? q - 1 : (p == 1 ? q + 1 : 5); }
+
+// Shared first upper-staple factors: A_pq = U_q(x+p) U_p(x+q)^+ for p < q,
+// the factor of direction p in plane q (direction q uses A_pq^+).  The six
+// planes are split 2, 2, 1, 1 over the wavefronts of directions 0..3.
+template <int W, bool ROWS, class R>
+__device__ __forceinline__ void share_upper_factors(Geo const &g, const R *__restrict__ U, long x, int const *c,
+                                                    vec2_t<R> (*own)[9][64], vec2_t<R> (*sh)[9][64], int lane)
+{
+	int sp = 0, sm = 0;
+	if constexpr (ROWS)
+		x3_slots(g, x, c, sp, sm);
+	auto one = [&](auto P, auto Q) {
+		constexpr int p = decltype(P)::value, q = decltype(Q)::value;
+		m3<R> a = (ROWS && p == 3) ? lds_m3<R>(own, q, sp) : load(U, q, g, shift_c(g, x, p, c[p], 1));
+		m3<R> b = (ROWS && q == 3) ? lds_m3<R>(own, p, sp) : load(U, p, g, shift_c(g, x, q, c[q], 1));
+		m3<R> A = mul_nd(a, b);
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
-		{
-			vec2_t<R> v = own[d][k][slot];
-			m.e[k] = {v.x, v.y};
-		}
-		return m;
+			sh[plane_index(p, q)][k][lane] = vec2_t<R>{A.e[k].re, A.e[k].im};
 	};
+	using I0 = std::integral_constant<int, 0>;
+	using I1 = std::integral_constant<int, 1>;
+	using I2 = std::integral_constant<int, 2>;
+	using I3 = std::integral_constant<int, 3>;
+	if constexpr (W == 0)
+	{
+		one(I0{}, I1{});
+		one(I0{}, I2{});
+	}
+	else if constexpr (W == 1)
+	{
+		one(I1{}, I2{});
+		one(I1{}, I3{});
+	}
+	else if constexpr (W == 2)
+		one(I2{}, I3{});
+	else
+		one(I0{}, I3{});
+}
+
+// SH: the first upper-staple factors come from sh (share_upper_factors)
+template <int MU, bool ROWS, class R, bool SH = false>
+__device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict__ U, long x, int const *c,
+                                               vec2_t<R> (*own)[9][64], int lane, vec2_t<R> (*sh)[9][64] = nullptr)
+{
+	auto own_at = [&](int d, int slot) { return lds_m3<R>(own, d, slot); };
 	// slots of the x3-neighbours x +/- e3 (used with ROWS only)
 	int sp = 0, sm = 0;
 	if constexpr (ROWS)
-	{
-		const int o = (int)(x & 63), c3 = c[3], L3 = g.L[3], o0 = o - c3;
-		sp = block_slot(o0 + (c3 + 1 == L3 ? 0 : c3 + 1));
-		sm = block_slot(o0 + (c3 == 0 ? L3 - 1 : c3 - 1));
-	}
+		x3_slots(g, x, c, sp, sm);
 	m3<R> S = zero_m3<R>();
 	long xpm = shift_c(g, x, MU, c[MU], 1);
 #pragma unroll
@@ -244,10 +302,16 @@ __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict
 		long xpn = nu3 ? 0 : shift_c(g, x, nu, c[nu], 1);
 		long xmn = nu3 ? 0 : shift_c(g, x, nu, c[nu], -1);
 		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
+		if constexpr (SH)
+		{
+			m3<R> T1 = nu < MU ? adj_m3(lds_m3<R>(sh, plane_index(nu, MU), lane)) : lds_m3<R>(sh, plane_index(MU, nu), lane);
+			madd_nd(S, T1, own_at(nu, lane));
+		}
+		else
 		{
 			m3<R> a = fwd3 ? own_at(nu, sp) : load(U, nu, g, xpm);
 			m3<R> b = nu3 ? own_at(MU, sp) : load(U, MU, g, xpn);
-			m3<R> T1 = mul_nd(a, b);
+			m3<R> T1 = upper_factor(a, b, nu < MU);
 			madd_nd(S, T1, own_at(nu, lane));
 		}
 		{
@@ -573,21 +637,58 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 		int c[4];
 		coords<R>(g, x, c);
 		constexpr bool RW = PATH == STAPLE_ROWS; // the block holds whole x3-rows
-		switch (mu)
+		constexpr bool SH = MESH_SHARE_UPPER && std::is_same<R, float>::value;
+		if constexpr (SH)
 		{
-		case 0:
-			S = staple_4d_own<0, RW>(g, U, x, c, own, lane);
-			break;
-		case 1:
-			S = staple_4d_own<1, RW>(g, U, x, c, own, lane);
-			break;
-		case 2:
-			S = staple_4d_own<2, RW>(g, U, x, c, own, lane);
-			break;
-		default:
-			S = staple_4d_own<3, RW>(g, U, x, c, own, lane);
-			break;
+			__shared__ vec2_t<R> sh[6][9][64];
+			switch (mu)
+			{
+			case 0:
+				share_upper_factors<0, RW>(g, U, x, c, own, sh, lane);
+				break;
+			case 1:
+				share_upper_factors<1, RW>(g, U, x, c, own, sh, lane);
+				break;
+			case 2:
+				share_upper_factors<2, RW>(g, U, x, c, own, sh, lane);
+				break;
+			default:
+				share_upper_factors<3, RW>(g, U, x, c, own, sh, lane);
+				break;
+			}
+			__syncthreads();
+			switch (mu)
+			{
+			case 0:
+				S = staple_4d_own<0, RW, R, true>(g, U, x, c, own, lane, sh);
+				break;
+			case 1:
+				S = staple_4d_own<1, RW, R, true>(g, U, x, c, own, lane, sh);
+				break;
+			case 2:
+				S = staple_4d_own<2, RW, R, true>(g, U, x, c, own, lane, sh);
+				break;
+			default:
+				S = staple_4d_own<3, RW, R, true>(g, U, x, c, own, lane, sh);
+				break;
+			}
 		}
+		else
+			switch (mu)
+			{
+			case 0:
+				S = staple_4d_own<0, RW>(g, U, x, c, own, lane);
+				break;
+			case 1:
+				S = staple_4d_own<1, RW>(g, U, x, c, own, lane);
+				break;
+			case 2:
+				S = staple_4d_own<2, RW>(g, U, x, c, own, lane);
+				break;
+			default:
+				S = staple_4d_own<3, RW>(g, U, x, c, own, lane);
+				break;
+			}
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
 		{
@@ -701,7 +802,7 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 			{
 				m3<R> a = mloc ? from_lds(nu, d2(mu, 1), d3(mu, 1)) : load(U, nu, g, xpm);
 				m3<R> b = nloc ? from_lds(mu, d2(nu, 1), d3(nu, 1)) : load(U, mu, g, xpn);
-				m3<R> T1 = mul_nd(a, b);
+				m3<R> T1 = nu < mu ? adj_m3(mul_nd(b, a)) : mul_nd(a, b);
 				madd_nd(S, T1, from_lds(nu, 0, 0));
 			}
 			{

@@ -90,6 +90,9 @@ template <class T> inline mat3<T> down(mat3<double> const &a)
 // staple sum for direction mu at x (src/gauge/utils.h:110-139):
 //   sum_{nu != mu} U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
 //                + U_nu(x-nu+mu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+// For nu < mu the first factor U_nu(x+mu) U_mu(x+nu)^+ is formed as the
+// adjoint of U_mu(x+nu) U_nu(x+mu)^+, the first factor of direction nu in
+// plane mu, so that both directions of a plane use one rounded product.
 template <class T> inline mat3<T> staple_at(T const *U, Geo const &g, int mu, int64_t x)
 {
 	mat3<T> S = zero3<T>();
@@ -101,7 +104,8 @@ template <class T> inline mat3<T> staple_at(T const *U, Geo const &g, int mu, in
 		int64_t xpn = g.shift(x, nu, 1);
 		int64_t xmn = g.shift(x, nu, -1);
 		int64_t xpmmn = g.shift(xpm, nu, -1);
-		mat3<T> T1 = mul_nd(ld(U, g.V, nu, xpm), ld(U, g.V, mu, xpn));
+		mat3<T> T1 = nu < mu ? adj3(mul_nd(ld(U, g.V, mu, xpn), ld(U, g.V, nu, xpm)))
+		                     : mul_nd(ld(U, g.V, nu, xpm), ld(U, g.V, mu, xpn));
 		madd_nd(S, T1, ld(U, g.V, nu, x));
 		mat3<T> T2 = mul_dd(ld(U, g.V, nu, xpmmn), ld(U, g.V, mu, xmn));
 		madd_nn(S, T2, ld(U, g.V, nu, xmn));
