@@ -54,12 +54,23 @@ template <class R, int PATH = 0> constexpr int force_min_waves()
 	           : (MESH_FORCE_WAVES > 0 ? MESH_FORCE_WAVES : 1);
 }
 
-// FP32 wavefront-per-direction kernel: the first upper-staple factor of each
-// plane formed once per workgroup and shared through LDS (0: each direction
-// forms its own; the same bits either way)
+// Wavefront-per-direction kernel: the first upper-staple factor of a plane
+// formed once per workgroup and shared through LDS (the same bits either
+// way).  MESH_SHARE_UPPER (FP32) / MESH_SHARE_UPPER_F64: 0 each direction
+// forms its own, 1 the three planes (0, nu) are shared, 2 all six.  FP64 with
+// all six would need 91 KB of LDS per workgroup, one workgroup per CU.
 #ifndef MESH_SHARE_UPPER
-#define MESH_SHARE_UPPER 1
+#define MESH_SHARE_UPPER 2
 #endif
+#ifndef MESH_SHARE_UPPER_F64
+#define MESH_SHARE_UPPER_F64 1
+#endif
+template <class R> constexpr int share_mode()
+{
+	return std::is_same<R, float>::value ? MESH_SHARE_UPPER : MESH_SHARE_UPPER_F64;
+}
+// is plane (p, q), p < q, shared in mode m
+__host__ __device__ constexpr bool plane_shared(int m, int p) { return m == 2 || (m == 1 && p == 0); }
 
 // non-temporal hints on the once-touched fields of the force kernels
 #ifndef MESH_STREAM_HINTS
@@ -242,9 +253,10 @@ __device__ __forceinline__ void x3_slots(Geo const &g, long x, int const *c, int
 __host__ __device__ constexpr int plane_index(int p, int q) { return p == 0 ? q - 1 : (p == 1 ? q + 1 : 5); }
 
 // Shared first upper-staple factors: A_pq = U_q(x+p) U_p(x+q)^+ for p < q,
-// the factor of direction p in plane q (direction q uses A_pq^+).  The six
-// planes are split 2, 2, 1, 1 over the wavefronts of directions 0..3.
-template <int W, bool ROWS, class R>
+// the factor of direction p in plane q (direction q uses A_pq^+).  Mode 2:
+// the six planes split 2, 2, 1, 1 over the wavefronts of directions 0..3;
+// mode 1: planes (0, w) formed by wavefront w = 1, 2, 3.
+template <int W, bool ROWS, int MODE, class R>
 __device__ __forceinline__ void share_upper_factors(Geo const &g, const R *__restrict__ U, long x, int const *c,
                                                     vec2_t<R> (*own)[9][64], vec2_t<R> (*sh)[9][64], int lane)
 {
@@ -264,7 +276,16 @@ __device__ __forceinline__ void share_upper_factors(Geo const &g, const R *__res
 	using I1 = std::integral_constant<int, 1>;
 	using I2 = std::integral_constant<int, 2>;
 	using I3 = std::integral_constant<int, 3>;
-	if constexpr (W == 0)
+	if constexpr (MODE == 1)
+	{
+		if constexpr (W == 1)
+			one(I0{}, I1{});
+		else if constexpr (W == 2)
+			one(I0{}, I2{});
+		else if constexpr (W == 3)
+			one(I0{}, I3{});
+	}
+	else if constexpr (W == 0)
 	{
 		one(I0{}, I1{});
 		one(I0{}, I2{});
@@ -280,8 +301,8 @@ __device__ __forceinline__ void share_upper_factors(Geo const &g, const R *__res
 		one(I0{}, I3{});
 }
 
-// SH: the first upper-staple factors come from sh (share_upper_factors)
-template <int MU, bool ROWS, class R, bool SH = false>
+// SH: share mode of the first upper-staple factors (share_upper_factors)
+template <int MU, bool ROWS, class R, int SH = 0>
 __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict__ U, long x, int const *c,
                                                vec2_t<R> (*own)[9][64], int lane, vec2_t<R> (*sh)[9][64] = nullptr)
 {
@@ -302,7 +323,7 @@ __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict
 		long xpn = nu3 ? 0 : shift_c(g, x, nu, c[nu], 1);
 		long xmn = nu3 ? 0 : shift_c(g, x, nu, c[nu], -1);
 		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
-		if constexpr (SH)
+		if (plane_shared(SH, nu < MU ? nu : MU))
 		{
 			m3<R> T1 = nu < MU ? adj_m3(lds_m3<R>(sh, plane_index(nu, MU), lane)) : lds_m3<R>(sh, plane_index(MU, nu), lane);
 			madd_nd(S, T1, own_at(nu, lane));
@@ -637,39 +658,40 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 		int c[4];
 		coords<R>(g, x, c);
 		constexpr bool RW = PATH == STAPLE_ROWS; // the block holds whole x3-rows
-		constexpr bool SH = MESH_SHARE_UPPER && std::is_same<R, float>::value;
-		if constexpr (SH)
+		constexpr int SH = share_mode<R>();
+		if constexpr (SH > 0)
 		{
-			__shared__ vec2_t<R> sh[6][9][64];
+			// planes (0, q) are indices 0..2, so mode 1 needs three
+			__shared__ vec2_t<R> sh[SH == 2 ? 6 : 3][9][64];
 			switch (mu)
 			{
 			case 0:
-				share_upper_factors<0, RW>(g, U, x, c, own, sh, lane);
+				share_upper_factors<0, RW, SH>(g, U, x, c, own, sh, lane);
 				break;
 			case 1:
-				share_upper_factors<1, RW>(g, U, x, c, own, sh, lane);
+				share_upper_factors<1, RW, SH>(g, U, x, c, own, sh, lane);
 				break;
 			case 2:
-				share_upper_factors<2, RW>(g, U, x, c, own, sh, lane);
+				share_upper_factors<2, RW, SH>(g, U, x, c, own, sh, lane);
 				break;
 			default:
-				share_upper_factors<3, RW>(g, U, x, c, own, sh, lane);
+				share_upper_factors<3, RW, SH>(g, U, x, c, own, sh, lane);
 				break;
 			}
 			__syncthreads();
 			switch (mu)
 			{
 			case 0:
-				S = staple_4d_own<0, RW, R, true>(g, U, x, c, own, lane, sh);
+				S = staple_4d_own<0, RW, R, SH>(g, U, x, c, own, lane, sh);
 				break;
 			case 1:
-				S = staple_4d_own<1, RW, R, true>(g, U, x, c, own, lane, sh);
+				S = staple_4d_own<1, RW, R, SH>(g, U, x, c, own, lane, sh);
 				break;
 			case 2:
-				S = staple_4d_own<2, RW, R, true>(g, U, x, c, own, lane, sh);
+				S = staple_4d_own<2, RW, R, SH>(g, U, x, c, own, lane, sh);
 				break;
 			default:
-				S = staple_4d_own<3, RW, R, true>(g, U, x, c, own, lane, sh);
+				S = staple_4d_own<3, RW, R, SH>(g, U, x, c, own, lane, sh);
 				break;
 			}
 		}
