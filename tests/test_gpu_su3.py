@@ -126,6 +126,41 @@ def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
     np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
 
 
+# 4-d shapes with V % 64 == 0 whose fastest extent does not divide 64: the
+# wavefront-per-direction kernel stages the block but reads x3-neighbours from
+# global memory (its "own links" path, with the shared upper-staple products)
+OWN_SHAPES = [(2, 4, 4, 6), (4, 2, 8, 12), (4, 4, 4, 3)]
+
+
+@pytest.mark.parametrize("remap", [0, 3])
+@pytest.mark.parametrize("variant", [-1, 1])
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", OWN_SHAPES)
+def test_force_own_path_bitexact(ctx, shape, prec, variant, remap):
+    U = O.random_gauge(list(shape), 23, _pname(prec))
+    P, _ = O.momenta(list(shape), 23, 0, _pname(prec))
+    g, fu = _field(ctx, shape, prec, U)
+    _, fp = _field(ctx, shape, prec, P)
+    with options(ctx, force_variant=variant, xcd_remap=remap):
+        gauge.momentumStep(fu, fp, 6.0, 0.0833)
+    np.testing.assert_array_equal(fp.download(), O.force_update(U, P, list(shape), 6.0, 0.0833))
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("shape", OWN_SHAPES[:2])
+def test_fused_trajectory_own_path_bitexact(ctx, shape, prec):
+    shape = list(shape)
+    hmc = gauge.Hmc(mesh_amd.Grid(shape), prec, seed=8, ctx=ctx)
+    hmc.randomizeGaugeField()
+    U = O.random_gauge(shape, 8, _pname(prec))
+    deltas = gauge.makeDeltas("2mn", 0.4, 2)
+    for traj in range(2):
+        r = hmc.runHmcUpdate(5.5, deltas, metropolis=True)
+        o = O.hmc_trajectory(U, shape, 5.5, deltas, 8, traj, metropolis=True)
+        assert r["accepted"] == o["accepted"]
+        np.testing.assert_array_equal(hmc.U.download(), U)
+
+
 @pytest.mark.parametrize("order", [3, 4])
 @pytest.mark.parametrize("tile", [1, 2, 4])
 @pytest.mark.parametrize("variant", [0, 2])
