@@ -37,16 +37,18 @@ int main(int argc, char **argv)
 			for (size_t i = 0; i < t.size(); ++i)
 				t[i] = (float)i / 3.0f;
 			io::write_npy(e.config_path("2"), "<f4", io::config_shape(Grid({5, 6}), 1), t.data());
-			// a second create of the same directory must refuse
-			try
-			{
-				io::Ensemble::create(dir, false);
-				std::printf("overwrite not refused\n");
-				return 1;
-			}
-			catch (std::runtime_error const &)
-			{
-			}
+			// a second create of the same directory must refuse, and so must a
+			// create over a non-empty directory that is not an ensemble
+			for (std::string d : {dir, dir + "/configs"})
+				try
+				{
+					io::Ensemble::create(d, false);
+					std::printf("overwrite of %s not refused\n", d.c_str());
+					return 1;
+				}
+				catch (std::runtime_error const &)
+				{
+				}
 			std::printf("written\n");
 			return 0;
 		}

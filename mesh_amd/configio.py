@@ -40,10 +40,11 @@ class Ensemble:
 
     @classmethod
     def create(cls, path, allow_overwrite=False):
-        """New ensemble; an existing one only with allow_overwrite, which drops
-        its configurations (Hdf5File::create(filename, allow_overwrite))."""
+        """New ensemble; an existing non-empty directory only with
+        allow_overwrite, which drops its configurations
+        (Hdf5File::create(filename, allow_overwrite))."""
         path = str(path)
-        if os.path.exists(os.path.join(path, _ATTR)) and not allow_overwrite:
+        if os.path.exists(path) and not (os.path.isdir(path) and not os.listdir(path)) and not allow_overwrite:
             raise FileExistsError(f"ensemble '{path}' exists (allow_overwrite to replace it)")
         cfg = os.path.join(path, "configs")
         if os.path.isdir(cfg):

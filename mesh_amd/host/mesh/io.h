@@ -387,12 +387,13 @@ class Ensemble
 	std::filesystem::path const &dir() const { return dir_; }
 	Attributes const &attributes() const { return attr_; }
 
-	// a new ensemble directory; an existing one only with allow_overwrite
-	// (the reference's Hdf5File::create(filename, allow_overwrite))
+	// a new ensemble directory; an existing non-empty one only with
+	// allow_overwrite, which drops its configurations (the reference's
+	// Hdf5File::create(filename, allow_overwrite))
 	static Ensemble create(std::string const &dir, bool allow_overwrite)
 	{
 		namespace fs = std::filesystem;
-		if (fs::exists(fs::path(dir) / "attributes.json") && !allow_overwrite)
+		if (fs::exists(dir) && !(fs::is_directory(dir) && fs::is_empty(dir)) && !allow_overwrite)
 			throw std::runtime_error("ensemble '" + dir + "' exists (allow overwriting to replace it)");
 		Ensemble e;
 		e.dir_ = dir;

@@ -75,6 +75,14 @@ def test_create_refuses_existing(tmp_path):
         configio.Ensemble.create(d)
     configio.Ensemble.create(d, allow_overwrite=True)
     assert os.listdir(os.path.join(d, "configs")) == []
+    # a directory that is not an ensemble is not touched either
+    other = tmp_path / "notes"
+    (other / "configs").mkdir(parents=True)
+    (other / "configs" / "keep.txt").write_text("x")
+    with pytest.raises(FileExistsError):
+        configio.Ensemble.create(str(other))
+    assert (other / "configs" / "keep.txt").exists()
+    configio.Ensemble.create(str(tmp_path / "empty"))
 
 
 def test_config_names(tmp_path):
