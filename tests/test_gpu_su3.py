@@ -424,3 +424,43 @@ def test_cshift_and_sum(ctx, shape, prec):
         gauge.cshift(f, 0, 0)
     with pytest.raises(mesh_amd.MeshError):
         gauge.cshift(f, 0, sh[0])
+
+
+def _binned(x, nbin=10):
+    """mean and error of the mean from nbin consecutive blocks (autocorrelation)."""
+    x = np.asarray(x, np.float64)
+    b = x[: len(x) // nbin * nbin].reshape(nbin, -1).mean(axis=1)
+    return x.mean(), b.std(ddof=1) / np.sqrt(nbin)
+
+
+def test_ensemble_f32_vs_f64_creutz():
+    """Ensemble statistics on 6^4 at beta 5.7 (2mn, eps 0.5, 6 substeps): the
+    FP32-mixed chain's mean plaquette and acceptance agree with the FP64
+    chain's, <exp(-dH)> = 1 (Creutz; src/gauge/hmc.h:98-99) and the acceptance
+    follows erfc(sqrt(<dH>)/2)."""
+    import math
+
+    ctx = mesh_amd.default_context()
+    g = mesh_amd.Grid([6, 6, 6, 6])
+    deltas = gauge.makeDeltas("2mn", 0.5, 6)
+    res = {}
+    for prec in (F64, F32):
+        hmc = gauge.Hmc(g, prec, seed=3, ctx=ctx)
+        hmc.randomizeGaugeField()
+        for _ in range(60):
+            hmc.runHmcUpdate(5.7, deltas, metropolis=False)
+        rows = [hmc.runHmcUpdate(5.7, deltas) for _ in range(400)]
+        res[prec] = dict(plaq=[r["plaq"] for r in rows], acc=[float(r["accepted"]) for r in rows],
+                         dH=[r["dH"] for r in rows])
+    p64, e64 = _binned(res[F64]["plaq"])
+    p32, e32 = _binned(res[F32]["plaq"])
+    print("plaquette", p64, e64, p32, e32)
+    assert abs(p64 - p32) < 5 * math.hypot(e64, e32) + 1e-4
+    a64, a32 = np.mean(res[F64]["acc"]), np.mean(res[F32]["acc"])
+    assert 0.5 < a64 < 1.0 and abs(a64 - a32) < 0.1, (a64, a32)
+    for prec in (F64, F32):
+        c, ec = _binned(np.exp(-np.asarray(res[prec]["dH"])))
+        print("creutz", prec, c, ec)
+        assert abs(c - 1) < 5 * ec + 0.02, (prec, c, ec)
+    dh = np.mean(res[F64]["dH"])
+    assert abs(a64 - math.erfc(math.sqrt(max(dh, 0.0)) / 2)) < 0.1, (a64, dh)
