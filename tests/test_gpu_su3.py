@@ -464,3 +464,33 @@ def test_ensemble_f32_vs_f64_creutz():
         assert abs(c - 1) < 5 * ec + 0.02, (prec, c, ec)
     dh = np.mean(res[F64]["dH"])
     assert abs(a64 - math.erfc(math.sqrt(max(dh, 0.0)) / 2)) < 0.1, (a64, dh)
+
+
+def test_integrator_order():
+    """rms dH over eight momentum draws from one 8^4 configuration, short
+    trajectories (eps 0.02, tau 0.2), per halving of the step: slope 2 for
+    leapfrog and 4 for 4mn (src/gauge/hmc.cpp:7-51).  FP64: at these steps
+    dH is below the FP32 storage resolution of the energies."""
+    import math
+
+    ctx = mesh_amd.default_context()
+    g = mesh_amd.Grid([8, 8, 8, 8])
+    hmc = gauge.Hmc(g, F64, seed=4, ctx=ctx)
+    hmc.randomizeGaugeField()
+    for _ in range(30):
+        hmc.runHmcUpdate(6.0, gauge.makeDeltas("2mn", 0.5, 6), metropolis=False)
+    start = mesh_amd.GaugeField(g, F64, ctx).copy_from(hmc.U)
+
+    def dh(scheme, n):
+        out = []
+        for t in range(8):
+            hmc.U.copy_from(start)
+            hmc.traj = 1000 + t  # the same eight momentum draws for every step size
+            out.append(hmc.runHmcUpdate(6.0, gauge.makeDeltas(scheme, 0.02, n), metropolis=False)["dH"])
+        return math.sqrt(np.mean(np.square(out)))
+
+    slope = lambda scheme, n: math.log2(dh(scheme, n) / dh(scheme, 2 * n))
+    s = slope("2lf", 8)
+    assert 1.58 < s < 2.33, s
+    s = slope("4mn", 4)
+    assert 3.3 < s < 4.5, s
