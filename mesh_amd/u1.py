@@ -1,6 +1,6 @@
 """U(1) 2-d ensembles on the GPU: batched HMC, the coupling-layer flow
 (training by reverse KL, sampling, inverse), independence Metropolis-Hastings
-interleaved with HMC, and the autocorrelation analysis that judges them.
+interleaved with HMC, and the autocorrelation analysis that compares them.
 
 Everything numerical runs in libmesh_hip.so (u1.hip, flow.hip); this module
 only sequences launches and reads back per-chain scalars.
