@@ -8,7 +8,8 @@
 //   FP32-mixed, 2 = FP64) --filename <ensemble> [--force]: every
 //   configuration and the histories go to an ensemble directory (the
 //   reference's HDF5 file, src/apps/hmc.cpp:63-86, 119-124; format in
-//   mesh/io.h).  A start configuration from an ensemble sets the geometry
+//   mesh/io.h); --spacing K writes every K-th configuration (the ensemble's
+//   markov_spacing attribute).  A start configuration from an ensemble sets the geometry
 //   when --geom is not given and, without --seed, continues that chain's
 //   Philox streams, so the run reproduces the uninterrupted chain bit for bit.
 // Additions: --therm K (first K trajectories without Metropolis),
@@ -35,7 +36,8 @@ namespace {
 	             "error: %s\nusage: mesh_hmc [--group su3|u1] [--geom L0 L1 [L2 [L3]]] [--beta B]\n"
 	             "                [--start random|cold|<ensemble>/configs/<n>] [--seed S] [--count N]\n"
 	             "                [--scheme 2lf|2mn|4mn] [--epsilon E] [--substeps N] [--precision 1|2]\n"
-	             "                [--filename <ensemble> [--force]] [--therm K] [--landau] [--landau-ascent]\n",
+	             "                [--filename <ensemble> [--force] [--spacing K]] [--therm K] [--landau]\n"
+	             "                [--landau-ascent]\n",
 	             msg);
 	std::exit(2);
 }
@@ -52,7 +54,7 @@ int main(int argc, char **argv)
 	std::string group = "su3", scheme = "4mn", start = "random", seed = "";
 	std::vector<int32_t> geom = {8, 8, 8, 8};
 	double beta = 6.0, epsilon = 1.0;
-	int substeps = 8, count = 100, precision = 2, therm = 0;
+	int substeps = 8, count = 100, precision = 2, therm = 0, spacing = 1;
 	bool landau = false, ascent = false, geom_given = false, allow_overwrite = false;
 	std::string filename;
 
@@ -101,11 +103,15 @@ int main(int argc, char **argv)
 			filename = next();
 		else if (a == "--force")
 			allow_overwrite = true;
+		else if (a == "--spacing")
+			spacing = std::atoi(next().c_str());
 		else if (a == "--help" || a == "-h")
 			usage("help requested");
 		else
 			usage(("unknown option " + a).c_str());
 	}
+	if (spacing < 1)
+		usage("--spacing must be positive");
 	const bool seed_given = !seed.empty();
 	if (seed.empty())
 		seed = std::to_string(std::random_device{}());
@@ -154,7 +160,7 @@ int main(int argc, char **argv)
 				    ens.set_attribute("markov_start", start);
 				    ens.set_attribute("markov_discard", therm);
 				    ens.set_attribute("markov_count", count);
-				    ens.set_attribute("markov_spacing", 1);
+				    ens.set_attribute("markov_spacing", spacing);
 				    ens.set_attribute("precision", precision);
 				    ens.set_attribute("philox_key", std::to_string(hmc.seed));
 				    ens.set_attribute("philox_traj0", (long long)hmc.traj);
@@ -164,7 +170,7 @@ int main(int argc, char **argv)
 				    hmc.runHmcUpdate(beta, deltas, iter >= therm);
 				    if (iter == therm - 1)
 					    hmc.reset_observables();
-				    if (ens)
+				    if (ens && (iter + 1) % spacing == 0)
 					    writeConfig(ens, std::to_string(iter + 1), hmc.U);
 			    }
 			    hmc.print_summary();

@@ -111,6 +111,13 @@ def test_cpp_chain_resumes_bit_exact(tmp_path, precision):
     assert r.returncode != 0 and "group mismatch" in r.stderr
     r = _run(["--geom", "4", "4", "4", "2", "--start", os.path.join(b, "configs", "2"), "--count", "1"] + common)
     assert r.returncode != 0 and "grid mismatch" in r.stderr
+    # --spacing: every second configuration, recorded in markov_spacing
+    s = str(tmp_path / "s")
+    r = _run(["--geom", "4", "4", "4", "4", "--count", "5", "--seed", "17", "--spacing", "2", "--filename", s] + common)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert sorted(os.listdir(os.path.join(s, "configs"))) == ["2.npy", "4.npy"]
+    assert configio.Ensemble.open(s).attributes["markov_spacing"] == 2
+    np.testing.assert_array_equal(np.load(os.path.join(s, "configs", "4.npy")), np.load(A.config_path("4")))
     # an existing ensemble is not overwritten without --force
     r = _run(["--geom", "4", "4", "4", "4", "--count", "1", "--seed", "1", "--filename", a] + common)
     assert r.returncode != 0 and "exists" in r.stderr
