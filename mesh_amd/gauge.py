@@ -186,6 +186,19 @@ class Hmc:
     def randomizeMomenta(self):
         randomMomenta(self.P, self.seed, self.traj, self.K)
 
+    def load(self, config_name, resume=True):
+        """U from a stored configuration "<ensemble>/configs/<n>" (readConfig,
+        src/gauge/utils.h:188-219); with resume, and where the ensemble records
+        them, the Philox key and trajectory counter continue the chain that
+        stored it, bit for bit."""
+        from . import configio
+
+        configio.read_config(config_name, self.U)
+        state = configio.resume_state(config_name) if resume else None
+        if state is not None:
+            self.seed, self.traj = state
+        return self
+
     def runHmcUpdate(self, beta, deltas, metropolis=True):
         """momenta -> trajectory -> accept/reject -> measure (hmc.cpp:75-107)."""
         t0 = time.perf_counter()

@@ -67,9 +67,8 @@ def test_python_chain_resumes_bit_exact(ctx, tmp_path, prec):
         configio.write_config(e, str(n + 1), first.U)
 
     name = os.path.join(e.path, "configs", "2")
-    second = gauge.Hmc(g, prec, seed=0, ctx=ctx)
-    configio.read_config(name, second.U)
-    second.seed, second.traj = configio.resume_state(name)
+    second = gauge.Hmc(g, prec, seed=0, ctx=ctx).load(name)
+    assert (second.seed, second.traj) == (11, 2)
     for _ in range(2):
         second.runHmcUpdate(6.0, deltas)
     np.testing.assert_array_equal(second.U.download(), straight.U.download())
