@@ -29,10 +29,12 @@ FLAGS = [
 def build(verbose=False, extra=(), out=LIB, tag="default"):
     """Compile every source and link ``out``; ``extra`` are additional hipcc
     flags (e.g. -DMESH_SITE_LB=5 for another site-block length)."""
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
+
     bdir = os.path.join(HERE, "_build", tag)
     os.makedirs(bdir, exist_ok=True)
-    for src in SOURCES:
+
+    def compile_one(src):
         obj = os.path.join(bdir, src.replace(".hip", ".o"))
         cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
@@ -44,7 +46,12 @@ def build(verbose=False, extra=(), out=LIB, tag="default"):
             print(err, file=sys.stderr)
         if r.returncode != 0:
             raise subprocess.CalledProcessError(r.returncode, cmd)
-        objs.append(obj)
+        return obj
+
+    # the sources are independent translation units: compile them side by side
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as pool:
+        objs = list(pool.map(compile_one, SOURCES))
     tmp = out + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
     os.replace(tmp, out)

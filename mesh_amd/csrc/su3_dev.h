@@ -298,11 +298,18 @@ MD void exp_series(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0
 	(exp_series_term<R, N0 + N>(a, b, al, be, ga, f0, f1, f2), ...);
 }
 
-// exp of a traceless antihermitian matrix: Cayley-Hamilton with
+// exp(X) - 1 of a traceless antihermitian matrix: Cayley-Hamilton with
 // power-series coefficients (X^3 = a X + b with a = tr X^2 / 2, b = det X),
-// argument halved until ||X||_F <= 1 and the result squared back; the series
-// stops after exp_terms_small() terms where ||X||_F^2 <= EXP_SMALL_R2.
-template <class R> MD m3<R> expsu3(m3<R> X)
+// exp X = f0 1 + f1 X + f2 X^2, argument halved until ||X||_F <= 1 and the
+// result squared back as D -> 2 D + D^2; the series stops after
+// exp_terms_small() terms where ||X||_F^2 <= EXP_SMALL_R2.
+//
+// The 1 is never added: f0 - 1 is summed from its small terms and every entry
+// of D is formed at its own magnitude.  Rounding exp X itself to FP32 puts a
+// diagonal near 1 on a grid of 6e-8 and makes |exp X| systematically 1.4e-9
+// too small per link step (measured over 4e5 random links); in U + D U only
+// the final sum is rounded at the magnitude of U, which is unbiased.
+template <class R> MD m3<R> expm1su3(m3<R> X)
 {
 	m3<R> X2 = mul_nn(X, X);
 	R trx2 = (X2.e[0].re + X2.e[4].re) + X2.e[8].re;
@@ -332,42 +339,117 @@ template <class R> MD m3<R> expsu3(m3<R> X)
 	R a = R(0.5) * trx2;
 	cpx<R> b = det(X);
 	cpx<R> al = {R(1), R(0)}, be = c0<R>(), ga = c0<R>();
-	cpx<R> f0 = {R(1), R(0)}, f1 = c0<R>(), f2 = c0<R>();
+	cpx<R> g0 = c0<R>(), f1 = c0<R>(), f2 = c0<R>();
 	constexpr int NS = exp_terms_small<R>(), NF = exp_terms<R>();
-	exp_series<R, 0>(a, b, al, be, ga, f0, f1, f2, std::make_integer_sequence<int, NS>{});
+	exp_series<R, 0>(a, b, al, be, ga, g0, f1, f2, std::make_integer_sequence<int, NS>{});
 	if (r2 > R(EXP_SMALL_R2))
-		exp_series<R, NS>(a, b, al, be, ga, f0, f1, f2, std::make_integer_sequence<int, NF - NS>{});
-	m3<R> E;
+		exp_series<R, NS>(a, b, al, be, ga, g0, f1, f2, std::make_integer_sequence<int, NF - NS>{});
+	m3<R> D;
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
-		cpx<R> acc = (k % 4 == 0) ? f0 : c0<R>();
+		cpx<R> acc = (k % 4 == 0) ? g0 : c0<R>();
 		cmadd(acc, f1, X.e[k]);
 		cmadd(acc, f2, X2.e[k]);
-		E.e[k] = acc;
+		D.e[k] = acc;
 	}
 	for (int k = 0; k < s; ++k)
-		E = mul_nn(E, E);
+	{
+		m3<R> N;
+#pragma unroll
+		for (int i = 0; i < 3; ++i)
+#pragma unroll
+			for (int j = 0; j < 3; ++j)
+			{
+				cpx<R> acc = {D.e[3 * i + j].re + D.e[3 * i + j].re, D.e[3 * i + j].im + D.e[3 * i + j].im};
+#pragma unroll
+				for (int l = 0; l < 3; ++l)
+					cmadd(acc, D.e[3 * i + l], D.e[3 * l + j]);
+				N.e[3 * i + j] = acc;
+			}
+		D = N;
+	}
+	return D;
+}
+
+// U + D U (D = exp X - 1): the product summed from zero, then one rounding at
+// the magnitude of U
+template <class R> MD m3<R> add_mul(m3<R> const &D, m3<R> const &U)
+{
+	m3<R> r;
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			cpx<R> acc = c0<R>();
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+				cmadd(acc, D.e[3 * i + k], U.e[3 * k + j]);
+			r.e[3 * i + j] = {U.e[3 * i + j].re + acc.re, U.e[3 * i + j].im + acc.im};
+		}
+	return r;
+}
+
+// link evolution exp(X) U (src/gauge/hmc.h:26-36) as U + (exp X - 1) U
+template <class R> MD m3<R> exp_mul(m3<R> const &X, m3<R> const &U) { return add_mul(expm1su3(X), U); }
+
+// exp of a traceless antihermitian matrix, 1 + (exp X - 1)
+template <class R> MD m3<R> expsu3(m3<R> const &X)
+{
+	m3<R> E = expm1su3(X);
+	E.e[0].re += R(1);
+	E.e[4].re += R(1);
+	E.e[8].re += R(1);
 	return E;
 }
 
-// project_on_group_fast: r = 1.5 a - 0.5 a a^+ a, r *= 4/3 - det(r)/3
-template <class R> MD m3<R> reunit(m3<R> const &A)
+// project_on_group_fast (src/gauge/groups.h:237-243): r = 1.5 a - 0.5 a a^+ a,
+// then r *= 4/3 - det(r)/3, evaluated as the same quantities in increment
+// form, r = a + (1 - a a^+) a / 2 and r += r (1 - det r) / 3: the constants
+// 4/3 and 1/3 would be rounded up in FP32 (each by 3e-8 relative) and scale
+// every link up by 1.3e-8; here 1/3 only multiplies a difference of order the
+// non-unitarity.
+template <class R> MD m3<R> reunit_inc(m3<R> const &A)
 {
-	m3<R> W = mul_nd(A, A);
-	m3<R> X = mul_nn(W, A);
+	m3<R> M; // 1 - A A^+
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			cpx<R> acc = {i == j ? R(1) : R(0), R(0)};
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+			{
+				cpx<R> a = A.e[3 * i + k], b = A.e[3 * j + k];
+				acc.re = fma(-a.re, b.re, acc.re);
+				acc.re = fma(-a.im, b.im, acc.re);
+				acc.im = fma(-a.im, b.re, acc.im);
+				acc.im = fma(a.re, b.im, acc.im);
+			}
+			M.e[3 * i + j] = acc;
+		}
 	m3<R> r;
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			cpx<R> acc = c0<R>();
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+				cmadd(acc, M.e[3 * i + k], A.e[3 * k + j]);
+			r.e[3 * i + j] = {fma(R(0.5), acc.re, A.e[3 * i + j].re), fma(R(0.5), acc.im, A.e[3 * i + j].im)};
+		}
+	cpx<R> d = det(r);
+	cpx<R> c = {(R(1) - d.re) * (R(1) / R(3)), -d.im * (R(1) / R(3))};
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 	{
-		r.e[k].re = fma(R(-0.5), X.e[k].re, R(1.5) * A.e[k].re);
-		r.e[k].im = fma(R(-0.5), X.e[k].im, R(1.5) * A.e[k].im);
+		cpx<R> q = cmul(r.e[k], c);
+		r.e[k] = {r.e[k].re + q.re, r.e[k].im + q.im};
 	}
-	cpx<R> d = det(r);
-	cpx<R> c = {fma(-(R(1) / R(3)), d.re, R(4) / R(3)), -(R(1) / R(3)) * d.im};
-#pragma unroll
-	for (int k = 0; k < 9; ++k)
-		r.e[k] = cmul(r.e[k], c);
 	return r;
 }
 
@@ -385,6 +467,53 @@ template <class R> MD m3<R> from_f64(m3<double> const &a)
 #pragma unroll
 	for (int k = 0; k < 9; ++k)
 		r.e[k] = {(R)a.e[k].re, (R)a.e[k].im};
+	return r;
+}
+
+// Reunitarisation of a stored link.  FP32 links are projected in FP64 and
+// rounded once: in FP32 the projection's own roundings (1 - a a^+ is formed
+// from terms near 1) leave a mean ||U||^2/3 - 1 of -3.9e-9, against 1e-11
+// when rounded once from FP64 (both over 4e5 random links).
+template <class R> MD m3<R> reunit(m3<R> const &A)
+{
+	if constexpr (sizeof(R) == 8)
+		return reunit_inc(A);
+	else
+		return from_f64<R>(reunit_inc(to_f64(A)));
+}
+
+// momentum step P <- P - k TA(M), M = U S (src/gauge/hmc.h:57 with
+// src/gauge/utils.h:148-153): TA and the update in FP64, P rounded once.  A
+// force rounded to FP32 before the update (a function of the links alone,
+// unlike P) raises the trajectory's mean Delta H by 4.6e-8 per site against
+// the FP64 force (2mn, 12 substeps, thermalised 12^4 at beta = 6); formed in
+// FP64 from the FP32 product the difference is at the noise floor.  With AH
+// the caller's P is exactly antihermitian (the compact layout): the strict
+// lower triangle is then written as -conj of the upper one, which is the
+// value the full update would round to.
+template <bool AH, class R> MD m3<R> mom_step(m3<R> const &p, m3<R> const &M, double k)
+{
+	m3<double> A = ta(to_f64(M));
+	m3<R> r;
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = 0; j < 3; ++j)
+		{
+			const int e = 3 * i + j;
+			if (AH && j < i)
+				continue;
+			r.e[e].re = (R)fma(-k, A.e[e].re, (double)p.e[e].re);
+			r.e[e].im = (R)fma(-k, A.e[e].im, (double)p.e[e].im);
+		}
+	if constexpr (AH)
+	{
+#pragma unroll
+		for (int i = 0; i < 3; ++i)
+#pragma unroll
+			for (int j = 0; j < i; ++j)
+				r.e[3 * i + j] = {-r.e[3 * j + i].re, r.e[3 * j + i].im};
+	}
 	return r;
 }
 

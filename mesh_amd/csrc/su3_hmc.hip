@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void k_momenta_link(Geo g, R *__restrict__ P, 
 			X.e[e].re *= t;
 			X.e[e].im *= t;
 		}
-		m3<R> u = mul_nn(expsu3(X), load_once(Uin, mu, g, x));
+		m3<R> u = exp_mul(X, load_once(Uin, mu, g, x));
 		if (reunit_flag)
 			u = reunit(u);
 		store_once(Uout, mu, g, x, u);
@@ -600,7 +600,7 @@ __device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Un
 		X.e[k].re *= tl;
 		X.e[k].im *= tl;
 	}
-	m3<R> un = mul_nn(expsu3(X), u);
+	m3<R> un = exp_mul(X, u);
 	if (reunit_flag)
 		un = reunit(un);
 	store_once(Unext, mu, g, x, un);
@@ -633,7 +633,7 @@ inline int staple_path(Geo const &g)
 template <class R, bool FUSE, bool PC, int PATH>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R, PATH>()))) void k_force_update(
-    Geo g, const R *__restrict__ U, R *__restrict__ P, R kc, int nb, int remap, R *__restrict__ Unext, R tl,
+    Geo g, const R *__restrict__ U, R *__restrict__ P, double kc, int nb, int remap, R *__restrict__ Unext, R tl,
     int reunit_flag)
 {
 	int lb = ordered_block(g, nb, SB, remap);
@@ -727,14 +727,7 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 		S = MESH_STATIC_MU ? staple_any(g, U, mu, x, c) : staple_at(g, U, mu, x, c);
 		u = load(U, mu, g, x);
 	}
-	m3<R> A = ta(mul_nn(u, S));
-	m3<R> p = load_mom<PC>(P, mu, g, x);
-#pragma unroll
-	for (int k = 0; k < 9; ++k)
-	{
-		p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
-		p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
-	}
+	m3<R> p = mom_step<PC>(load_mom<PC>(P, mu, g, x), mul_nn(u, S), kc);
 	store_mom<PC>(P, mu, g, x, p);
 	if constexpr (FUSE)
 		fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
@@ -749,7 +742,7 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 // direction w; the arithmetic (and the layout of P) is that of k_force_update.
 template <class R, int L3, bool FUSE, bool PC = FUSE>
 __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, const R *__restrict__ U, R *__restrict__ P,
-                                                                     R kc, int nb, int remap, R *__restrict__ Unext,
+                                                                     double kc, int nb, int remap, R *__restrict__ Unext,
                                                                      R tl, int reunit_flag)
 {
 	constexpr int RW = 64 / L3; // rows per tile
@@ -836,14 +829,7 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 			}
 		}
 		m3<R> u = from_lds(mu, 0, 0);
-		m3<R> A = ta(mul_nn(u, S));
-		m3<R> p = load_mom<PC>(P, mu, g, x);
-#pragma unroll
-		for (int k = 0; k < 9; ++k)
-		{
-			p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
-			p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
-		}
+		m3<R> p = mom_step<PC>(load_mom<PC>(P, mu, g, x), mul_nn(u, S), kc);
 		store_mom<PC>(P, mu, g, x, p);
 		if constexpr (FUSE)
 			fused_link_step(g, Unext, mu, x, p, u, tl, reunit_flag);
@@ -872,7 +858,7 @@ __global__ __launch_bounds__(256) FORCE_OCC void k_force_update_tile(Geo g, cons
 
 // variant: one thread per site looping over the directions
 template <class R>
-__global__ __launch_bounds__(256) void k_force_update_site(Geo g, const R *__restrict__ U, R *__restrict__ P, R kc,
+__global__ __launch_bounds__(256) void k_force_update_site(Geo g, const R *__restrict__ U, R *__restrict__ P, double kc,
                                                            int nb, int remap)
 {
 	int lb = ordered_block(g, nb, 256, remap);
@@ -886,15 +872,7 @@ __global__ __launch_bounds__(256) void k_force_update_site(Geo g, const R *__res
 	for (int mu = 0; mu < g.nd; ++mu)
 	{
 		m3<R> S = staple_at(g, U, mu, x, c);
-		m3<R> A = ta(mul_nn(load(U, mu, g, x), S));
-		m3<R> p = load(P, mu, g, x);
-#pragma unroll
-		for (int k = 0; k < 9; ++k)
-		{
-			p.e[k].re = fma(-kc, A.e[k].re, p.e[k].re);
-			p.e[k].im = fma(-kc, A.e[k].im, p.e[k].im);
-		}
-		store(P, mu, g, x, p);
+		store(P, mu, g, x, mom_step<false>(load(P, mu, g, x), mul_nn(load(U, mu, g, x), S), kc));
 	}
 }
 
@@ -915,7 +893,7 @@ __global__ __launch_bounds__(256) void k_link_update(Geo g, int nf, R *Uout, con
 		X.e[k].re *= t;
 		X.e[k].im *= t;
 	}
-	m3<R> u = mul_nn(expsu3(X), load(Uin, f, g, x));
+	m3<R> u = exp_mul(X, load(Uin, f, g, x));
 	if (reunit_flag)
 		u = reunit(u);
 	store(Uout, f, g, x, u);
@@ -1090,13 +1068,13 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 			constexpr int l3v = decltype(l3)::value;
 			if (fused && compact)
 				k_force_update_tile<Rt, l3v, true, true><<<gb, 256, 0, ctx->stream>>>(
-				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
+				    geo, (const Rt *)U, (Rt *)P, k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else if (fused)
 				k_force_update_tile<Rt, l3v, true, false><<<gb, 256, 0, ctx->stream>>>(
-				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
+				    geo, (const Rt *)U, (Rt *)P, k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else
 				k_force_update_tile<Rt, l3v, false><<<gb, 256, 0, ctx->stream>>>(
-				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), nullptr, (Rt)0, 0);
+				    geo, (const Rt *)U, (Rt *)P, k, nb, mesh_order(ctx), nullptr, (Rt)0, 0);
 		};
 		using I16 = std::integral_constant<int, 16>;
 		using I32 = std::integral_constant<int, 32>;
@@ -1114,7 +1092,7 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 			k_force_update_site<double><<<gb, 256, 0, ctx->stream>>>(geo, (const double *)U, (double *)P, k, nb,
 			                                                          ctx->xcd_remap);
 		else
-			k_force_update_site<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, (float)k,
+			k_force_update_site<float><<<gb, 256, 0, ctx->stream>>>(geo, (const float *)U, (float *)P, k,
 			                                                         nb, ctx->xcd_remap);
 		fused = false;
 	}
@@ -1127,13 +1105,13 @@ int launch_force_update(mesh_ctx *ctx, Geo const &geo, int prec, const void *U, 
 			constexpr int PT = decltype(path)::value;
 			if (fused && compact)
 				k_force_update<Rt, true, true, PT><<<gb, 256, 0, ctx->stream>>>(
-				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
+				    geo, (const Rt *)U, (Rt *)P, k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else if (fused)
 				k_force_update<Rt, true, false, PT><<<gb, 256, 0, ctx->stream>>>(
-				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
+				    geo, (const Rt *)U, (Rt *)P, k, nb, mesh_order(ctx), (Rt *)Unext, (Rt)tl, reunit_flag);
 			else
 				k_force_update<Rt, false, false, PT><<<gb, 256, 0, ctx->stream>>>(
-				    geo, (const Rt *)U, (Rt *)P, (Rt)k, nb, mesh_order(ctx), nullptr, (Rt)0, 0);
+				    geo, (const Rt *)U, (Rt *)P, k, nb, mesh_order(ctx), nullptr, (Rt)0, 0);
 		};
 		auto by_path = [&](auto tag) {
 			switch (staple_path(geo))
@@ -1637,3 +1615,4 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 }
 
 } // extern "C"
+

@@ -179,6 +179,8 @@ __global__ __launch_bounds__(U1_THREADS) void k_u1_hmc(U1Geo g, float *th, const
 		double k = (double)q[x] * q[x] + (double)q[g.V + x] * q[g.V + x];
 		eold += beta * (1.0 - cos((double)plaq(g, t, x))) + k;
 	}
+	// every wave has read the start angles before the first link step moves them
+	__syncthreads();
 	for (int s = 0; s < nd; ++s)
 	{
 		const float d = (float)deltas[s];

@@ -175,7 +175,7 @@ template <class T> void momenta(T *P, double *K, Geo const &g, uint64_t seed, ui
 
 template <class T> void force_update(T const *U, T *P, Geo const &g, double beta, double delta)
 {
-	T k = (T)(delta * (beta / 6.0));
+	double k = delta * (beta / 6.0);
 #pragma omp parallel for schedule(static)
 	for (int64_t x = 0; x < g.V; ++x)
 		for (int mu = 0; mu < g.nd; ++mu)

@@ -260,14 +260,16 @@ template <class T> constexpr int exp_terms() { return sizeof(T) == 8 ? 18 : 10; 
 template <class T> constexpr int exp_terms_small() { return sizeof(T) == 8 ? 11 : 6; }
 constexpr double exp_small_r2 = 0.04;
 
-// exp(X) for traceless antihermitian X.
+// exp(X) - 1 for traceless antihermitian X.
 // Cayley-Hamilton: X^3 = a X + b 1 with a = tr(X^2)/2, b = det X, so that
 // X^n = alpha_n 1 + beta_n X + gamma_n X^2 with
 //   alpha_{n+1} = gamma_n b, beta_{n+1} = alpha_n + gamma_n a, gamma_{n+1} = beta_n
 // and exp X = f0 1 + f1 X + f2 X^2, f_k = sum_n (coefficient_n)/n!.  For
 // ||X||_F > 1 the argument is first divided by 2^s (exact) and the result
-// squared s times; the (scaled) ||X||_F^2 sets the number of terms.
-template <class T> inline mat3<T> expsu3(mat3<T> X)
+// squared s times (D -> 2 D + D^2); the (scaled) ||X||_F^2 sets the number of
+// terms.  The 1 of exp X is never added: f0 - 1 is summed from its small terms
+// (g0 below), so every entry of D is rounded at its own magnitude.
+template <class T> inline mat3<T> expm1su3(mat3<T> X)
 {
 	mat3<T> X2 = mul_nn(X, X);
 	T trx2 = (X2.e[0][0].re + X2.e[1][1].re) + X2.e[2][2].re;
@@ -296,7 +298,7 @@ template <class T> inline mat3<T> expsu3(mat3<T> X)
 	T a = T(0.5) * trx2;
 	cplx<T> b = det3(X);
 	cplx<T> al = {T(1), T(0)}, be = {T(0), T(0)}, ga = {T(0), T(0)};
-	cplx<T> f0 = {T(1), T(0)}, f1 = {T(0), T(0)}, f2 = {T(0), T(0)};
+	cplx<T> g0 = {T(0), T(0)}, f1 = {T(0), T(0)}, f2 = {T(0), T(0)};
 	const int nterms = r2 > T(exp_small_r2) ? exp_terms<T>() : exp_terms_small<T>();
 	for (int n = 0; n < nterms; ++n)
 	{
@@ -309,48 +311,122 @@ template <class T> inline mat3<T> expsu3(mat3<T> X)
 		al = nal;
 		be = nbe;
 		ga = nga;
-		f0.re += al.re;
-		f0.im += al.im;
+		g0.re += al.re;
+		g0.im += al.im;
 		f1.re += be.re;
 		f1.im += be.im;
 		f2.re += ga.re;
 		f2.im += ga.im;
 	}
-	mat3<T> E;
+	mat3<T> D;
 	for (int i = 0; i < 3; ++i)
 		for (int j = 0; j < 3; ++j)
 		{
-			cplx<T> acc = (i == j) ? f0 : cplx<T>{T(0), T(0)};
+			cplx<T> acc = (i == j) ? g0 : cplx<T>{T(0), T(0)};
 			cmadd(acc, f1, X.e[i][j]);
 			cmadd(acc, f2, X2.e[i][j]);
-			E.e[i][j] = acc;
+			D.e[i][j] = acc;
 		}
 	for (int k = 0; k < s; ++k)
-		E = mul_nn(E, E);
+	{
+		mat3<T> N;
+		for (int i = 0; i < 3; ++i)
+			for (int j = 0; j < 3; ++j)
+			{
+				cplx<T> acc = {D.e[i][j].re + D.e[i][j].re, D.e[i][j].im + D.e[i][j].im};
+				for (int l = 0; l < 3; ++l)
+					cmadd(acc, D.e[i][l], D.e[l][j]);
+				N.e[i][j] = acc;
+			}
+		D = N;
+	}
+	return D;
+}
+
+// exp(X) = 1 + (exp X - 1)
+template <class T> inline mat3<T> expsu3(mat3<T> const &X)
+{
+	mat3<T> E = expm1su3(X);
+	for (int i = 0; i < 3; ++i)
+		E.e[i][i].re += T(1);
 	return E;
 }
 
-// project_on_group_fast: r = 1.5 a - 0.5 a a^+ a; r *= (4/3 - det(r)/3)
-template <class T> inline mat3<T> reunit3(mat3<T> const &A)
+// U + D U: the product summed from zero, then one rounding at U's magnitude
+template <class T> inline mat3<T> add_mul(mat3<T> const &D, mat3<T> const &U)
 {
-	mat3<T> W = mul_nd(A, A);
-	mat3<T> X = mul_nn(W, A);
 	mat3<T> r;
 	for (int i = 0; i < 3; ++i)
 		for (int j = 0; j < 3; ++j)
 		{
-			r.e[i][j].re = std::fma(T(-0.5), X.e[i][j].re, T(1.5) * A.e[i][j].re);
-			r.e[i][j].im = std::fma(T(-0.5), X.e[i][j].im, T(1.5) * A.e[i][j].im);
+			cplx<T> acc = {T(0), T(0)};
+			for (int k = 0; k < 3; ++k)
+				cmadd(acc, D.e[i][k], U.e[k][j]);
+			r.e[i][j] = {U.e[i][j].re + acc.re, U.e[i][j].im + acc.im};
 		}
-	cplx<T> d = det3(r);
-	cplx<T> c = {std::fma(-(T(1) / T(3)), d.re, T(4) / T(3)), -(T(1) / T(3)) * d.im};
-	for (int i = 0; i < 3; ++i)
-		for (int j = 0; j < 3; ++j)
-			r.e[i][j] = cmul(r.e[i][j], c);
 	return r;
 }
 
-// link update U <- exp(t * TA(P)) * U
+template <class A, class B> inline mat3<B> convert3(mat3<A> const &a)
+{
+	mat3<B> r;
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j)
+			r.e[i][j] = {(B)a.e[i][j].re, (B)a.e[i][j].im};
+	return r;
+}
+
+// project_on_group_fast (src/gauge/groups.h:237-243): r = 1.5 a - 0.5 a a^+ a
+// and r *= 4/3 - det(r)/3, evaluated in increment form:
+// r = a + (1 - a a^+) a / 2, then r += r (1 - det r) / 3.
+template <class T> inline mat3<T> reunit_inc(mat3<T> const &A)
+{
+	mat3<T> M;
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j)
+		{
+			cplx<T> acc = {i == j ? T(1) : T(0), T(0)};
+			for (int k = 0; k < 3; ++k)
+			{
+				cplx<T> a = A.e[i][k], b = A.e[j][k];
+				acc.re = std::fma(-a.re, b.re, acc.re);
+				acc.re = std::fma(-a.im, b.im, acc.re);
+				acc.im = std::fma(-a.im, b.re, acc.im);
+				acc.im = std::fma(a.re, b.im, acc.im);
+			}
+			M.e[i][j] = acc;
+		}
+	mat3<T> r;
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j)
+		{
+			cplx<T> acc = {T(0), T(0)};
+			for (int k = 0; k < 3; ++k)
+				cmadd(acc, M.e[i][k], A.e[k][j]);
+			r.e[i][j] = {std::fma(T(0.5), acc.re, A.e[i][j].re), std::fma(T(0.5), acc.im, A.e[i][j].im)};
+		}
+	cplx<T> d = det3(r);
+	cplx<T> c = {(T(1) - d.re) * (T(1) / T(3)), -d.im * (T(1) / T(3))};
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j)
+		{
+			cplx<T> q = cmul(r.e[i][j], c);
+			r.e[i][j] = {r.e[i][j].re + q.re, r.e[i][j].im + q.im};
+		}
+	return r;
+}
+
+// reunitarisation of a stored link: FP32 links are projected in FP64 and
+// rounded once
+template <class T> inline mat3<T> reunit3(mat3<T> const &A)
+{
+	if constexpr (sizeof(T) == 8)
+		return reunit_inc(A);
+	else
+		return convert3<double, T>(reunit_inc(convert3<T, double>(A)));
+}
+
+// link update U <- exp(t * TA(P)) * U, as U + (exp(t TA(P)) - 1) U
 template <class T> inline mat3<T> evolve_link(mat3<T> const &U, mat3<T> const &P, T t)
 {
 	mat3<T> X = ta3(P);
@@ -360,20 +436,21 @@ template <class T> inline mat3<T> evolve_link(mat3<T> const &U, mat3<T> const &P
 			X.e[i][j].re *= t;
 			X.e[i][j].im *= t;
 		}
-	return mul_nn(expsu3(X), U);
+	return add_mul(expm1su3(X), U);
 }
 
-// momentum update P <- P - k * TA(U * S)
+// momentum update P <- P - k * TA(U * S): the product in T, TA and the update
+// in FP64, P rounded once
 template <class T> inline mat3<T> force_link_update(mat3<T> const &P, mat3<T> const &U,
-                                                   mat3<T> const &S, T k)
+                                                   mat3<T> const &S, double k)
 {
-	mat3<T> A = ta3(mul_nn(U, S));
+	mat3<double> A = ta3(convert3<T, double>(mul_nn(U, S)));
 	mat3<T> r;
 	for (int i = 0; i < 3; ++i)
 		for (int j = 0; j < 3; ++j)
 		{
-			r.e[i][j].re = std::fma(-k, A.e[i][j].re, P.e[i][j].re);
-			r.e[i][j].im = std::fma(-k, A.e[i][j].im, P.e[i][j].im);
+			r.e[i][j].re = (T)std::fma(-k, A.e[i][j].re, (double)P.e[i][j].re);
+			r.e[i][j].im = (T)std::fma(-k, A.e[i][j].im, (double)P.e[i][j].im);
 		}
 	return r;
 }

@@ -333,6 +333,118 @@ def test_f32_vs_f64_same_config(ctx):
     assert abs(res[F32][0]["dH"] - res[F64][0]["dH"]) < 1e-2 * (1 + abs(res[F64][0]["dH"]))
 
 
+def _cold_thermalised(ctx, shape, ntherm, seed=2):
+    """FP64 chain from the unit field, ntherm trajectories without Metropolis
+    (beta 6, 2mn, 12 substeps)"""
+    g = mesh_amd.Grid(shape)
+    h = gauge.Hmc(g, F64, seed=seed, ctx=ctx)
+    h.U.fill_unit()
+    d = gauge.makeDeltas("2mn", 0.5, 12)
+    for _ in range(ntherm):
+        h.runHmcUpdate(6.0, d, metropolis=False)
+    return g, h.U, d
+
+
+@pytest.mark.parametrize("L,draws", [(16, 6), (32, 2)])
+def test_f32_vs_f64_delta_h_thermalised(ctx, L, draws):
+    """FP32-mixed trajectories against FP64 ones from the same thermalised
+    configuration and the same momenta (beta 6, 2mn, 12 substeps): Delta H
+    differs by rounding noise only, with no term proportional to the volume
+    (the earlier FP32 link step and reunitarisation gave -1.0e-6 per site,
+    -1.02 at 32^4; a force rounded to FP32 before the momentum update gives
+    +4.6e-8 per site), and the plaquettes after the trajectory agree to
+    better than 1e-6."""
+    shape = [L] * 4
+    g, U, d = _cold_thermalised(ctx, shape, 40)
+    V = g.size()
+    diffs = []
+    for k in range(draws):
+        res = {}
+        for prec in (F64, F32):
+            h = gauge.Hmc(g, prec, seed=100 + k, ctx=ctx)
+            h.U = U.convert(prec)
+            r = h.runHmcUpdate(6.0, d, metropolis=False)
+            res[prec] = (r["dH"], r["plaq"])
+        print(L, k, res[F64], res[F32][0] - res[F64][0], flush=True)
+        assert abs(res[F32][1] - res[F64][1]) < 1e-6
+        diffs.append(res[F32][0] - res[F64][0])
+    assert abs(np.mean(diffs)) / V < 1e-8, diffs
+
+
+def test_ensemble_f32_vs_f64_16(ctx):
+    """Thermalised 16^4 at beta 6 (2mn, 12 substeps), 300 Metropolis
+    trajectories in each precision: acceptance, <exp(-dH)> (= 1, Creutz;
+    src/gauge/hmc.h:98-99), <dH> and the mean plaquette of the FP32-mixed
+    chain agree with the FP64 chain's."""
+    import math
+
+    g, U, d = _cold_thermalised(ctx, [16] * 4, 40)
+    res = {}
+    for prec in (F64, F32):
+        h = gauge.Hmc(g, prec, seed=7, ctx=ctx)
+        h.U = U.convert(prec)
+        rows = [h.runHmcUpdate(6.0, d) for _ in range(300)]
+        res[prec] = dict(plaq=[r["plaq"] for r in rows], acc=[float(r["accepted"]) for r in rows],
+                         dH=np.array([r["dH"] for r in rows]))
+    a64, a32 = np.mean(res[F64]["acc"]), np.mean(res[F32]["acc"])
+    ea = math.sqrt(a64 * (1 - a64) / 300 + a32 * (1 - a32) / 300) + 1e-3
+    print("acceptance", a64, a32, "dH", res[F64]["dH"].mean(), res[F32]["dH"].mean(), flush=True)
+    assert abs(a64 - a32) < 4 * ea, (a64, a32)
+    for prec in (F64, F32):
+        c, ec = _binned(np.exp(-res[prec]["dH"]))
+        print("creutz", prec, c, ec, flush=True)
+        assert abs(c - 1) < 5 * ec + 0.01, (prec, c, ec)
+    m64, e64 = _binned(res[F64]["dH"])
+    m32, e32 = _binned(res[F32]["dH"])
+    assert abs(m64 - m32) < 5 * math.hypot(e64, e32) + 0.01, (m64, m32)
+    p64, ep64 = _binned(res[F64]["plaq"])
+    p32, ep32 = _binned(res[F32]["plaq"])
+    assert abs(p64 - p32) < 5 * math.hypot(ep64, ep32) + 1e-5, (p64, p32)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_c2_fused_trajectory_32(ctx, prec):
+    """The benchmarked path at full size: one 2mn, 12-substep trajectory of
+    the default fused kernels with compact momenta (k_momenta_link, 24 x the
+    fused momentum + link step, the last with reunitarisation, Delta H) from
+    the 32^4 configuration thermalised by 50 trajectories, against the
+    oracle: the new field bit for bit, Delta H to 1e-12 relative."""
+    shape = [32, 32, 32, 32]
+    g = mesh_amd.Grid(shape)
+    hmc = gauge.Hmc(g, F64, seed=2, ctx=ctx)
+    hmc.randomizeGaugeField()
+    d = gauge.makeDeltas("2mn", 0.5, 12)
+    for _ in range(50):
+        hmc.runHmcUpdate(6.0, d, metropolis=False)
+    h = gauge.Hmc(g, prec, seed=2, ctx=ctx)
+    h.U = hmc.U if prec == F64 else hmc.U.convert(prec)
+    h.traj = 50
+    U = h.U.download()
+    r = h.runHmcUpdate(6.0, d, metropolis=False)
+    o = O.hmc_trajectory(U, shape, 6.0, d, 2, 50, metropolis=False)
+    np.testing.assert_array_equal(h.U.download(), U)
+    assert abs(r["dH"] - o["dH"]) <= 1e-12 * max(1.0, abs(o["dH"])), (r["dH"], o["dH"])
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_fused_step_48(ctx, prec):
+    """48^4 (the larger size of the mixed-precision workload; the fastest
+    extent does not divide 64, so the own-links staple path): a trajectory
+    of one leapfrog substep runs the momenta with the first link step and one
+    fused momentum + link step with reunitarisation; the field bit for bit
+    and Delta H to 1e-12 relative against the oracle."""
+    shape = [48, 48, 48, 48]
+    g = mesh_amd.Grid(shape)
+    h = gauge.Hmc(g, prec, seed=5, ctx=ctx)
+    h.randomizeGaugeField()
+    d = gauge.makeDeltas("2lf", 0.02, 1)
+    U = h.U.download()
+    r = h.runHmcUpdate(6.0, d, metropolis=False)
+    o = O.hmc_trajectory(U, shape, 6.0, d, 5, 0, metropolis=False)
+    np.testing.assert_array_equal(h.U.download(), U)
+    assert abs(r["dH"] - o["dH"]) <= 1e-12 * max(1.0, abs(o["dH"])), (r["dH"], o["dH"])
+
+
 def test_errors(ctx):
     g = mesh_amd.Grid((4, 4, 4, 4))
     U = mesh_amd.GaugeField(g, F64, ctx)
@@ -378,15 +490,16 @@ def test_c1_workload_matches_oracle(ctx):
 
 @pytest.mark.parametrize("prec", [F64, F32])
 def test_c2_force_parity_32(ctx, prec):
-    """Full-size force parity: the fused momentum step on a 32^4 configuration
-    thermalised by 20 HMC trajectories, against the oracle element by
-    element (bit-exact; the required bound is 1e-12 relative)."""
+    """Full-size force parity: the momentum step on the 32^4 configuration
+    thermalised by 50 HMC trajectories from the seed-2 Haar start, against
+    the oracle element by element (bit-exact; the required bound is 1e-12
+    relative)."""
     shape = [32, 32, 32, 32]
     g = mesh_amd.Grid(shape)
     hmc = gauge.Hmc(g, F64, seed=2, ctx=ctx)
     hmc.randomizeGaugeField()
     d = gauge.makeDeltas("2mn", 0.5, 12)
-    for _ in range(20):
+    for _ in range(50):
         hmc.runHmcUpdate(6.0, d, metropolis=False)
     U = hmc.U if prec == F64 else hmc.U.convert(prec)
     P = mesh_amd.GaugeField(g, prec, ctx)

@@ -456,10 +456,13 @@ def test_u1_nd_hmc_against_oracle(ctx, dims):
         assert o["Q"][c] == 0.0
 
 
-def test_u1_general_path_matches_lds_path_2d(ctx):
+@pytest.mark.parametrize("L0,L1,B", [(8, 12, 6), (32, 32, 8), (64, 64, 4)])
+def test_u1_general_path_matches_lds_path_2d(ctx, L0, L1, B):
     """On a 2-d lattice the site-parallel kernels give the LDS kernels' bits:
-    the same fields after three Metropolis trajectories, the same decisions."""
-    L0, L1, B = 8, 12, 6
+    the same fields after three Metropolis trajectories, the same decisions.
+    At 32^2 and 64^2 each of the LDS kernel's four waves handles many sites,
+    so a wave that starts moving angles before the others have summed their
+    start energy would change Delta H and the decisions."""
     d = gauge.makeDeltas("2mn", 0.5, 4)
     res = {}
     for general in (0, 1):

@@ -1,0 +1,73 @@
+"""FP32-mixed arithmetic of the CPU oracle, which the GPU's FP32 path
+reproduces bit for bit: link steps that keep the norm of the links, an
+unbiased reunitarisation, and a Delta H of FP32 trajectories that tracks the
+FP64 one on a thermalised configuration with no term proportional to the
+volume.
+
+Exact arithmetic fixes each target: exp(X) U and the projection onto SU(3)
+leave ||U||_F^2 = 3, and FP32 trajectories from the same configuration and
+momenta differ from FP64 ones by rounding noise only.  The sizes of the
+systematic errors the checks exclude (measured with the earlier arithmetic
+on the same inputs) are given at each assertion."""
+import numpy as np
+
+from oracle import oracle as O
+
+
+def _norm_dev(U):
+    """mean over links of ||U||_F^2 / 3 - 1, in float64"""
+    u = U.astype(np.complex128)
+    return float(np.mean(np.sum(np.abs(u) ** 2, axis=(-2, -1)) / 3.0 - 1.0))
+
+
+def test_fp32_link_steps_keep_the_norm():
+    """20 link steps t = 1/12 on 16384 FP32 links: exp(X) rounded to FP32
+    before the product shrank ||U||^2/3 by 1.4e-9 per step (-2.8e-8 here);
+    U + (exp X - 1) U keeps it within the rounding noise (~2e-9)."""
+    dims = [8, 8, 8, 8]
+    U = O.random_gauge(dims, 5, "f32")
+    P, _ = O.momenta(dims, 6, 0, "f32")
+    n0 = _norm_dev(U)
+    for _ in range(20):
+        U = O.link_update(U, P, dims, 1.0 / 12)
+    assert abs(_norm_dev(U) - n0) < 6e-9
+
+
+def test_fp32_reunit_is_unbiased():
+    """Reunitarising FP32 links: with 4/3 and 1/3 rounded to FP32 the result
+    had ||U||^2/3 - 1 = +2.6e-8 on average; rounded once from the FP64
+    projection it is unbiased (noise ~3e-10 over 65536 links)."""
+    dims = [8, 8, 8, 8]
+    U = O.random_gauge(dims, 7, "f32").astype(np.complex128)
+    rng = np.random.default_rng(8)
+    U = U + 1e-5 * (rng.normal(size=U.shape) + 1j * rng.normal(size=U.shape))
+    R = O.reunit_field(U.astype(np.complex64))
+    assert abs(_norm_dev(R)) < 2e-9
+    # and the projection itself is second order in the distance from SU(3)
+    r64 = R.astype(np.complex128).reshape(-1, 3, 3)
+    dev = np.abs(r64 @ r64.conj().transpose(0, 2, 1) - np.eye(3)).max()
+    assert dev < 5e-7
+
+
+def test_fp32_delta_h_tracks_fp64_on_a_thermalised_field():
+    """8^4, beta = 6, 2mn with 12 substeps, thermalised from a cold start; six
+    momentum draws.  FP32 minus FP64 Delta H per site, averaged over the
+    draws: -1.0e-6 with the earlier link step and reunitarisation, +4.4e-8
+    with the force rounded to FP32 before the momentum update; now -3e-10
+    (single draws scatter by 2e-8 per site at this volume)."""
+    dims = [8, 8, 8, 8]
+    V = O.volume(dims)
+    d = O.make_deltas("2mn", 0.5, 12)
+    U = np.zeros((4, V, 3, 3), np.complex128)
+    U[:, :] = np.eye(3)
+    for t in range(40):
+        O.hmc_trajectory(U, dims, 6.0, d, 3, t, metropolis=False)
+    assert 0.58 < O.plaquette(U, dims) < 0.61
+    U32 = U.astype(np.complex64)
+    diffs = []
+    for k in range(6):
+        r64 = O.hmc_trajectory(U.copy(), dims, 6.0, d, 3, 1000 + k, metropolis=False)
+        r32 = O.hmc_trajectory(U32.copy(), dims, 6.0, d, 3, 1000 + k, metropolis=False)
+        diffs.append(r32["dH"] - r64["dH"])
+    per_site = np.mean(diffs) / V
+    assert abs(per_site) < 2.5e-8, diffs
