@@ -183,6 +183,17 @@ int mesh_su3_momenta(mesh_ctx *ctx, const mesh_grid *g, int prec, void *P, doubl
 /* *out = normalised average plaquette in [0,1]: plaquette(U)
  * (src/gauge/utils.h:93-108). */
 int mesh_su3_plaquette(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double *out);
+/* The value of mesh_su3_plaquette, bit for bit, written to device memory
+ * (out_dev: one double) by an entry point that allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The per-block partial sums live in a fixed
+ * 512 KiB device array, one per GPU, held until the process ends: lattices
+ * up to 2^24 sites (64^4); larger ones return MESH_E_SIZE.  Asynchronous on
+ * the context's stream (no copy to the host, no synchronisation).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (from
+ * several contexts or streams) race. */
+int mesh_su3_plaquette_noalloc(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double *out_dev);
 /* *out = Wilson action beta * sum_P (1 - Re tr P / 3): wilsonAction
  * (src/gauge/utils.h:141-146). */
 int mesh_su3_wilson_action(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double beta,

@@ -61,6 +61,7 @@ _SIGS = {
     "mesh_su3_random_gauge": (i32, [vp, gp, i32, i32, vp, u64, u32]),
     "mesh_su3_momenta": (i32, [vp, gp, i32, vp, vp, u64, u32]),
     "mesh_su3_plaquette": (i32, [vp, gp, i32, vp, dp]),
+    "mesh_su3_plaquette_noalloc": (i32, [vp, gp, i32, vp, vp]),
     "mesh_su3_wilson_action": (i32, [vp, gp, i32, vp, f64, dp]),
     "mesh_su3_norm2": (i32, [vp, gp, i32, i32, vp, dp]),
     "mesh_su3_sum_real_trace_prod": (i32, [vp, gp, i32, i32, vp, vp, dp]),

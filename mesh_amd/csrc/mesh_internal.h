@@ -51,6 +51,9 @@ struct mesh_ctx
 	int u1_general = 0;       // U(1): site-parallel kernels also for small 2-d lattices
 	void *u1_ws = nullptr;    // workspace of the site-parallel U(1) kernels
 	size_t u1_ws_bytes = 0;
+	// device address of the static scratch array of the no-allocation entry
+	// point on this context's device (mesh_scratch_setup)
+	double *scratch = nullptr;
 	// profiler: events around launches, resolved after a stream sync
 	int prof_on = 0;
 	std::vector<hipEvent_t> prof_pool;
@@ -106,6 +109,12 @@ int mesh_stage_doubles(mesh_ctx *ctx, const double *host, size_t n);
 int mesh_launch_adam(mesh_ctx *ctx, int n, float *params, const float *grad, float *m, float *v, int t, double lr);
 // sums nq quantities of nb partials each into host out[nq] (synchronises)
 int mesh_finish_reduction(mesh_ctx *ctx, int nb, int nq, double *out);
+// the same fixed-order sum of nb partials (nq = 1) into device out_dev, on the
+// context's stream, without copying or synchronising
+int mesh_reduce_to_device(mesh_ctx *ctx, const double *partials, int nb, double *out_dev);
+// resolves the device address of the no-allocation scratch array for the
+// context's device (once per device; the device must be current)
+int mesh_scratch_setup(mesh_ctx *ctx);
 
 // launch shape of the site-blocked kernels: nb logical blocks; with XCD
 // remapping the grid is padded to a multiple of 8

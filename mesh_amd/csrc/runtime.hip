@@ -201,6 +201,13 @@ int mesh_finish_reduction(mesh_ctx *ctx, int nb, int nq, double *out)
 	return MESH_OK;
 }
 
+int mesh_reduce_to_device(mesh_ctx *ctx, const double *partials, int nb, double *out_dev)
+{
+	k_reduce_partials<1><<<1, 1024, 0, ctx->stream>>>(partials, nb, out_dev);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
 extern "C" {
 
 int mesh_ctx_create(mesh_ctx **out, int device)
@@ -231,6 +238,14 @@ int mesh_ctx_create(mesh_ctx **out, int device)
 		(void)hipStreamDestroy(ctx->stream);
 		delete ctx;
 		return MESH_E_ALLOC;
+	}
+	if (mesh_scratch_setup(ctx) != MESH_OK)
+	{
+		(void)hipFree(ctx->red_dev);
+		(void)hipHostFree(ctx->red_host);
+		(void)hipStreamDestroy(ctx->stream);
+		delete ctx;
+		return MESH_E_HIP;
 	}
 	*out = ctx;
 	return MESH_OK;

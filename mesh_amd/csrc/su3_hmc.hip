@@ -13,6 +13,7 @@
 #include "rng_dev.h"
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1616,3 +1617,76 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 
 } // extern "C"
 
+// ---- plaquette without allocation -------------------------------------------
+//
+// mesh_su3_plaquette_noalloc keeps its per-block partial sums in this
+// file-scope array instead of the context's buffer (which grows with
+// hipMalloc).  In host code the array's name is not a device address:
+// mesh_scratch_setup resolves the address once per device with
+// hipGetSymbolAddress, and launches take that pointer.  One partial per 256
+// sites: 65536 doubles (512 KiB per GPU, held until the process ends) cover
+// lattices up to 2^24 sites (64^4).
+constexpr int NOALLOC_PARTIALS = 65536;
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+__global__ void k_div1(double *o, double d)
+{
+	if (threadIdx.x == 0)
+		o[0] = o[0] / d;
+}
+
+} // namespace
+
+int mesh_scratch_setup(mesh_ctx *ctx)
+{
+	std::lock_guard<std::mutex> lock(scratch_mu);
+	if (ctx->device < 0 || ctx->device >= SCRATCH_MAX_DEVICES)
+		return mesh_fail(ctx, MESH_E_ARG, "device index beyond the scratch table");
+	if (!scratch_dev[ctx->device])
+	{
+		void *p = nullptr;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			return mesh_hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+		scratch_dev[ctx->device] = static_cast<double *>(p);
+	}
+	ctx->scratch = scratch_dev[ctx->device];
+	return MESH_OK;
+}
+
+extern "C" {
+
+int mesh_su3_plaquette_noalloc(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double *out_dev)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, prec);
+	if (s.rc)
+		return s.rc;
+	const int nb = nblk(s.geo.V, 256);
+	if (nb > NOALLOC_PARTIALS)
+		return mesh_fail(ctx, MESH_E_SIZE,
+		                 "plaquette_noalloc: lattice exceeds 2^24 sites (the fixed scratch holds 65536 block sums)");
+	if (!U || !out_dev)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to plaquette_noalloc");
+	if (!ctx->scratch)
+		return mesh_fail(ctx, MESH_E_STATE, "plaquette_noalloc: scratch address not resolved");
+	const int gb = mesh_grid_blocks(ctx, nb);
+	if (prec == MESH_F64)
+		k_plaq<double><<<gb, 256, 0, ctx->stream>>>(s.geo, (const double *)U, ctx->scratch, nb, mesh_order(ctx));
+	else
+		k_plaq<float><<<gb, 256, 0, ctx->stream>>>(s.geo, (const float *)U, ctx->scratch, nb, mesh_order(ctx));
+	MESH_LAUNCHED(ctx);
+	if (int rc = mesh_reduce_to_device(ctx, ctx->scratch, nb, out_dev))
+		return rc;
+	k_div1<<<1, 64, 0, ctx->stream>>>(out_dev, n_plaquettes(s.geo) * 3.0);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+} // extern "C"
