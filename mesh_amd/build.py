@@ -62,6 +62,9 @@ def build(verbose=False, extra=(), out=LIB, tag="default"):
 
 APPS = {"mesh_hmc": "hmc.cpp", "mesh_landau": "landau.cpp", "mesh_sweep_beta": "sweep_beta.cpp",
         "test_lattice_gauge": "test_lattice_gauge.cpp", "test_io": "test_io.cpp"}
+# programs with device code of their own (the generic lattice operations of
+# mesh/lattice_apply.h), compiled by hipcc
+HIP_APPS = {"test_lattice_apply": "test_lattice_apply.hip"}
 
 
 def build_apps(verbose=False):
@@ -77,6 +80,17 @@ def build_apps(verbose=False):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
+    for exe, src in HIP_APPS.items():
+        cmd = [HIPCC, *FLAGS, "-std=c++20", f"-I{root}/include", f"-I{HERE}/host", os.path.join(HERE, "apps", src),
+               f"-L{HERE}", "-lmesh_hip", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(bindir, exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stderr=subprocess.PIPE, text=True)
+        err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+        if err.strip():
+            print(err, file=sys.stderr)
+        if r.returncode != 0:
+            raise subprocess.CalledProcessError(r.returncode, cmd)
 
 
 if __name__ == "__main__":

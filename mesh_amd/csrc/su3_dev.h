@@ -10,11 +10,15 @@
 // Reference semantics: src/gauge/groups.h:198-268 (adj, real_trace,
 // project_on_algebra, project_on_group_fast, exp).
 #pragma once
+#include <cmath>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <utility>
 
 namespace mesh_dev {
+
+// float and double overloads of fma also on the host side of hipcc
+using std::fma;
 
 template <class R> struct vec2;
 template <> struct vec2<double>
@@ -38,11 +42,14 @@ template <class R> struct m3
 };
 
 #define MD __device__ __forceinline__
+// the element algebra also compiles for the host side of a hipcc translation
+// unit (the generic lattice operations evaluate the same lambdas on the host)
+#define MHD __host__ __device__ __forceinline__
 
-template <class R> MD cpx<R> c0() { return {R(0), R(0)}; }
+template <class R> MHD cpx<R> c0() { return {R(0), R(0)}; }
 
 // acc += a*b
-template <class R> MD void cmadd(cpx<R> &acc, cpx<R> a, cpx<R> b)
+template <class R> MHD void cmadd(cpx<R> &acc, cpx<R> a, cpx<R> b)
 {
 	acc.re = fma(a.re, b.re, acc.re);
 	acc.re = fma(-a.im, b.im, acc.re);
@@ -50,7 +57,7 @@ template <class R> MD void cmadd(cpx<R> &acc, cpx<R> a, cpx<R> b)
 	acc.im = fma(a.im, b.re, acc.im);
 }
 // acc -= a*b
-template <class R> MD void cmsub(cpx<R> &acc, cpx<R> a, cpx<R> b)
+template <class R> MHD void cmsub(cpx<R> &acc, cpx<R> a, cpx<R> b)
 {
 	acc.re = fma(-a.re, b.re, acc.re);
 	acc.re = fma(a.im, b.im, acc.re);
@@ -58,7 +65,7 @@ template <class R> MD void cmsub(cpx<R> &acc, cpx<R> a, cpx<R> b)
 	acc.im = fma(-a.im, b.re, acc.im);
 }
 // acc += a*conj(b)
-template <class R> MD void cmadd_cb(cpx<R> &acc, cpx<R> a, cpx<R> b)
+template <class R> MHD void cmadd_cb(cpx<R> &acc, cpx<R> a, cpx<R> b)
 {
 	acc.re = fma(a.re, b.re, acc.re);
 	acc.re = fma(a.im, b.im, acc.re);
@@ -66,7 +73,7 @@ template <class R> MD void cmadd_cb(cpx<R> &acc, cpx<R> a, cpx<R> b)
 	acc.im = fma(-a.re, b.im, acc.im);
 }
 // acc += conj(a)*b
-template <class R> MD void cmadd_ca(cpx<R> &acc, cpx<R> a, cpx<R> b)
+template <class R> MHD void cmadd_ca(cpx<R> &acc, cpx<R> a, cpx<R> b)
 {
 	acc.re = fma(a.re, b.re, acc.re);
 	acc.re = fma(a.im, b.im, acc.re);
@@ -74,14 +81,14 @@ template <class R> MD void cmadd_ca(cpx<R> &acc, cpx<R> a, cpx<R> b)
 	acc.im = fma(-a.im, b.re, acc.im);
 }
 // acc += conj(a)*conj(b)
-template <class R> MD void cmadd_cc(cpx<R> &acc, cpx<R> a, cpx<R> b)
+template <class R> MHD void cmadd_cc(cpx<R> &acc, cpx<R> a, cpx<R> b)
 {
 	acc.re = fma(a.re, b.re, acc.re);
 	acc.re = fma(-a.im, b.im, acc.re);
 	acc.im = fma(-a.re, b.im, acc.im);
 	acc.im = fma(-a.im, b.re, acc.im);
 }
-template <class R> MD cpx<R> cmul(cpx<R> a, cpx<R> b)
+template <class R> MHD cpx<R> cmul(cpx<R> a, cpx<R> b)
 {
 	cpx<R> r = c0<R>();
 	cmadd(r, a, b);
@@ -89,7 +96,7 @@ template <class R> MD cpx<R> cmul(cpx<R> a, cpx<R> b)
 }
 
 // C = A B
-template <class R> MD m3<R> mul_nn(m3<R> const &A, m3<R> const &B)
+template <class R> MHD m3<R> mul_nn(m3<R> const &A, m3<R> const &B)
 {
 	m3<R> C;
 #pragma unroll
@@ -106,7 +113,7 @@ template <class R> MD m3<R> mul_nn(m3<R> const &A, m3<R> const &B)
 	return C;
 }
 // C = A B^+
-template <class R> MD m3<R> mul_nd(m3<R> const &A, m3<R> const &B)
+template <class R> MHD m3<R> mul_nd(m3<R> const &A, m3<R> const &B)
 {
 	m3<R> C;
 #pragma unroll
@@ -123,7 +130,7 @@ template <class R> MD m3<R> mul_nd(m3<R> const &A, m3<R> const &B)
 	return C;
 }
 // C = A^+ B^+
-template <class R> MD m3<R> mul_dd(m3<R> const &A, m3<R> const &B)
+template <class R> MHD m3<R> mul_dd(m3<R> const &A, m3<R> const &B)
 {
 	m3<R> C;
 #pragma unroll
@@ -140,7 +147,7 @@ template <class R> MD m3<R> mul_dd(m3<R> const &A, m3<R> const &B)
 	return C;
 }
 // S += A B
-template <class R> MD void madd_nn(m3<R> &S, m3<R> const &A, m3<R> const &B)
+template <class R> MHD void madd_nn(m3<R> &S, m3<R> const &A, m3<R> const &B)
 {
 #pragma unroll
 	for (int i = 0; i < 3; ++i)
@@ -155,7 +162,7 @@ template <class R> MD void madd_nn(m3<R> &S, m3<R> const &A, m3<R> const &B)
 		}
 }
 // S += A B^+
-template <class R> MD void madd_nd(m3<R> &S, m3<R> const &A, m3<R> const &B)
+template <class R> MHD void madd_nd(m3<R> &S, m3<R> const &A, m3<R> const &B)
 {
 #pragma unroll
 	for (int i = 0; i < 3; ++i)
@@ -171,7 +178,7 @@ template <class R> MD void madd_nd(m3<R> &S, m3<R> const &A, m3<R> const &B)
 }
 
 // A^+ (exact: a transpose and a sign)
-template <class R> MD m3<R> adj_m3(m3<R> const &A)
+template <class R> MHD m3<R> adj_m3(m3<R> const &A)
 {
 	m3<R> r;
 #pragma unroll
@@ -185,11 +192,11 @@ template <class R> MD m3<R> adj_m3(m3<R> const &A)
 // U_nu(x+mu) U_mu(x+nu)^+ from a = U_nu(x+mu), b = U_mu(x+nu).  For nu < mu
 // it is the adjoint of b a^+, the factor of direction nu in plane mu, so the
 // two directions of a plane use one rounded product.
-template <class R> MD m3<R> upper_factor(m3<R> const &a, m3<R> const &b, bool nu_below_mu)
+template <class R> MHD m3<R> upper_factor(m3<R> const &a, m3<R> const &b, bool nu_below_mu)
 {
 	return nu_below_mu ? adj_m3(mul_nd(b, a)) : mul_nd(a, b);
 }
-template <class R> MD m3<R> zero_m3()
+template <class R> MHD m3<R> zero_m3()
 {
 	m3<R> r;
 #pragma unroll
@@ -197,7 +204,7 @@ template <class R> MD m3<R> zero_m3()
 		r.e[k] = c0<R>();
 	return r;
 }
-template <class R> MD m3<R> unit_m3()
+template <class R> MHD m3<R> unit_m3()
 {
 	m3<R> r = zero_m3<R>();
 	r.e[0].re = R(1);
@@ -207,7 +214,7 @@ template <class R> MD m3<R> unit_m3()
 }
 
 // Re tr(A B^+)
-template <class R> MD R retr_nd(m3<R> const &A, m3<R> const &B)
+template <class R> MHD R retr_nd(m3<R> const &A, m3<R> const &B)
 {
 	R s = R(0);
 #pragma unroll
@@ -218,7 +225,7 @@ template <class R> MD R retr_nd(m3<R> const &A, m3<R> const &B)
 	}
 	return s;
 }
-template <class R> MD R norm2_m3(m3<R> const &A)
+template <class R> MHD R norm2_m3(m3<R> const &A)
 {
 	R s = R(0);
 #pragma unroll
@@ -231,7 +238,7 @@ template <class R> MD R norm2_m3(m3<R> const &A)
 }
 
 // traceless antihermitian part (M - M^+)/2 - tr(M - M^+)/6
-template <class R> MD m3<R> ta(m3<R> const &M)
+template <class R> MHD m3<R> ta(m3<R> const &M)
 {
 	m3<R> A;
 	R t = ((M.e[0].im + M.e[4].im) + M.e[8].im) * (R(1) / R(3));
@@ -250,7 +257,7 @@ template <class R> MD m3<R> ta(m3<R> const &M)
 	return A;
 }
 
-template <class R> MD cpx<R> det(m3<R> const &X)
+template <class R> MHD cpx<R> det(m3<R> const &X)
 {
 	cpx<R> m0 = c0<R>(), m1 = c0<R>(), m2 = c0<R>();
 	cmadd(m0, X.e[4], X.e[8]);
@@ -274,7 +281,7 @@ constexpr double EXP_SMALL_R2 = 0.04;
 template <class R> constexpr int exp_terms_small() { return sizeof(R) == 8 ? 11 : 6; }
 
 template <class R, int N>
-MD void exp_series_term(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2)
+MHD void exp_series_term(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2)
 {
 	const R inv = R(1) / R(N + 1);
 	cpx<R> gb = cmul(ga, b);
@@ -292,7 +299,7 @@ MD void exp_series_term(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R
 	f2.im += ga.im;
 }
 template <class R, int N0, int... N>
-MD void exp_series(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2,
+MHD void exp_series(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2,
                    std::integer_sequence<int, N...>)
 {
 	(exp_series_term<R, N0 + N>(a, b, al, be, ga, f0, f1, f2), ...);
@@ -309,7 +316,7 @@ MD void exp_series(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0
 // diagonal near 1 on a grid of 6e-8 and makes |exp X| systematically 1.4e-9
 // too small per link step (measured over 4e5 random links); in U + D U only
 // the final sum is rounded at the magnitude of U, which is unbiased.
-template <class R> MD m3<R> expm1su3(m3<R> X)
+template <class R> MHD m3<R> expm1su3(m3<R> X)
 {
 	m3<R> X2 = mul_nn(X, X);
 	R trx2 = (X2.e[0].re + X2.e[4].re) + X2.e[8].re;
@@ -374,7 +381,7 @@ template <class R> MD m3<R> expm1su3(m3<R> X)
 
 // U + D U (D = exp X - 1): the product summed from zero, then one rounding at
 // the magnitude of U
-template <class R> MD m3<R> add_mul(m3<R> const &D, m3<R> const &U)
+template <class R> MHD m3<R> add_mul(m3<R> const &D, m3<R> const &U)
 {
 	m3<R> r;
 #pragma unroll
@@ -392,10 +399,10 @@ template <class R> MD m3<R> add_mul(m3<R> const &D, m3<R> const &U)
 }
 
 // link evolution exp(X) U (src/gauge/hmc.h:26-36) as U + (exp X - 1) U
-template <class R> MD m3<R> exp_mul(m3<R> const &X, m3<R> const &U) { return add_mul(expm1su3(X), U); }
+template <class R> MHD m3<R> exp_mul(m3<R> const &X, m3<R> const &U) { return add_mul(expm1su3(X), U); }
 
 // exp of a traceless antihermitian matrix, 1 + (exp X - 1)
-template <class R> MD m3<R> expsu3(m3<R> const &X)
+template <class R> MHD m3<R> expsu3(m3<R> const &X)
 {
 	m3<R> E = expm1su3(X);
 	E.e[0].re += R(1);
@@ -410,7 +417,7 @@ template <class R> MD m3<R> expsu3(m3<R> const &X)
 // 4/3 and 1/3 would be rounded up in FP32 (each by 3e-8 relative) and scale
 // every link up by 1.3e-8; here 1/3 only multiplies a difference of order the
 // non-unitarity.
-template <class R> MD m3<R> reunit_inc(m3<R> const &A)
+template <class R> MHD m3<R> reunit_inc(m3<R> const &A)
 {
 	m3<R> M; // 1 - A A^+
 #pragma unroll
@@ -453,7 +460,7 @@ template <class R> MD m3<R> reunit_inc(m3<R> const &A)
 	return r;
 }
 
-template <class R> MD m3<double> to_f64(m3<R> const &a)
+template <class R> MHD m3<double> to_f64(m3<R> const &a)
 {
 	m3<double> r;
 #pragma unroll
@@ -461,7 +468,7 @@ template <class R> MD m3<double> to_f64(m3<R> const &a)
 		r.e[k] = {(double)a.e[k].re, (double)a.e[k].im};
 	return r;
 }
-template <class R> MD m3<R> from_f64(m3<double> const &a)
+template <class R> MHD m3<R> from_f64(m3<double> const &a)
 {
 	m3<R> r;
 #pragma unroll
@@ -474,7 +481,7 @@ template <class R> MD m3<R> from_f64(m3<double> const &a)
 // rounded once: in FP32 the projection's own roundings (1 - a a^+ is formed
 // from terms near 1) leave a mean ||U||^2/3 - 1 of -3.9e-9, against 1e-11
 // when rounded once from FP64 (both over 4e5 random links).
-template <class R> MD m3<R> reunit(m3<R> const &A)
+template <class R> MHD m3<R> reunit(m3<R> const &A)
 {
 	if constexpr (sizeof(R) == 8)
 		return reunit_inc(A);
@@ -491,7 +498,7 @@ template <class R> MD m3<R> reunit(m3<R> const &A)
 // the caller's P is exactly antihermitian (the compact layout): the strict
 // lower triangle is then written as -conj of the upper one, which is the
 // value the full update would round to.
-template <bool AH, class R> MD m3<R> mom_step(m3<R> const &p, m3<R> const &M, double k)
+template <bool AH, class R> MHD m3<R> mom_step(m3<R> const &p, m3<R> const &M, double k)
 {
 	m3<double> A = ta(to_f64(M));
 	m3<R> r;

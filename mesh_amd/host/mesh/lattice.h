@@ -155,14 +155,19 @@ class Grid
 // ---------------------------------------------------------------- groups
 
 // element types: the reference's U1<T> = complex<T>, SU3<T> = Matrix<complex<T>,3>
+//
+// SU3<T> is one 3x3 complex matrix, row-major (re, im) pairs -- the value a
+// device lambda of lattice_apply (mesh/lattice_apply.h) sees for one site --
+// and carries the group's traits.
 template <class T> struct SU3
 {
 	using real = T;
-	using element = std::array<std::complex<T>, 9>; // row-major
+	using element = std::array<std::complex<T>, 9>; // row-major (host transfers)
 	static constexpr int prec = sizeof(T) == 8 ? MESH_F64 : MESH_F32;
 	static std::string name() { return "su3"; }
 	static constexpr int Nc() { return 3; }
 	static constexpr int dim() { return 8; }
+	T v[18];
 };
 
 // U(1): the reference's complex<T> link exp(i theta) is stored as its angle,
@@ -234,12 +239,17 @@ template <class G> class Field
 		allocate();
 		device_memcpy(ptr_, o.ptr_, bytes_);
 	}
+	// copy assignment reallocates only if the size differs
+	// (src/lattice/lattice.h:83-93); a view of one direction of a stack keeps
+	// its memory, as the reference's U[mu] = ... writes into the stack
 	Field &operator=(Field const &o)
 	{
 		if (this == &o)
 			return *this;
-		if (bytes_ != o.bytes_ || !owner_)
+		if (!ptr_ || bytes_ != o.bytes_)
 		{
+			if (!owner_ && ptr_)
+				throw std::runtime_error("assignment of a differently sized lattice to a view");
 			grid_ = o.grid_;
 			nf_ = o.nf_;
 			allocate();
@@ -248,7 +258,21 @@ template <class G> class Field
 		return *this;
 	}
 	Field(Field &&o) noexcept = default;
-	Field &operator=(Field &&o) noexcept = default;
+	// move assignment steals the buffer (src/lattice/lattice.h:95-100), except
+	// into a view, whose memory belongs to its stack: there it copies
+	Field &operator=(Field &&o)
+	{
+		if (this == &o)
+			return *this;
+		if (!owner_ && ptr_)
+			return *this = static_cast<Field const &>(o);
+		grid_ = std::move(o.grid_);
+		nf_ = std::exchange(o.nf_, 0);
+		bytes_ = std::exchange(o.bytes_, 0);
+		owner_ = std::move(o.owner_);
+		ptr_ = std::exchange(o.ptr_, nullptr);
+		return *this;
+	}
 
 	Grid const &grid() const { return grid_; }
 	int nfields() const { return nf_; }
@@ -313,15 +337,52 @@ template <class G> class Lattice : public Field<G>
 
 template <class G> class LatticeStack : public Field<G>
 {
+	// one non-owning Lattice view per direction, rebuilt when the stack's
+	// memory changes (copy, move, swap)
+	mutable std::vector<Lattice<G>> views_;
+	mutable void const *views_of_ = nullptr;
+
+	std::vector<Lattice<G>> &views() const
+	{
+		if (views_of_ != this->ptr_ || (int)views_.size() != this->nf_)
+		{
+			views_.clear();
+			for (int mu = 0; mu < this->nf_; ++mu)
+				views_.push_back(Lattice<G>::view(const_cast<LatticeStack &>(*this), mu));
+			views_of_ = this->ptr_;
+		}
+		return views_;
+	}
+
   public:
 	LatticeStack() = default;
 	explicit LatticeStack(Grid const &g) : Field<G>(g, g.ndim()) {}
-	// direction mu as a non-owning Lattice view (src/lattice/lattice.h:190-191)
-	Lattice<G> operator[](size_t mu)
+	LatticeStack(LatticeStack const &o) : Field<G>(o) {}
+	LatticeStack(LatticeStack &&o) noexcept : Field<G>(std::move(o)) {}
+	LatticeStack &operator=(LatticeStack const &o)
+	{
+		Field<G>::operator=(o);
+		return *this;
+	}
+	LatticeStack &operator=(LatticeStack &&o)
+	{
+		Field<G>::operator=(std::move(o));
+		return *this;
+	}
+	size_t size() const { return (size_t)this->nf_; }
+	// direction mu: a Lattice that views the stack's memory
+	// (src/lattice/lattice.h:190-191 returns a reference to it as well)
+	Lattice<G> &operator[](size_t mu)
 	{
 		if ((int)mu >= this->nf_)
 			throw std::out_of_range("direction out of range");
-		return Lattice<G>::view(*this, (int)mu);
+		return views()[mu];
+	}
+	Lattice<G> const &operator[](size_t mu) const
+	{
+		if ((int)mu >= this->nf_)
+			throw std::out_of_range("direction out of range");
+		return views()[mu];
 	}
 };
 

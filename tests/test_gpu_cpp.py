@@ -19,6 +19,17 @@ def test_cpp_lattice_gauge():
     assert "PASSED" in r.stdout
 
 
+def test_cpp_lattice_apply():
+    """mesh/lattice_apply.h: the reference's staple, evolve and reunitize
+    lambdas and its plaquette, wilsonDeriv and Landau expressions compiled by
+    hipcc over the library's layout, against the named kernels (evolve and
+    reunitize bit for bit) and against the same lambdas on the host (bits)."""
+    r = subprocess.run([os.path.join(BIN, "test_lattice_apply")], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout and r.stdout.count("PASS") >= 40
+
+
 @pytest.mark.parametrize("precision", ["2", "1"])
 def test_cpp_hmc_driver(precision):
     r = subprocess.run([os.path.join(BIN, "mesh_hmc"), "--geom", "8", "8", "8", "8", "--count", "6", "--therm", "3",
