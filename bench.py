@@ -20,8 +20,14 @@ recorded around every launch on the library's stream during the timed region.
 Per-site algorithmic bytes of every kernel are in kernel_bytes(); the
 trajectory's total is the sum over the launches actually made.
 
+The acceptance, plaquette and <exp(-dH)> of the timed window belong to a
+chain only 50 + W trajectories from a Haar start; the "physics" objects come
+from the same chain continued untimed (--equil trajectories without
+Metropolis, then --measure recorded with it), near equilibrium.
+
 cpu_baseline: the CPU oracle (FP64, OpenMP) running one trajectory of the
-same workload on the host cores.
+same workload on the host cores, with the one-thread and reference-structured
+(cshift copies, separate passes) figures beside it.
 """
 import argparse
 from ctypes import c_double as C_double
@@ -81,7 +87,7 @@ def measured_traffic(prec_name, L):
     return None
 
 
-def run_gpu(prec, L, steps, warmup, therm, seed, scheme, eps, substeps):
+def run_gpu(prec, L, steps, warmup, therm, seed, scheme, eps, substeps, equil=0, measure=0):
     import numpy as np
 
     import mesh_amd
@@ -111,8 +117,51 @@ def run_gpu(prec, L, steps, warmup, therm, seed, scheme, eps, substeps):
         ms, n = ctx.profile_read(k)
         kern[k] = dict(avg_ms=ms / n if n else None, launches=n)
     s = hmc.summary()
-    return dict(hmc=hmc, seconds=dt, V=g.size(), kernels=kern, acceptance=s["acceptance"],
-                plaquette=s["plaquette"], exp_mdH=s["exp_mdH"], dH=list(hmc.deltaH_history))
+    out = dict(hmc=hmc, seconds=dt, V=g.size(), kernels=kern, acceptance=s["acceptance"],
+               plaquette=s["plaquette"], exp_mdH=s["exp_mdH"], dH=list(hmc.deltaH_history))
+    if measure > 0:
+        out["physics"] = physics(hmc, deltas, therm + warmup + steps, equil, measure)
+    return out
+
+
+def physics(hmc, deltas, done, equil, measure):
+    """The timed chain continued, untimed: `equil` trajectories without
+    Metropolis (at 32^4 this integrator accepts ~10% at equilibrium, so a
+    Metropolis chain would hardly move towards it), then `measure` Metropolis
+    trajectories recorded.  Acceptance (against erfc(sqrt(<dH>)/2), its
+    equilibrium value for Gaussian dH), plaquette, <dH> and <exp(-dH)> (= 1 at
+    equilibrium, src/gauge/hmc.h:98-99) with errors from 10 blocks, the
+    plaquette of the window's two halves, and the standard error
+    sqrt((exp(2 <dH>) - 1) / N) that <exp(-dH)> has at equilibrium: at <dH> ~ 4
+    it is far above 1 and the Creutz average cannot be checked from this
+    window."""
+    import numpy as np
+
+    for _ in range(equil):
+        hmc.runHmcUpdate(6.0, deltas, metropolis=False)
+    hmc.reset_observables()
+    for _ in range(measure):
+        hmc.runHmcUpdate(6.0, deltas)
+
+    def blocked(x):
+        x = np.asarray(x, np.float64)
+        b = x[: len(x) // 10 * 10].reshape(10, -1).mean(axis=1)
+        return float(x.mean()), float(b.std(ddof=1) / math.sqrt(10))
+
+    dH = np.array(hmc.deltaH_history)
+    p = np.array(hmc.plaq_history)
+    acc, acc_e = blocked(hmc.accept_history)
+    pl, pl_e = blocked(p)
+    md, md_e = blocked(dH)
+    ex, ex_e = blocked(np.exp(-dH))
+    h = len(p) // 2
+    return {"trajectories_before": done + equil, "measured": measure,
+            "acceptance": acc, "acceptance_err": acc_e,
+            "acceptance_erfc": math.erfc(math.sqrt(max(md, 0.0)) / 2),
+            "plaquette": pl, "plaquette_err": pl_e,
+            "plaquette_first_half": float(p[:h].mean()), "plaquette_second_half": float(p[h:].mean()),
+            "mean_dH": md, "mean_dH_err": md_e, "exp_minus_dH": ex, "exp_minus_dH_err_blocks": ex_e,
+            "exp_minus_dH_err_equilibrium": math.sqrt(math.expm1(min(2 * max(md, 0.0), 700.0)) / len(dH))}
 
 
 def same_config_compare(U64, seed, scheme, eps, substeps):
@@ -135,19 +184,55 @@ def same_config_compare(U64, seed, scheme, eps, substeps):
                 H_old_f32_minus_f64=out[F32][2] - out[F64][2], H_new_f32_minus_f64=out[F32][3] - out[F64][3])
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(L, seed, scheme, eps, substeps):
+    """The CPU oracle (FP64) on the host cores of this box.  `value`: one full
+    trajectory of the workload, fused per-site restatement, every OpenMP
+    thread.  Beside it, extrapolated from one pass of each stage on the same
+    field (t_momenta + N_F t_force + N_L t_link + t_end): the fused restatement
+    on one thread, and the reference's pass structure (cshift copies and
+    separate full-lattice passes, oracle_refpass.cpp) on all threads and on one."""
     from oracle import oracle as O
 
     dims = [L] * 4
+    V = L ** 4
     U = O.random_gauge(dims, seed)
     d = O.make_deltas(scheme, eps, substeps)
+    n_force, n_link = (len(d) - 1) // 2, (len(d) + 1) // 2
+    threads = O.max_threads()
     t0 = time.perf_counter()
-    O.hmc_trajectory(U, dims, 6.0, d, seed, 0, metropolis=False)
-    dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=L ** 4 / dt, unit="site-updates/s", cores=cores, kind="port",
+    O.hmc_trajectory(U.copy(), dims, 6.0, d, seed, 0, metropolis=False)
+    full = time.perf_counter() - t0
+
+    def per_traj(t, refpass):
+        return t[0] + n_force * t[1] + (n_link if refpass else n_link - 1) * t[2] + t[3]
+
+    modes = {}
+    for name, refpass, nt in (("fused_1_thread", False, 1), ("reference_structured", True, threads),
+                              ("reference_structured_1_thread", True, 1)):
+        O.set_threads(nt)
+        t = O.stage_times(U, dims, 6.0, d[1], refpass=refpass)
+        modes[name] = {"value": V / per_traj(t, refpass), "threads": nt,
+                       "stage_seconds": [round(float(x), 4) for x in t]}
+    O.set_threads(threads)
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return dict(value=V / full, unit="site-updates/s", cores=threads, kind="port",
                 sample=f"one FP64 trajectory of the same workload ({L}^4, {scheme}, eps={eps}, "
-                       f"{substeps} substeps) from the Haar start, CPU oracle with OpenMP; {dt:.2f} s")
+                       f"{substeps} substeps) from the Haar start, fused CPU oracle, {threads} OpenMP threads; "
+                       f"{full:.2f} s",
+                cpu_model=cpu_model(), affinity_cpus=aff,
+                modes=modes,
+                modes_note=("per trajectory from one timed pass of each stage at the same size: momenta, "
+                            f"{n_force} momentum steps, {n_link} link steps, end (reunitarisation, Delta H)"))
 
 
 def kernel_only(name, prec, L, reps, variant=-1):
@@ -331,6 +416,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--therm", type=int, default=50)
+    ap.add_argument("--equil", type=int, default=300,
+                    help="untimed trajectories without Metropolis after the timed ones, before the physics window")
+    ap.add_argument("--measure", type=int, default=100, help="trajectories of the physics window (0: none)")
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--scheme", default="2mn")
@@ -388,7 +476,8 @@ def main():
     mesh_amd.default_context().set_option("compact_momenta", int(compact))
     momenta = ("compact antihermitian, 9 reals per link (bit-identical to 18-real matrices)" if compact
                else "18-real matrices")
-    r64 = run_gpu(F64, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
+    r64 = run_gpu(F64, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps, a.equil,
+                  a.measure)
     V = r64["V"]
     value = V * a.steps / r64["seconds"]
     dom = dominant(r64["kernels"])
@@ -422,10 +511,12 @@ def main():
         "acceptance": r64["acceptance"],
         "plaquette": r64["plaquette"],
         "exp_minus_dH": r64["exp_mdH"],
+        **({"physics": r64["physics"]} if "physics" in r64 else {}),
         "kernels_f64": r64["kernels"],
     }
     if not a.no_f32:
-        r32 = run_gpu(F32, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps)
+        r32 = run_gpu(F32, a.L, a.steps, a.warmup, a.therm, a.seed, a.scheme, a.epsilon, a.substeps, a.equil,
+                      a.measure)
         v32 = V * a.steps / r32["seconds"]
         dom32 = dominant(r32["kernels"])
         fk32 = r32["kernels"][dom32]
@@ -434,6 +525,7 @@ def main():
         line["f32_mixed"] = {
             "value": v32, "ms_per_step": 1e3 * r32["seconds"] / a.steps, "ratio_vs_f64": v32 / value,
             "acceptance": r32["acceptance"], "plaquette": r32["plaquette"],
+            **({"physics": r32["physics"]} if "physics" in r32 else {}),
             "dominant_kernel": dom32, "dominant_frac_of_hbm": ach32 / HBM_PEAK_GBS,
             "dominant_traffic": measured_traffic("f32", a.L) if dom32 == "force_link" and compact else None,
             "trajectory_roofline_frac": v32 * trajectory_bytes_per_site(r32["kernels"], 4, a.steps, compact) / 8e12,

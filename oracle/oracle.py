@@ -83,6 +83,12 @@ def _declare(L):
     L.orc_axpy_f64.argtypes = [_dp, C.c_double, _dp, C.c_int64]
     L.orc_su3_random_transform_f64.argtypes = [_dp, _ip, C.c_int, C.c_uint64]
     L.orc_su3_gauge_transform_f64.argtypes = [_dp, _dp, _ip, C.c_int]
+    L.orc_su3_stage_times_f64.argtypes = [_dp, _ip, C.c_int, C.c_double, C.c_double, _dp]
+    L.orc_refpass_trajectory_f64.argtypes = [_dp, _ip, C.c_int, C.c_double, _dp, C.c_int, C.c_uint64,
+                                             C.c_uint32, C.c_int, _dp]
+    L.orc_refpass_stage_times_f64.argtypes = [_dp, _ip, C.c_int, C.c_double, C.c_double, _dp]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_max_threads.restype = C.c_int
 
 
 # ---------------------------------------------------------------- helpers
@@ -393,3 +399,34 @@ TAG_HAAR = 0x48414152
 TAG_MOM = 0x4D4F4D30
 TAG_METRO = 0x4D455452
 TAG_GAUGE = 0x47415547
+
+
+# ---------------------------------------------------------------- CPU baselines
+
+def stage_times(U, dims, beta, delta, refpass=False):
+    """Seconds of one pass of each trajectory stage on U (FP64):
+    (momenta, momentum step, link step, last link step with Delta H); with
+    refpass the reference's pass structure (cshift copies, separate passes,
+    oracle_refpass.cpp), whose last stage is reunitarisation + H_new."""
+    t = np.zeros(4)
+    f = lib().orc_refpass_stage_times_f64 if refpass else lib().orc_su3_stage_times_f64
+    f(_real(U), _dims(dims), len(dims), beta, delta, t)
+    return t
+
+
+def refpass_trajectory(U, dims, beta, deltas, seed, traj, metropolis=True):
+    """One trajectory in the reference's pass structure, in place on U.
+    Returns dict(dH, accepted, H_old, H_new)."""
+    out = np.zeros(4)
+    d = np.ascontiguousarray(deltas, np.float64)
+    lib().orc_refpass_trajectory_f64(_real(U), _dims(dims), len(dims), beta, d, d.size, seed, traj,
+                                     int(metropolis), out)
+    return dict(dH=out[0], accepted=bool(out[1]), H_old=out[2], H_new=out[3])
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return lib().orc_max_threads()

@@ -16,7 +16,9 @@
 // src/lattice/device.h:55-57, 82-84).
 #include "oracle_su3.h"
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
+#include <omp.h>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -775,6 +777,37 @@ void orc_su3_hmc_trajectory_f32(float *U, int const *dims, int nd, double beta, 
 {
 	hmc_trajectory(U, Geo(dims, nd), beta, deltas, nde, seed, traj, metropolis, out);
 }
+
+// Wall time of each stage of hmc_trajectory, run once on U (not modified):
+// t[0] momenta with k(x), t[1] one momentum step, t[2] one link step, t[3]
+// the last link step with reunitarisation and Delta H.  A trajectory with
+// N_F momentum and N_L link steps takes t[0] + N_F t[1] + (N_L - 1) t[2] + t[3].
+void orc_su3_stage_times_f64(double const *U, int const *dims, int nd, double beta, double delta, double *t)
+{
+	Geo g(dims, nd);
+	size_t n = (size_t)g.nd * g.V * 18;
+	std::vector<double> P(n), Un(n), K(g.V);
+	auto now = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+	double a = now();
+	momenta(P.data(), K.data(), g, 1, 0);
+	double b = now();
+	force_update(U, P.data(), g, beta, delta);
+	double c = now();
+	link_update(U, Un.data(), P.data(), g, delta, 0);
+	double d = now();
+	link_update(U, Un.data(), P.data(), g, delta, 1);
+	double dh[5];
+	delta_h(U, Un.data(), P.data(), K.data(), g, beta, dh);
+	double e = now();
+	t[0] = b - a;
+	t[1] = c - b;
+	t[2] = d - c;
+	t[3] = e - d;
+}
+
+// OpenMP threads of the oracle's passes
+void orc_set_threads(int n) { omp_set_num_threads(n); }
+int orc_max_threads() { return omp_get_max_threads(); }
 
 double orc_landau_functional_f64(double const *U, double const *h, int const *dims, int nd)
 {

@@ -43,3 +43,15 @@ def test_measured_traffic_from_profiles():
     assert bench.kernel_bytes("force_link", 8) * V < f64 < 2 * bench.kernel_bytes("force_link", 8) * V
     assert bench.kernel_bytes("force_link", 4) * V < f32 < 2 * bench.kernel_bytes("force_link", 4) * V
     assert bench.measured_traffic("f64", 48) is None
+
+
+def test_cpu_baseline_fields():
+    """the CPU baseline reports the fused oracle on all threads as its value
+    and, beside it, the one-thread and reference-structured figures, the CPU
+    model and the affinity core count"""
+    r = bench.cpu_baseline(4, 2, "2mn", 0.5, 2)
+    assert r["value"] > 0 and r["unit"] == "site-updates/s" and r["kind"] == "port"
+    assert r["cores"] >= 1 and r["affinity_cpus"] >= 1
+    assert set(r["modes"]) == {"fused_1_thread", "reference_structured", "reference_structured_1_thread"}
+    assert all(m["value"] > 0 and len(m["stage_seconds"]) == 4 for m in r["modes"].values())
+    assert r["modes"]["fused_1_thread"]["threads"] == 1

@@ -333,3 +333,28 @@ def test_axpy_rounds_product_then_sum():
     a = 0.3712
     r = O.axpy(y, a, x)
     assert np.array_equal(r.real, y.real + x.real * a) and np.array_equal(r.imag, y.imag + x.imag * a)
+
+
+def test_reference_structured_trajectory_matches_fused():
+    """The reference's pass structure (cshift copies, separate passes, global
+    action sums; oracle_refpass.cpp) and the fused per-site restatement give
+    the same trajectory to rounding: the products associate differently, so
+    the fields agree to ~1e-13 and Delta H to ~1e-10, not bit for bit."""
+    dims = [4, 4, 4, 4]
+    d = O.make_deltas("2mn", 0.5, 4)
+    U1 = O.random_gauge(dims, 7)
+    U2 = U1.copy()
+    for traj in range(3):
+        a = O.hmc_trajectory(U1, dims, 6.0, d, 7, traj, metropolis=False)
+        b = O.refpass_trajectory(U2, dims, 6.0, d, 7, traj, metropolis=False)
+        assert abs(a["dH"] - b["dH"]) < 1e-9 * max(1.0, abs(a["dH"])), (traj, a["dH"], b["dH"])
+        assert abs(a["H_old"] - b["H_old"]) < 1e-9 * abs(a["H_old"])
+    assert np.abs(U1 - U2).max() < 1e-11
+
+
+def test_stage_times_report_every_stage():
+    dims = [4, 4, 4, 4]
+    U = O.random_gauge(dims, 8)
+    for refpass in (False, True):
+        t = O.stage_times(U, dims, 6.0, 0.05, refpass=refpass)
+        assert t.shape == (4,) and (t > 0).all()
