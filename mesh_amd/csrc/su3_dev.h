@@ -280,11 +280,66 @@ template <class R> constexpr int exp_terms() { return sizeof(R) == 8 ? 18 : 10; 
 constexpr double EXP_SMALL_R2 = 0.04;
 template <class R> constexpr int exp_terms_small() { return sizeof(R) == 8 ? 11 : 6; }
 
+// X^2 of an exactly antihermitian X (every t TA(P) is one: TA's roundings are
+// antisymmetric): hermitian, so the upper triangle is formed and the lower
+// one is its conjugate, and the diagonal is the real -sum_k |X_ik|^2
+template <class R> MHD m3<R> sq_ah(m3<R> const &X)
+{
+	m3<R> Q;
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+	{
+		R s = R(0);
+#pragma unroll
+		for (int k = 0; k < 3; ++k)
+		{
+			s = fma(X.e[3 * i + k].re, X.e[3 * i + k].re, s);
+			s = fma(X.e[3 * i + k].im, X.e[3 * i + k].im, s);
+		}
+		Q.e[4 * i] = {-s, R(0)};
+	}
+#pragma unroll
+	for (int i = 0; i < 3; ++i)
+#pragma unroll
+		for (int j = i + 1; j < 3; ++j)
+		{
+			cpx<R> acc = c0<R>();
+#pragma unroll
+			for (int k = 0; k < 3; ++k)
+				cmadd(acc, X.e[3 * i + k], X.e[3 * k + j]);
+			Q.e[3 * i + j] = acc;
+			Q.e[3 * j + i] = {acc.re, -acc.im};
+		}
+	return Q;
+}
+
+// Im det X of an exactly antihermitian X (its determinant is imaginary):
+// with H = -i X hermitian, det X = -i det H and
+// det H = h0 h1 h2 + 2 Re(H01 H12 H20) - h0 |H12|^2 - h1 |H02|^2 - h2 |H01|^2
+template <class R> MHD R imdet_ah(m3<R> const &X)
+{
+	const R h0 = X.e[0].im, h1 = X.e[4].im, h2 = X.e[8].im;
+	const cpx<R> a = {X.e[1].im, -X.e[1].re}; // H01
+	const cpx<R> b = {X.e[5].im, -X.e[5].re}; // H12
+	const cpx<R> c = {X.e[2].im, -X.e[2].re}; // H02 (H20 = conj H02)
+	const cpx<R> p = cmul(a, b);
+	const R t = fma(p.re, c.re, p.im * c.im);
+	const R n01 = fma(a.re, a.re, a.im * a.im), n12 = fma(b.re, b.re, b.im * b.im),
+	        n02 = fma(c.re, c.re, c.im * c.im);
+	R d = (h0 * h1) * h2;
+	d = fma(R(2), t, d);
+	d = fma(-h0, n12, d);
+	d = fma(-h1, n02, d);
+	d = fma(-h2, n01, d);
+	return -d;
+}
+
+// one term of the Cayley-Hamilton series; b = i bi is the (imaginary) det X
 template <class R, int N>
-MHD void exp_series_term(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2)
+MHD void exp_series_term(R a, R bi, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2)
 {
 	const R inv = R(1) / R(N + 1);
-	cpx<R> gb = cmul(ga, b);
+	cpx<R> gb = {-ga.im * bi, ga.re * bi};
 	cpx<R> nal = {gb.re * inv, gb.im * inv};
 	cpx<R> nbe = {fma(ga.re, a, al.re) * inv, fma(ga.im, a, al.im) * inv};
 	cpx<R> nga = {be.re * inv, be.im * inv};
@@ -299,14 +354,15 @@ MHD void exp_series_term(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<
 	f2.im += ga.im;
 }
 template <class R, int N0, int... N>
-MHD void exp_series(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2,
+MHD void exp_series(R a, R bi, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f0, cpx<R> &f1, cpx<R> &f2,
                    std::integer_sequence<int, N...>)
 {
-	(exp_series_term<R, N0 + N>(a, b, al, be, ga, f0, f1, f2), ...);
+	(exp_series_term<R, N0 + N>(a, bi, al, be, ga, f0, f1, f2), ...);
 }
 
-// exp(X) - 1 of a traceless antihermitian matrix: Cayley-Hamilton with
-// power-series coefficients (X^3 = a X + b with a = tr X^2 / 2, b = det X),
+// exp(X) - 1 of an exactly antihermitian, traceless matrix (X = t TA(P)):
+// Cayley-Hamilton with power-series coefficients (X^3 = a X + b with
+// a = tr X^2 / 2, b = det X, imaginary for such X),
 // exp X = f0 1 + f1 X + f2 X^2, argument halved until ||X||_F <= 1 and the
 // result squared back as D -> 2 D + D^2; the series stops after
 // exp_terms_small() terms where ||X||_F^2 <= EXP_SMALL_R2.
@@ -318,7 +374,7 @@ MHD void exp_series(R a, cpx<R> b, cpx<R> &al, cpx<R> &be, cpx<R> &ga, cpx<R> &f
 // the final sum is rounded at the magnitude of U, which is unbiased.
 template <class R> MHD m3<R> expm1su3(m3<R> X)
 {
-	m3<R> X2 = mul_nn(X, X);
+	m3<R> X2 = sq_ah(X);
 	R trx2 = (X2.e[0].re + X2.e[4].re) + X2.e[8].re;
 	R r2 = -trx2;
 	int s = 0;
@@ -344,13 +400,13 @@ template <class R> MHD m3<R> expm1su3(m3<R> X)
 		trx2 *= sc2;
 	}
 	R a = R(0.5) * trx2;
-	cpx<R> b = det(X);
+	const R bi = imdet_ah(X);
 	cpx<R> al = {R(1), R(0)}, be = c0<R>(), ga = c0<R>();
 	cpx<R> g0 = c0<R>(), f1 = c0<R>(), f2 = c0<R>();
 	constexpr int NS = exp_terms_small<R>(), NF = exp_terms<R>();
-	exp_series<R, 0>(a, b, al, be, ga, g0, f1, f2, std::make_integer_sequence<int, NS>{});
+	exp_series<R, 0>(a, bi, al, be, ga, g0, f1, f2, std::make_integer_sequence<int, NS>{});
 	if (r2 > R(EXP_SMALL_R2))
-		exp_series<R, NS>(a, b, al, be, ga, g0, f1, f2, std::make_integer_sequence<int, NF - NS>{});
+		exp_series<R, NS>(a, bi, al, be, ga, g0, f1, f2, std::make_integer_sequence<int, NF - NS>{});
 	m3<R> D;
 #pragma unroll
 	for (int k = 0; k < 9; ++k)

@@ -260,7 +260,55 @@ template <class T> constexpr int exp_terms() { return sizeof(T) == 8 ? 18 : 10; 
 template <class T> constexpr int exp_terms_small() { return sizeof(T) == 8 ? 11 : 6; }
 constexpr double exp_small_r2 = 0.04;
 
-// exp(X) - 1 for traceless antihermitian X.
+// X^2 of an exactly antihermitian X: hermitian, upper triangle formed, lower
+// one its conjugate, real diagonal -sum_k |X_ik|^2
+template <class T> inline mat3<T> sq_ah(mat3<T> const &X)
+{
+	mat3<T> Q;
+	for (int i = 0; i < 3; ++i)
+	{
+		T s = T(0);
+		for (int k = 0; k < 3; ++k)
+		{
+			s = std::fma(X.e[i][k].re, X.e[i][k].re, s);
+			s = std::fma(X.e[i][k].im, X.e[i][k].im, s);
+		}
+		Q.e[i][i] = {-s, T(0)};
+	}
+	for (int i = 0; i < 3; ++i)
+		for (int j = i + 1; j < 3; ++j)
+		{
+			cplx<T> acc = {T(0), T(0)};
+			for (int k = 0; k < 3; ++k)
+				cmadd(acc, X.e[i][k], X.e[k][j]);
+			Q.e[i][j] = acc;
+			Q.e[j][i] = {acc.re, -acc.im};
+		}
+	return Q;
+}
+
+// Im det X of an exactly antihermitian X: with H = -i X hermitian,
+// det X = -i det H, det H = h0 h1 h2 + 2 Re(H01 H12 H20) - h0 |H12|^2
+// - h1 |H02|^2 - h2 |H01|^2
+template <class T> inline T imdet_ah(mat3<T> const &X)
+{
+	const T h0 = X.e[0][0].im, h1 = X.e[1][1].im, h2 = X.e[2][2].im;
+	const cplx<T> a = {X.e[0][1].im, -X.e[0][1].re};
+	const cplx<T> b = {X.e[1][2].im, -X.e[1][2].re};
+	const cplx<T> c = {X.e[0][2].im, -X.e[0][2].re};
+	const cplx<T> p = cmul(a, b);
+	const T t = std::fma(p.re, c.re, p.im * c.im);
+	const T n01 = std::fma(a.re, a.re, a.im * a.im), n12 = std::fma(b.re, b.re, b.im * b.im),
+	        n02 = std::fma(c.re, c.re, c.im * c.im);
+	T d = (h0 * h1) * h2;
+	d = std::fma(T(2), t, d);
+	d = std::fma(-h0, n12, d);
+	d = std::fma(-h1, n02, d);
+	d = std::fma(-h2, n01, d);
+	return -d;
+}
+
+// exp(X) - 1 for an exactly antihermitian, traceless X (X = t TA(P)).
 // Cayley-Hamilton: X^3 = a X + b 1 with a = tr(X^2)/2, b = det X, so that
 // X^n = alpha_n 1 + beta_n X + gamma_n X^2 with
 //   alpha_{n+1} = gamma_n b, beta_{n+1} = alpha_n + gamma_n a, gamma_{n+1} = beta_n
@@ -271,7 +319,7 @@ constexpr double exp_small_r2 = 0.04;
 // (g0 below), so every entry of D is rounded at its own magnitude.
 template <class T> inline mat3<T> expm1su3(mat3<T> X)
 {
-	mat3<T> X2 = mul_nn(X, X);
+	mat3<T> X2 = sq_ah(X);
 	T trx2 = (X2.e[0][0].re + X2.e[1][1].re) + X2.e[2][2].re;
 	T r2 = -trx2; // ||X||_F^2
 	int s = 0;
@@ -296,14 +344,14 @@ template <class T> inline mat3<T> expm1su3(mat3<T> X)
 		trx2 *= sc2;
 	}
 	T a = T(0.5) * trx2;
-	cplx<T> b = det3(X);
+	const T bi = imdet_ah(X);
 	cplx<T> al = {T(1), T(0)}, be = {T(0), T(0)}, ga = {T(0), T(0)};
 	cplx<T> g0 = {T(0), T(0)}, f1 = {T(0), T(0)}, f2 = {T(0), T(0)};
 	const int nterms = r2 > T(exp_small_r2) ? exp_terms<T>() : exp_terms_small<T>();
 	for (int n = 0; n < nterms; ++n)
 	{
 		T inv = T(1) / T(n + 1);
-		cplx<T> gb = cmul(ga, b);
+		cplx<T> gb = {-ga.im * bi, ga.re * bi};
 		cplx<T> nal = {gb.re * inv, gb.im * inv};
 		cplx<T> nbe = {std::fma(ga.re, a, al.re) * inv,
 		               std::fma(ga.im, a, al.im) * inv};
