@@ -354,6 +354,93 @@ int mesh_flow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, fl
 int mesh_flow_mh(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, double *state, const float *prop_theta,
                  const double *prop, double beta, uint64_t seed, uint32_t step, double *acc_dev);
 
+/* ---- SU(2) pure-gauge HMC (Hmc<SU2<T>>, src/gauge/hmc.cpp:111-114) --------
+ *
+ * Links are unit quaternions (w, x, y, z), the 2x2 matrices w - i (x, y, z).sigma
+ * -- the reference's SU2<T> =
+ * util::quaternion<T> (src/gauge/groups.h:114-196) -- multiplied as Hamilton
+ * products; adj = conjugate, Re tr = 2 w, project_on_algebra keeps (0, x, y, z).
+ * Device fields are structure of arrays, component c of direction f at
+ * base[(4 f + c) S + x] with S = V rounded up to 256 (mesh_su2_field_bytes);
+ * host arrays are [nf][V][4] in that component order.  Momenta are (0, p/2)
+ * with p ~ N(0,1) (random_algebra_element, groups.h:171-181) and the kinetic
+ * energy is 2 (x^2 + y^2 + z^2), the 2x2 matrix norm, which the force
+ * (beta/4) project_on_algebra(U Sigma) conserves (the reference adds the
+ * quaternion norm, half of it).  exp of an algebra element is
+ * (cos a, v sin(a)/a) as in groups.h:159-169, with the limit at a = 0. */
+size_t mesh_su2_field_bytes(const mesh_grid *g, int nf, int prec);
+/* host <-> device copies of nf directions; upload != 0: host to device */
+int mesh_su2_transfer(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *host, void *dev, int upload);
+/* Haar-random links: four Philox normals normalised (random_group_element, groups.h:183-192) */
+int mesh_su2_random(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U, uint64_t seed);
+/* the unit field */
+int mesh_su2_unit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U);
+/* momenta of trajectory `traj` (randomizeMomenta); K (device, V doubles, may be NULL) gets k(x) */
+int mesh_su2_momenta(mesh_ctx *ctx, const mesh_grid *g, int prec, void *P, double *K, uint64_t seed, uint32_t traj);
+/* P_mu -= delta (beta/4) project_on_algebra(U_mu Sigma_mu) for every mu (runHmd's momentum step) */
+int mesh_su2_force_update(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, void *P, double beta,
+                          double delta);
+/* Uout = exp(t project_on_algebra(P)) Uin (evolve, src/gauge/hmc.h:26-36), reunitarised if asked */
+int mesh_su2_link_update(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *Uout, const void *Uin,
+                         const void *P, double t, int reunit);
+/* project_on_group_fast on every link (reunitize) */
+int mesh_su2_reunit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U);
+/* *out = mean Re tr P / 2 over all plaquettes (plaquette, src/gauge/utils.h:93-108) */
+int mesh_su2_plaquette(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, double *out);
+/* out[5] = {Delta H, sum_x ps_old, sum_x ps_new, K_old, K_new}, ps(x) = sum of Re P / (Re tr / 2) at x */
+int mesh_su2_delta_h(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U_old, const void *U_new,
+                     const void *P, const double *K_old, double beta, double *out);
+/* one trajectory as mesh_su3_hmc_trajectory: out[6] = {Delta H, accepted,
+ * plaquette of the resulting U, H_old, H_new, u}; *U and *U_work swap on
+ * acceptance */
+int mesh_su2_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **U, void **U_work, void *P, double *K,
+                            double beta, const double *deltas, int ndeltas, uint64_t seed, uint32_t traj,
+                            int metropolis, double *out);
+
+/* ---- Landau gauge for SU(2) (Landau<SU2<T>>, src/gauge/landau.h; instantiated
+ * by src/tests/lattice_gauge.cpp:46-53).  h is the gauge field, one quaternion
+ * per site (nf = 1); h == NULL means h = 1 where allowed.  Sums in FP64. */
+/* *out = sum_x,mu Re tr (h U_mu h(x+mu)^+) / (Nc Nd V) (operator(), landau.h:23-34) */
+int mesh_su2_landau_functional(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, const void *h,
+                               double *out);
+/* *out = sum_x |project_on_algebra(h d)|^2; condition() = *out / (Nd V) (landau.h:37-49) */
+int mesh_su2_landau_divsum(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, const void *h, double *out);
+/* F = -0.5 / (Nc Nd V) project_on_algebra(h d) (deriv, landau.h:52-65) */
+int mesh_su2_landau_deriv(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U, const void *h, void *F);
+/* Uout_mu(x) = h(x) U_mu(x) h(x+mu)^+ */
+int mesh_su2_gauge_transform(mesh_ctx *ctx, const mesh_grid *g, int prec, void *Uout, const void *U, const void *h);
+/* overrelaxation of the sites of one parity on the transformed links U (in
+ * place), h = G h when h != NULL; replaces the ascent loop of landau.h:100-136
+ * as mesh_landau_or_halfsweep does for SU(3); needs even extents */
+int mesh_su2_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, int prec, void *U, void *h, int parity,
+                                 double omega);
+/* *out = sum Re tr (A B) over nf directions (sum_real_trace(A * B), src/gauge/utils.h:27-33) */
+int mesh_su2_dot(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void *A, const void *B, double *out);
+/* Y += a X (the F += last * beta of landau.h:128) */
+int mesh_su2_axpy(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *Y, double a, const void *X);
+
+/* ---- Landau gauge for U(1) (Landau<U1<T>>, src/gauge/landau.h) on angles:
+ * links theta[mu][x] and the gauge field g(x) = exp(i phi(x)), phi[x], FP32;
+ * values in FP64 from the angles, rounded once; phi == NULL means phi = 0. */
+/* *out = sum_x,mu cos(phi(x) + theta_mu(x) - phi(x+mu)) / (Nd V) (landau.h:23-34) */
+int mesh_u1_landau_functional(mesh_ctx *ctx, const mesh_grid *g, const float *theta, const float *phi, double *out);
+/* *out = sum_x Im(g d)^2; condition() = *out / (Nd V) (landau.h:37-49) */
+int mesh_u1_landau_divsum(mesh_ctx *ctx, const mesh_grid *g, const float *theta, const float *phi, double *out);
+/* F[x] = -0.5 / (Nd V) Im(g d)(x), the algebra element i F (deriv, landau.h:52-65) */
+int mesh_u1_landau_deriv(mesh_ctx *ctx, const mesh_grid *g, const float *theta, const float *phi, float *F);
+/* theta_out_mu(x) = wrap(phi(x) + theta_mu(x) - phi(x+mu)) */
+int mesh_u1_gauge_transform(mesh_ctx *ctx, const mesh_grid *g, float *theta_out, const float *theta, const float *phi);
+/* overrelaxation of one parity on the transformed angles (in place): each
+ * site turns by -omega arg K(x), phi += the same when phi != NULL; even extents */
+int mesh_u1_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, float *theta, float *phi, int parity,
+                                double omega);
+/* phi_out = wrap(phi_in + t F): g = exp(t i F) g (landau.h:85) */
+int mesh_u1_gauge_step(mesh_ctx *ctx, const mesh_grid *g, float *phi_out, const float *phi_in, const float *F,
+                       double t);
+/* *out = sum Re tr (i A i B) = -sum A B over n values; Y += a X */
+int mesh_u1_dot(mesh_ctx *ctx, long n, const float *A, const float *B, double *out);
+int mesh_u1_axpy(mesh_ctx *ctx, long n, float *Y, double a, const float *X);
+
 /* ---- U(1) plaquette flow (no counterpart in the reference) -----------------
  *
  * On a periodic 2-d lattice the plaquette angles are independent except for

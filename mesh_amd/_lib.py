@@ -62,6 +62,33 @@ _SIGS = {
     "mesh_su3_momenta": (i32, [vp, gp, i32, vp, vp, u64, u32]),
     "mesh_su3_plaquette": (i32, [vp, gp, i32, vp, dp]),
     "mesh_su3_plaquette_noalloc": (i32, [vp, gp, i32, vp, vp]),
+    "mesh_su2_field_bytes": (C.c_size_t, [gp, i32, i32]),
+    "mesh_su2_transfer": (i32, [vp, gp, i32, i32, vp, vp, i32]),
+    "mesh_su2_random": (i32, [vp, gp, i32, i32, vp, u64]),
+    "mesh_su2_unit": (i32, [vp, gp, i32, i32, vp]),
+    "mesh_su2_momenta": (i32, [vp, gp, i32, vp, vp, u64, u32]),
+    "mesh_su2_force_update": (i32, [vp, gp, i32, vp, vp, f64, f64]),
+    "mesh_su2_link_update": (i32, [vp, gp, i32, i32, vp, vp, vp, f64, i32]),
+    "mesh_su2_reunit": (i32, [vp, gp, i32, i32, vp]),
+    "mesh_su2_plaquette": (i32, [vp, gp, i32, vp, dp]),
+    "mesh_su2_delta_h": (i32, [vp, gp, i32, vp, vp, vp, vp, f64, dp]),
+    "mesh_su2_hmc_trajectory": (i32, [vp, gp, i32, C.POINTER(vp), C.POINTER(vp), vp, vp, f64, dp, i32, u64, u32,
+                                      i32, dp]),
+    "mesh_su2_landau_functional": (i32, [vp, gp, i32, vp, vp, dp]),
+    "mesh_su2_landau_divsum": (i32, [vp, gp, i32, vp, vp, dp]),
+    "mesh_su2_landau_deriv": (i32, [vp, gp, i32, vp, vp, vp]),
+    "mesh_su2_gauge_transform": (i32, [vp, gp, i32, vp, vp, vp]),
+    "mesh_su2_landau_or_halfsweep": (i32, [vp, gp, i32, vp, vp, i32, f64]),
+    "mesh_su2_dot": (i32, [vp, gp, i32, i32, vp, vp, dp]),
+    "mesh_su2_axpy": (i32, [vp, gp, i32, i32, vp, f64, vp]),
+    "mesh_u1_landau_functional": (i32, [vp, gp, vp, vp, dp]),
+    "mesh_u1_landau_divsum": (i32, [vp, gp, vp, vp, dp]),
+    "mesh_u1_landau_deriv": (i32, [vp, gp, vp, vp, vp]),
+    "mesh_u1_gauge_transform": (i32, [vp, gp, vp, vp, vp]),
+    "mesh_u1_landau_or_halfsweep": (i32, [vp, gp, vp, vp, i32, f64]),
+    "mesh_u1_gauge_step": (i32, [vp, gp, vp, vp, vp, f64]),
+    "mesh_u1_dot": (i32, [vp, C.c_long, vp, vp, dp]),
+    "mesh_u1_axpy": (i32, [vp, C.c_long, vp, f64, vp]),
     "mesh_su3_wilson_action": (i32, [vp, gp, i32, vp, f64, dp]),
     "mesh_su3_norm2": (i32, [vp, gp, i32, i32, vp, dp]),
     "mesh_su3_sum_real_trace_prod": (i32, [vp, gp, i32, i32, vp, vp, dp]),

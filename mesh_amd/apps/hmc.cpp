@@ -33,7 +33,7 @@ namespace {
 [[noreturn]] void usage(char const *msg)
 {
 	std::fprintf(stderr,
-	             "error: %s\nusage: mesh_hmc [--group su3|u1] [--geom L0 L1 [L2 [L3]]] [--beta B]\n"
+	             "error: %s\nusage: mesh_hmc [--group su3|su2|u1] [--geom L0 L1 [L2 [L3]]] [--beta B]\n"
 	             "                [--start random|cold|<ensemble>/configs/<n>] [--seed S] [--count N]\n"
 	             "                [--scheme 2lf|2mn|4mn] [--epsilon E] [--substeps N] [--precision 1|2]\n"
 	             "                [--filename <ensemble> [--force] [--spacing K]] [--therm K] [--landau]\n"
@@ -181,12 +181,7 @@ int main(int argc, char **argv)
 				    ens.write_data("deltaH_history", hmc.deltaH_history);
 				    ens.write_data("time_history", hmc.time_history);
 			    }
-			    if constexpr (G::Nc() == 1)
-			    {
-				    if (landau)
-					    throw std::runtime_error("--landau needs an SU(3) field");
-			    }
-			    else if (landau)
+			    if (landau)
 			    {
 				    Landau<G> lan(hmc.U);
 				    lan.verbose = true;

@@ -70,34 +70,27 @@ int main(int argc, char **argv)
 		Grid g(configGeometry(configName));
 		dispatchByGroup(
 		    [&]<typename G>() {
-			    if constexpr (G::Nc() == 1)
-				    throw std::runtime_error("Landau gauge fixing is implemented for SU(3)");
+			    typename FieldOf<G>::type U(g);
+			    readConfig(configName, U);
+			    Landau<G> lan(U);
+			    lan.verbose = true;
+			    const double F0 = lan(), theta0 = lan.condition() * g.ndim();
+			    std::printf("start: functional = %.12f, theta = %.3e\n", F0, theta0);
+			    if (ascent)
+			    {
+				    int it = lan.run_ascent(max_iter);
+				    std::printf("landau ascent: %d iterations, condition = %.3e, functional = %.12f\n", it,
+				                lan.condition(), lan());
+			    }
 			    else
 			    {
-				    GaugeField<G> U(g);
-				    readConfig(configName, U);
-				    Landau<G> lan(U);
-				    lan.verbose = true;
-				    const double F0 = lan(), theta0 = lan.condition() * g.ndim();
-				    std::printf("start: functional = %.12f, theta = %.3e\n", F0, theta0);
-				    if (ascent)
-				    {
-					    int it = lan.run_ascent(max_iter);
-					    std::printf("landau ascent: %d iterations, condition = %.3e, functional = %.12f\n", it,
-					                lan.condition(), lan());
-				    }
-				    else
-				    {
-					    int it = lan.run(max_iter, omega);
-					    std::printf("landau: %d sweeps, theta = %.3e, functional = %.12f\n", it, lan.last_theta,
-					                lan());
-				    }
-				    // the result as a gauge transformation: g back on SU(3), U^g rebuilt
-				    reunitize(lan.g);
-				    std::printf("reunitarised g: theta = %.3e, functional = %.12f\n", lan.condition() * g.ndim(),
-				                lan());
-				    std::printf("plaquette: before %.15f, after %.15f\n", plaquette(U), plaquette(lan.gauge_fixed()));
+				    int it = lan.run(max_iter, omega);
+				    std::printf("landau: %d sweeps, theta = %.3e, functional = %.12f\n", it, lan.last_theta, lan());
 			    }
+			    // the result as a gauge transformation: g back on the group, U^g rebuilt
+			    lan.reunit(lan.g);
+			    std::printf("reunitarised g: theta = %.3e, functional = %.12f\n", lan.condition() * g.ndim(), lan());
+			    std::printf("plaquette: before %.15f, after %.15f\n", plaquette(U), plaquette(lan.gauge_fixed()));
 		    },
 		    group, precision);
 	}

@@ -588,9 +588,257 @@ int u1_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, U1Geo *ug)
 	return MESH_OK;
 }
 
+// ---- Landau gauge (src/gauge/landau.h for U1<T>) on angles: the gauge
+// field g(x) = exp(i phi(x)); every value in FP64 from the FP32 angles and
+// rounded once
+
+__device__ __forceinline__ double wrap_pi_d(double a)
+{
+	// to [-pi, pi)
+	const double tp = 6.283185307179586;
+	a = a - tp * floor((a + 3.141592653589793) * (1.0 / tp));
+	return a >= 3.141592653589793 ? a - tp : (a < -3.141592653589793 ? a + tp : a);
+}
+
+// angle of g(x) U_mu(x) g(x+mu)^*: (phi(x) + theta_mu(x)) - phi(x+mu)
+__device__ __forceinline__ double u1_tang(U1NGeo const &g, const float *th, const float *ph, int mu, int x)
+{
+	const double a = (double)th[mu * g.V + x];
+	return ph ? ((double)ph[x] + a) - (double)ph[n_fwd(g, x, mu)] : a;
+}
+
+// Im(g(x) d(x)) = sum_mu sin(angle at (mu, x)) - sin(angle at (mu, x - mu)) (landau.h:41-48)
+__device__ __forceinline__ double u1_lgd(U1NGeo const &g, const float *th, const float *ph, int x)
+{
+	double s = 0.0;
+	for (int mu = 0; mu < g.nd; ++mu)
+	{
+		s += sin(u1_tang(g, th, ph, mu, x));
+		s -= sin(u1_tang(g, th, ph, mu, n_bwd(g, x, mu)));
+	}
+	return s;
+}
+
+// per site: mode 0 sum_mu cos(angle) (the functional), mode 1 Im(g d)^2 (the condition)
+__global__ __launch_bounds__(256) void k_u1_lsum(U1NGeo g, const float *__restrict__ th,
+                                                 const float *__restrict__ ph, int mode, double *partials, int nb)
+{
+	int x = blockIdx.x * blockDim.x + threadIdx.x;
+	double v[1] = {0.0};
+	if (x < g.V)
+	{
+		if (mode == 0)
+			for (int mu = 0; mu < g.nd; ++mu)
+				v[0] += cos(u1_tang(g, th, ph, mu, x));
+		else
+		{
+			double s = u1_lgd(g, th, ph, x);
+			v[0] = s * s;
+		}
+	}
+	block_sum_store<1>(v, partials, nb, blockIdx.x);
+}
+
+// F(x) = c Im(g d), c = -0.5 / (Nd V) (landau.h:52-65; the algebra element i F)
+__global__ __launch_bounds__(256) void k_u1_lderiv(U1NGeo g, const float *__restrict__ th,
+                                                   const float *__restrict__ ph, double c, float *__restrict__ F)
+{
+	int x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x < g.V)
+		F[x] = (float)(c * u1_lgd(g, th, ph, x));
+}
+
+// theta^g_mu(x) = wrap(phi(x) + theta_mu(x) - phi(x+mu))
+__global__ __launch_bounds__(256) void k_u1_gauge_transform(U1NGeo g, float *__restrict__ tho,
+                                                            const float *__restrict__ th,
+                                                            const float *__restrict__ ph)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= g.nd * g.V)
+		return;
+	int mu = i / g.V;
+	tho[i] = (float)wrap_pi_d(u1_tang(g, th, ph, mu, i - mu * g.V));
+}
+
+// overrelaxation of one parity on the transformed angles: K(x) = sum_mu
+// exp(i theta_mu(x)) + exp(-i theta_mu(x - mu)), the local maximum of
+// Re(exp(i a) K) is a = -arg K, the step is omega times that angle
+__global__ __launch_bounds__(256) void k_u1_or_half(U1NGeo g, float *th, float *ph, int parity, double omega)
+{
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (g.V >> 1))
+		return;
+	const int f = g.nd - 1, hl = g.L[f] >> 1;
+	const int row = t / hl;
+	const int x0 = row * g.L[f];
+	int ps = 0;
+	for (int mu = 0; mu < f; ++mu)
+		ps += (x0 / g.st[mu]) % g.L[mu];
+	const int x = x0 + 2 * (t - row * hl) + ((parity + ps) & 1);
+	double kr = 0.0, ki = 0.0;
+	for (int mu = 0; mu < g.nd; ++mu)
+	{
+		const double a = (double)th[mu * g.V + x], b = (double)th[mu * g.V + n_bwd(g, x, mu)];
+		kr += cos(a);
+		ki += sin(a);
+		kr += cos(b);
+		ki -= sin(b);
+	}
+	if (kr == 0.0 && ki == 0.0)
+		return;
+	const double al = -omega * atan2(ki, kr);
+	for (int mu = 0; mu < g.nd; ++mu)
+	{
+		const int xm = n_bwd(g, x, mu);
+		th[mu * g.V + x] = (float)wrap_pi_d((double)th[mu * g.V + x] + al);
+		th[mu * g.V + xm] = (float)wrap_pi_d((double)th[mu * g.V + xm] - al);
+	}
+	if (ph)
+		ph[x] = (float)wrap_pi_d((double)ph[x] + al);
+}
+
+// phi_out = wrap(phi_in + t F): exp(t i F) g (landau.h:85)
+__global__ __launch_bounds__(256) void k_u1_gauge_step(int n, float *out, const float *in, const float *__restrict__ F,
+                                                       double t)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+	{
+		const double d = t * (double)F[i];
+		out[i] = (float)wrap_pi_d(d + (double)in[i]);
+	}
+}
+
+// sum Re tr (i a i b) = -sum a b, in FP64
+__global__ __launch_bounds__(256) void k_u1_dot(long n, const float *__restrict__ A, const float *__restrict__ B,
+                                                double *partials, int nb)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	double v[1] = {i < n ? -((double)A[i] * (double)B[i]) : 0.0};
+	block_sum_store<1>(v, partials, nb, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_u1_axpy(long n, float *Y, float a, const float *__restrict__ X)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		Y[i] = fmaf(a, X[i], Y[i]);
+}
+
+int u1_lsum(mesh_ctx *ctx, const mesh_grid *g, const float *th, const float *ph, int mode, double *out)
+{
+	U1NGeo ng;
+	if (int rc = u1n_setup(ctx, g, 1, &ng))
+		return rc;
+	if (!th || !out)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to the U(1) Landau sums");
+	const int nb = (ng.V + 255) / 256;
+	if (int rc = mesh_reserve_partials(ctx, nb))
+		return rc;
+	k_u1_lsum<<<nb, 256, 0, ctx->stream>>>(ng, th, ph, mode, ctx->partials, nb);
+	MESH_LAUNCHED(ctx);
+	double s;
+	if (int rc = mesh_finish_reduction(ctx, nb, 1, &s))
+		return rc;
+	*out = mode == 0 ? s / ((double)ng.nd * ng.V) : s;
+	return MESH_OK;
+}
+
 } // namespace
 
 extern "C" {
+
+int mesh_u1_landau_functional(mesh_ctx *ctx, const mesh_grid *g, const float *theta, const float *phi, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	return u1_lsum(ctx, g, theta, phi, 0, out);
+}
+
+int mesh_u1_landau_divsum(mesh_ctx *ctx, const mesh_grid *g, const float *theta, const float *phi, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	return u1_lsum(ctx, g, theta, phi, 1, out);
+}
+
+int mesh_u1_landau_deriv(mesh_ctx *ctx, const mesh_grid *g, const float *theta, const float *phi, float *F)
+{
+	MESH_CHECK_CTX(ctx);
+	U1NGeo ng;
+	if (int rc = u1n_setup(ctx, g, 1, &ng))
+		return rc;
+	if (!theta || !F || F == theta || F == phi)
+		return mesh_fail(ctx, MESH_E_ARG, "bad u1_landau_deriv arguments");
+	k_u1_lderiv<<<(ng.V + 255) / 256, 256, 0, ctx->stream>>>(ng, theta, phi, -0.5 / ((double)ng.nd * ng.V), F);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_u1_gauge_transform(mesh_ctx *ctx, const mesh_grid *g, float *theta_out, const float *theta, const float *phi)
+{
+	MESH_CHECK_CTX(ctx);
+	U1NGeo ng;
+	if (int rc = u1n_setup(ctx, g, 1, &ng))
+		return rc;
+	if (!theta_out || !theta || !phi || theta_out == theta || theta_out == phi)
+		return mesh_fail(ctx, MESH_E_ARG, "bad u1_gauge_transform arguments");
+	k_u1_gauge_transform<<<(ng.nd * ng.V + 255) / 256, 256, 0, ctx->stream>>>(ng, theta_out, theta, phi);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_u1_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, float *theta, float *phi, int parity,
+                                double omega)
+{
+	MESH_CHECK_CTX(ctx);
+	U1NGeo ng;
+	if (int rc = u1n_setup(ctx, g, 1, &ng))
+		return rc;
+	if (!theta || theta == phi || (parity != 0 && parity != 1))
+		return mesh_fail(ctx, MESH_E_ARG, "bad u1_landau_or_halfsweep arguments");
+	for (int i = 0; i < ng.nd; ++i)
+		if (ng.L[i] % 2)
+			return mesh_fail(ctx, MESH_E_SIZE, "checkerboard update needs even extents");
+	k_u1_or_half<<<(ng.V / 2 + 255) / 256, 256, 0, ctx->stream>>>(ng, theta, phi, parity, omega);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_u1_gauge_step(mesh_ctx *ctx, const mesh_grid *g, float *phi_out, const float *phi_in, const float *F,
+                       double t)
+{
+	MESH_CHECK_CTX(ctx);
+	U1NGeo ng;
+	if (int rc = u1n_setup(ctx, g, 1, &ng))
+		return rc;
+	if (!phi_out || !phi_in || !F)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to u1_gauge_step");
+	k_u1_gauge_step<<<(ng.V + 255) / 256, 256, 0, ctx->stream>>>(ng.V, phi_out, phi_in, F, t);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_u1_dot(mesh_ctx *ctx, long n, const float *A, const float *B, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	if (n < 1 || n >= (1L << 31) || !A || !B || !out)
+		return mesh_fail(ctx, MESH_E_ARG, "bad u1_dot arguments");
+	const int nb = (int)((n + 255) / 256);
+	if (int rc = mesh_reserve_partials(ctx, nb))
+		return rc;
+	k_u1_dot<<<nb, 256, 0, ctx->stream>>>(n, A, B, ctx->partials, nb);
+	MESH_LAUNCHED(ctx);
+	return mesh_finish_reduction(ctx, nb, 1, out);
+}
+
+int mesh_u1_axpy(mesh_ctx *ctx, long n, float *Y, double a, const float *X)
+{
+	MESH_CHECK_CTX(ctx);
+	if (n < 1 || n >= (1L << 31) || !Y || !X)
+		return mesh_fail(ctx, MESH_E_ARG, "bad u1_axpy arguments");
+	k_u1_axpy<<<(int)((n + 255) / 256), 256, 0, ctx->stream>>>(n, Y, (float)a, X);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
 
 int mesh_u1_random(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, uint64_t seed)
 {

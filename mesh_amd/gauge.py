@@ -235,71 +235,46 @@ class Hmc:
         print(f"<exp(-dH)> = {s['exp_mdH']:.4f}")
 
 
-class Landau:
-    """Landau gauge fixing of a gauge field (src/gauge/landau.h:9-136).
-
-    Holds the gauge transformation ``g`` (identity after ``reset``, as
-    landau.h:14) and, for the overrelaxation solver, the transformed links
-    ``Ug`` = U^g.  ``run`` replaces the reference's line-search ascent by
-    even/odd overrelaxation and stops at theta = (1/V) sum|div A|^2 < tol."""
-
-    def __init__(self, U):
-        self.U = U
-        self.g = Lattice(U.grid, U.prec, U.ctx)
-        self.verbose = False
-        self.reset()
-
-    def reset(self):
-        self.g.fill_unit()
-
-    def __call__(self, h=None):
-        """functional F[h] (landau.h:23-34); default h = g."""
-        h = self.g if h is None else h
-        o = _out()
-        self.U.ctx.call("mesh_landau_functional", self.U.grid.ref, self.U.prec, self.U.ptr, h.ptr, o)
-        return o[0]
-
-    def divsum(self, Ug=None):
-        o = _out()
-        if Ug is None:
-            self.U.ctx.call("mesh_landau_divsum", self.U.grid.ref, self.U.prec, self.U.ptr, self.g.ptr, o)
-        else:
-            Ug.ctx.call("mesh_landau_divsum", Ug.grid.ref, Ug.prec, Ug.ptr, None, o)
-        return o[0]
+class LandauSolver:
+    """The solvers of src/gauge/landau.h:67-136 (and the overrelaxation that
+    replaces its ascent) over a group's field operations, which a subclass
+    supplies: ``g`` (the gauge transformation), ``__call__(h)`` (the
+    functional), ``divsum()``, ``deriv()``, ``_links()`` (a new field shaped
+    like U), ``_transform_into(Ug)``, ``_or_half(Ug, parity, omega)``,
+    ``_divsum_of(Ug)``, ``_clone(h)``, ``_assign(dst, src)``,
+    ``_step(h, F, e)`` (h = exp(e F) h), ``_reunit(h)``, ``_dot(A, B)``
+    (sum Re tr A B) and ``_axpy(Y, a, X)``."""
 
     def condition(self):
-        """(1/(Nd V)) sum_x ||TA(g d)||^2 (landau.h:37-49)."""
+        """(1/(Nd V)) sum_x ||project_on_algebra(g d)||^2 (landau.h:37-49)."""
         return self.divsum() / (self.U.grid.ndim() * self.U.grid.size())
 
     def theta(self):
         """(1/V) sum_x |div A(x)|^2 = Nd * condition()."""
         return self.divsum() / self.U.grid.size()
 
-    def deriv(self):
-        """-0.5/(Nc Nd V) TA(g d) (landau.h:52-65)."""
-        F = Lattice(self.U.grid, self.U.prec, self.U.ctx)
-        self.U.ctx.call("mesh_landau_deriv", self.U.grid.ref, self.U.prec, self.U.ptr, self.g.ptr, F.ptr)
-        return F
+    def gauge_fixed(self):
+        """U^g_mu(x) = g(x) U_mu(x) g(x+mu)^+"""
+        Ug = self._links()
+        self._transform_into(Ug)
+        return Ug
 
     def run(self, max_iter, omega=1.7, tol=1e-12, check_every=10, reunit_every=100):
-        """Overrelaxation sweeps until theta < tol; returns (sweeps, theta)."""
-        ctx, grid = self.U.ctx, self.U.grid
-        Ug = Field(grid, grid.ndim(), self.U.prec, ctx)
-        ctx.call("mesh_su3_gauge_transform", grid.ref, self.U.prec, Ug.ptr, self.U.ptr, self.g.ptr)
+        """Overrelaxation sweeps until theta < tol; returns (sweeps, theta).
+        The sweeps update U^g and g together; every reunit_every sweeps g is
+        reunitarised and U^g rebuilt from it."""
+        Ug = self.gauge_fixed()
         theta = math.inf
         it = 0
         while it < max_iter:
             for _ in range(check_every):
                 for parity in (0, 1):
-                    ctx.call("mesh_landau_or_halfsweep", grid.ref, self.U.prec, Ug.ptr, self.g.ptr, parity,
-                             _d(omega))
+                    self._or_half(Ug, parity, omega)
                 it += 1
                 if reunit_every and it % reunit_every == 0:
-                    reunitize(self.g)
-                    ctx.call("mesh_su3_gauge_transform", grid.ref, self.U.prec, Ug.ptr, self.U.ptr, self.g.ptr)
-            o = _out()
-            ctx.call("mesh_landau_divsum", grid.ref, self.U.prec, Ug.ptr, None, o)
-            theta = o[0] / grid.size()
+                    self._reunit(self.g)
+                    self._transform_into(Ug)
+            theta = self._divsum_of(Ug) / self.U.grid.size()
             if self.verbose:
                 print(f"iter = {it}, theta = {theta}")
             if theta < tol:
@@ -318,11 +293,12 @@ class Landau:
         of exp(eps F) g exceeds that of g, try 1.2 eps, else the vertex of the
         parabola through eps, 1.2 eps and 0.8 eps."""
         base = self()
-        h = self.g.clone()
+        h = self._clone(self.g)
 
         def trial(e):
-            h.copy_from(self.g)
-            return self(evolve(h, F, e))
+            self._assign(h, self.g)
+            self._step(h, F, e)
+            return self(h)
 
         y1 = trial(eps)
         while y1 <= base:
@@ -354,14 +330,87 @@ class Landau:
             if cond < tol:
                 return it, cond
             F = self.deriv()
-            norm = sum_real_trace_prod(F, F)
+            norm = self._dot(F, F)
             if it > 20:
-                beta = (norm - sum_real_trace_prod(F, last)) / last_norm
-            last = F.clone()
+                beta = (norm - self._dot(F, last)) / last_norm
+            last = self._clone(F)
             if it > 20 and beta > 0:
-                add_mul(F, beta, last)
+                self._axpy(F, beta, last)
             eps = self.line_search(F, eps)
-            evolve(self.g, F, eps)
-            reunitize(self.g)
+            self._step(self.g, F, eps)
+            self._reunit(self.g)
             last_norm = norm
         return max_iter, cond
+
+
+class Landau(LandauSolver):
+    """Landau gauge fixing of an SU(3) gauge field (src/gauge/landau.h:9-136).
+
+    Holds the gauge transformation ``g`` (identity after ``reset``, as
+    landau.h:14) and, for the overrelaxation solver, the transformed links
+    ``Ug`` = U^g.  ``run`` replaces the reference's line-search ascent by
+    even/odd overrelaxation and stops at theta = (1/V) sum|div A|^2 < tol."""
+
+    def __init__(self, U):
+        self.U = U
+        self.g = Lattice(U.grid, U.prec, U.ctx)
+        self.verbose = False
+        self.reset()
+
+    def reset(self):
+        self.g.fill_unit()
+
+    def __call__(self, h=None):
+        """functional F[h] (landau.h:23-34); default h = g."""
+        h = self.g if h is None else h
+        o = _out()
+        self.U.ctx.call("mesh_landau_functional", self.U.grid.ref, self.U.prec, self.U.ptr, h.ptr, o)
+        return o[0]
+
+    def divsum(self, Ug=None):
+        if Ug is not None:
+            return self._divsum_of(Ug)
+        o = _out()
+        self.U.ctx.call("mesh_landau_divsum", self.U.grid.ref, self.U.prec, self.U.ptr, self.g.ptr, o)
+        return o[0]
+
+    def deriv(self):
+        """-0.5/(Nc Nd V) TA(g d) (landau.h:52-65)."""
+        F = Lattice(self.U.grid, self.U.prec, self.U.ctx)
+        self.U.ctx.call("mesh_landau_deriv", self.U.grid.ref, self.U.prec, self.U.ptr, self.g.ptr, F.ptr)
+        return F
+
+    def _links(self):
+        return Field(self.U.grid, self.U.grid.ndim(), self.U.prec, self.U.ctx)
+
+    def _transform_into(self, Ug):
+        self.U.ctx.call("mesh_su3_gauge_transform", self.U.grid.ref, self.U.prec, Ug.ptr, self.U.ptr, self.g.ptr)
+
+    def _or_half(self, Ug, parity, omega):
+        self.U.ctx.call("mesh_landau_or_halfsweep", self.U.grid.ref, self.U.prec, Ug.ptr, self.g.ptr, parity,
+                        _d(omega))
+
+    def _divsum_of(self, Ug):
+        o = _out()
+        Ug.ctx.call("mesh_landau_divsum", Ug.grid.ref, Ug.prec, Ug.ptr, None, o)
+        return o[0]
+
+    def _clone(self, h):
+        return h.clone()
+
+    def _assign(self, dst, src):
+        dst.copy_from(src)
+
+    def _step(self, h, F, e):
+        evolve(h, F, e)
+
+    def _reunit(self, h):
+        reunitize(h)
+
+    def _dot(self, A, B):
+        return sum_real_trace_prod(A, B)
+
+    def _axpy(self, Y, a, X):
+        add_mul(Y, a, X)
+
+

@@ -265,15 +265,18 @@ template <class G> class Hmc
 	}
 };
 
-// U(1) links as angles on a 2- to 4-d lattice: theta[mu][x], FP32 (the
-// reference's GaugeField<U1<T>> of complex<T> links exp(i theta))
+// U(1) links as angles on a 2- to 4-d lattice: theta[f][x] for nf
+// directions (nf = Nd: the gauge field, nf = 1: a gauge transformation
+// g = exp(i phi)), FP32 (the reference's GaugeField<U1<T>> of complex<T>
+// links exp(i theta))
 class U1Field
 {
 	Grid const *g_;
+	int nf_;
 	void *p_ = nullptr;
 
   public:
-	explicit U1Field(Grid const &g) : g_(&g)
+	explicit U1Field(Grid const &g, int nf = -1) : g_(&g), nf_(nf < 0 ? g.ndim() : nf)
 	{
 		if (g.ndim() < 2 || g.ndim() > 4)
 			throw std::runtime_error("U(1) fields are implemented on lattices of 2 to 4 dimensions");
@@ -286,18 +289,46 @@ class U1Field
 	}
 	U1Field(U1Field const &) = delete;
 	U1Field &operator=(U1Field const &) = delete;
+	U1Field(U1Field &&o) noexcept : g_(o.g_), nf_(o.nf_), p_(o.p_) { o.p_ = nullptr; }
 	Grid const &grid() const { return *g_; }
-	size_t bytes() const { return g_->ndim() * g_->size() * sizeof(float); }
+	int nf() const { return nf_; }
+	size_t bytes() const { return (size_t)nf_ * g_->size() * sizeof(float); }
 	float *data() const { return (float *)p_; }
 	// theta = 0: the unit configuration
 	void fill_unit() { ctx().check(mesh_memclear(ctx().get(), p_, bytes())); }
+	void copy_from(U1Field const &o)
+	{
+		if (o.bytes() != bytes())
+			throw std::runtime_error("U1Field::copy_from: size mismatch");
+		ctx().check(mesh_memcpy(ctx().get(), p_, o.p_, bytes()));
+	}
 	std::vector<float> download() const
 	{
-		std::vector<float> h(g_->ndim() * g_->size());
+		std::vector<float> h((size_t)nf_ * g_->size());
 		ctx().check(mesh_memcpy_to_host(ctx().get(), h.data(), p_, bytes()));
 		return h;
 	}
+	void upload(std::span<const float> h)
+	{
+		if (h.size() * sizeof(float) != bytes())
+			throw std::runtime_error("U1Field::upload: size mismatch");
+		ctx().check(mesh_memcpy_from_host(ctx().get(), p_, h.data(), bytes()));
+	}
 };
+
+// mean cos theta_P (= Re tr P / Nc; plaquette, src/gauge/utils.h:93-108)
+inline double plaquette(U1Field const &U)
+{
+	double *d = nullptr, h[4];
+	ctx().check(mesh_malloc(ctx().get(), (void **)&d, sizeof h));
+	int rc = mesh_u1_observables(ctx().get(), U.grid().c(), 1, U.data(), d);
+	if (rc == MESH_OK)
+		rc = mesh_memcpy_to_host(ctx().get(), h, d, sizeof h);
+	mesh_free(ctx().get(), d);
+	ctx().check(rc);
+	const int nd = U.grid().ndim();
+	return h[2] / ((double)U.grid().size() * nd * (nd - 1) / 2);
+}
 
 // HMC for U(1) (Hmc<U1<T>>, src/gauge/hmc.cpp:109-110) through mesh_u1_hmc
 // with a batch of one: on small 2-d lattices the whole trajectory is one
@@ -360,10 +391,249 @@ template <class T> class Hmc<U1<T>>
 	}
 };
 
+// SU(2) links as quaternions in the library's SoA layout (4 planes per
+// direction); FP64 or FP32 after T
+template <class T> class SU2Field
+{
+	Grid const *g_;
+	int nf_;
+	void *p_ = nullptr;
+
+  public:
+	static constexpr int prec = sizeof(T) == 8 ? MESH_F64 : MESH_F32;
+	explicit SU2Field(Grid const &g, int nf = -1) : g_(&g), nf_(nf < 0 ? g.ndim() : nf)
+	{
+		if (g.ndim() < 2 || g.ndim() > 4)
+			throw std::runtime_error("SU(2) fields are implemented on lattices of 2 to 4 dimensions");
+		ctx().check(mesh_malloc(ctx().get(), &p_, bytes()));
+	}
+	~SU2Field()
+	{
+		if (p_)
+			mesh_free(ctx().get(), p_);
+	}
+	SU2Field(SU2Field const &) = delete;
+	SU2Field &operator=(SU2Field const &) = delete;
+	SU2Field(SU2Field &&o) noexcept : g_(o.g_), nf_(o.nf_), p_(o.p_) { o.p_ = nullptr; }
+	void copy_from(SU2Field const &o)
+	{
+		if (o.bytes() != bytes())
+			throw std::runtime_error("SU2Field::copy_from: size mismatch");
+		ctx().check(mesh_memcpy(ctx().get(), p_, o.p_, bytes()));
+	}
+	friend void swap(SU2Field &a, SU2Field &b) noexcept
+	{
+		std::swap(a.g_, b.g_);
+		std::swap(a.nf_, b.nf_);
+		std::swap(a.p_, b.p_);
+	}
+	Grid const &grid() const { return *g_; }
+	int nf() const { return nf_; }
+	size_t bytes() const { return mesh_su2_field_bytes(g_->c(), nf_, prec); }
+	void *data() const { return p_; }
+	void fill_unit() { ctx().check(mesh_su2_unit(ctx().get(), g_->c(), nf_, prec, p_)); }
+	// host layout [nf][V][4] (w, x, y, z)
+	std::vector<T> download() const
+	{
+		std::vector<T> h((size_t)nf_ * g_->size() * 4);
+		ctx().check(mesh_su2_transfer(ctx().get(), g_->c(), nf_, prec, h.data(), p_, 0));
+		return h;
+	}
+	void upload(std::span<const T> h)
+	{
+		if (h.size() != (size_t)nf_ * g_->size() * 4)
+			throw std::runtime_error("SU2Field::upload: size mismatch");
+		ctx().check(mesh_su2_transfer(ctx().get(), g_->c(), nf_, prec, (void *)h.data(), p_, 1));
+	}
+};
+
+// mean Re tr P / 2 (plaquette, src/gauge/utils.h:93-108)
+template <class T> double plaquette(SU2Field<T> const &U)
+{
+	double r;
+	ctx().check(mesh_su2_plaquette(ctx().get(), U.grid().c(), SU2Field<T>::prec, U.data(), &r));
+	return r;
+}
+
+// HMC for SU(2) (Hmc<SU2<T>>, src/gauge/hmc.cpp:111-112) through
+// mesh_su2_hmc_trajectory.  The kinetic energy is the 2x2 matrix norm of the
+// momenta, 2 (x^2 + y^2 + z^2): the norm that the force
+// (beta / 4) project_on_algebra(U Sigma) conserves.
+template <class T> class Hmc<SU2<T>>
+{
+  public:
+	Grid const &g;
+	SU2Field<T> U, P, U_new;
+	uint64_t seed = 0;
+	uint32_t traj = 0;
+	std::vector<double> plaq_history, deltaH_history, accept_history, time_history;
+
+  private:
+	void *K_ = nullptr;
+
+  public:
+	explicit Hmc(Grid const &g_) : g(g_), U(g_), P(g_), U_new(g_)
+	{
+		ctx().check(mesh_malloc(ctx().get(), &K_, g_.size() * sizeof(double)));
+	}
+	~Hmc()
+	{
+		if (K_)
+			mesh_free(ctx().get(), K_);
+	}
+	Hmc(Hmc const &) = delete;
+	Hmc &operator=(Hmc const &) = delete;
+
+	void reset_observables()
+	{
+		plaq_history.clear();
+		deltaH_history.clear();
+		accept_history.clear();
+		time_history.clear();
+	}
+	void randomizeGaugeField()
+	{
+		ctx().check(mesh_su2_random(ctx().get(), g.c(), g.ndim(), SU2Field<T>::prec, U.data(), seed));
+	}
+
+	void runHmcUpdate(double beta, std::vector<double> const &deltas, bool metropolis = true)
+	{
+		auto t0 = std::chrono::steady_clock::now();
+		void *pu = U.data(), *pw = U_new.data();
+		double out[6];
+		ctx().check(mesh_su2_hmc_trajectory(ctx().get(), g.c(), SU2Field<T>::prec, &pu, &pw, P.data(), (double *)K_,
+		                                    beta, deltas.data(), (int)deltas.size(), seed, traj,
+		                                    metropolis ? 1 : 0, out));
+		if (pu != U.data())
+			swap(U, U_new);
+		++traj;
+		double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		accept_history.push_back(out[1]);
+		plaq_history.push_back(out[2]);
+		deltaH_history.push_back(out[0]);
+		time_history.push_back(dt);
+	}
+
+	void print_summary() const
+	{
+		detail::print_hmc_summary(plaq_history, deltaH_history, accept_history, time_history);
+	}
+};
+
 // ---- Landau gauge (src/gauge/landau.h) ----------------------------------------
 
-template <class G> struct Landau
+namespace detail {
+// The reference's solver (landau.h:67-136) over the field operations of a
+// group, supplied by D: the gauge field `g` (type D::gauge_type), the
+// functional F[h] as operator()(h), condition(), deriv() (D::deriv_type),
+// dot(A, B) = sum Re tr (A B), axpy(Y, a, X), assign(dst, src),
+// step(h, F, e): h = exp(e F) h, and reunit(h).
+template <class D> struct LandauAscent
 {
+	// abscissa of the vertex of the parabola through three points (landau.h:68-75)
+	static double solve_quadratic(double x1, double x2, double x3, double y1, double y2, double y3)
+	{
+		return 0.5 * ((x2 * x2 - x3 * x3) * y1 - (x1 * x1 - x3 * x3) * y2 + (x1 * x1 - x2 * x2) * y3) /
+		       ((x2 - x3) * y1 - (x1 - x3) * y2 + (x1 - x2) * y3);
+	}
+
+	// step length along F (landau.h:77-98)
+	template <class Fd> double line_search(Fd const &F, double eps) const
+	{
+		D const &d = static_cast<D const &>(*this);
+		double base = d();
+		typename D::gauge_type h = d.new_gauge();
+		auto trial = [&](double e) {
+			d.assign(h, d.g);
+			d.step(h, F, e);
+			return d(h);
+		};
+		double y1 = trial(eps);
+		while (y1 <= base)
+		{
+			eps *= 0.5;
+			y1 = trial(eps);
+			if (eps < 1e-10)
+				return 0.0;
+		}
+		double y2 = trial(eps * 1.2);
+		if (y2 > y1)
+			return 1.2 * eps;
+		double y3 = trial(eps * 0.8);
+		return solve_quadratic(eps, 1.2 * eps, 0.8 * eps, y1, y2, y3);
+	}
+
+	// ascent along deriv() with the line search, Polak-Ribiere steps after
+	// iteration 20 (as written in landau.h:126-129, `last = F` precedes
+	// `F += last * beta`, so the step is (1 + beta) F), then
+	// g = reunitize(exp(eps F) g), until condition() < tol; returns iterations
+	int run_ascent(int max_iter, double tol = 1e-10)
+	{
+		D &d = static_cast<D &>(*this);
+		double eps = 3000.0, lastNorm = 0.0;
+		typename D::deriv_type last = d.new_deriv();
+		for (int iter = 0; iter < max_iter; ++iter)
+		{
+			double cond = d.condition();
+			if (d.verbose)
+				std::printf("iter = %d, cond = %.6e, term = %.12f, eps = %.6e\n", iter + 1, cond, d(), eps);
+			if (cond < tol)
+				return iter;
+			typename D::deriv_type F = d.deriv();
+			double norm = d.dot(F, F);
+			double beta = iter > 20 ? (norm - d.dot(F, last)) / lastNorm : 0.0;
+			d.assign(last, F);
+			if (iter > 20 && beta > 0)
+				d.axpy(F, beta, last);
+			eps = line_search(F, eps);
+			d.step(d.g, F, eps);
+			d.reunit(d.g);
+			lastNorm = norm;
+		}
+		return max_iter;
+	}
+
+	// Even/odd overrelaxation until theta = Nd * condition() < tol
+	// (replaces the line-search ascent of landau.h:100-136); returns sweeps.
+	// The sweeps update the transformed links U^g and g together; every
+	// reunit_every sweeps g is reunitarised and U^g rebuilt from it, which
+	// bounds the drift of g from the group that FP32 products accumulate.
+	int run(int max_iter, double omega = 1.7, double tol = 1e-12, int check_every = 10, int reunit_every = 100)
+	{
+		D &d = static_cast<D &>(*this);
+		auto Ug = d.gauge_fixed();
+		int it = 0;
+		while (it < max_iter)
+		{
+			for (int k = 0; k < check_every && it < max_iter; ++k)
+			{
+				for (int parity = 0; parity < 2; ++parity)
+					d.or_halfsweep(Ug, parity, omega);
+				++it;
+				if (reunit_every > 0 && it % reunit_every == 0)
+				{
+					d.reunit(d.g);
+					d.transform_into(Ug);
+				}
+			}
+			last_theta = d.divsum_of(Ug) / (double)d.U.grid().size();
+			if (d.verbose)
+				std::printf("iter = %d, theta = %.3e\n", it, last_theta);
+			if (last_theta < tol)
+				break;
+		}
+		return it;
+	}
+
+	// theta of the transformed links at the last check of run()
+	double last_theta = 0;
+};
+} // namespace detail
+
+template <class G> struct Landau : detail::LandauAscent<Landau<G>>
+{
+	using gauge_type = Lattice<G>;
+	using deriv_type = Lattice<G>;
 	GaugeField<G> const &U;
 	Lattice<G> g;
 	bool verbose = false;
@@ -401,109 +671,193 @@ template <class G> struct Landau
 	GaugeField<G> gauge_fixed() const
 	{
 		GaugeField<G> Ug(U.grid());
-		ctx().check(mesh_su3_gauge_transform(ctx().get(), U.grid().c(), G::prec, Ug.data(), U.data(), g.data()));
+		transform_into(Ug);
 		return Ug;
 	}
 
-	// theta of the transformed links at the last check of run()
-	double last_theta = 0;
-
-	// Even/odd overrelaxation until theta = Nd * condition() < tol
-	// (replaces the line-search ascent of landau.h:100-136); returns sweeps.
-	// The sweeps update the transformed links U^g and g together; every
-	// reunit_every sweeps g is reunitarised and U^g rebuilt from it, which
-	// bounds the drift of g from SU(3) that FP32 products accumulate.
-	int run(int max_iter, double omega = 1.7, double tol = 1e-12, int check_every = 10, int reunit_every = 100)
+	// field operations of the solvers
+	Lattice<G> new_gauge() const { return Lattice<G>(U.grid()); }
+	Lattice<G> new_deriv() const { return Lattice<G>(U.grid()); }
+	void assign(Lattice<G> &dst, Lattice<G> const &src) const { dst = src; }
+	void step(Lattice<G> &h, Lattice<G> const &F, double e) const { evolve(h, F, e); }
+	void reunit(Lattice<G> &h) const { reunitize(h); }
+	double dot(Lattice<G> const &A, Lattice<G> const &B) const { return sum_real_trace_prod(A, B); }
+	void axpy(Lattice<G> &Y, double a, Lattice<G> const &X) const { add_mul(Y, a, X); }
+	void transform_into(GaugeField<G> &Ug) const
 	{
-		Context &c = ctx();
-		GaugeField<G> Ug(U.grid());
-		c.check(mesh_su3_gauge_transform(c.get(), U.grid().c(), G::prec, Ug.data(), U.data(), g.data()));
-		int it = 0;
-		while (it < max_iter)
-		{
-			for (int k = 0; k < check_every && it < max_iter; ++k)
-			{
-				for (int parity = 0; parity < 2; ++parity)
-					c.check(mesh_landau_or_halfsweep(c.get(), U.grid().c(), G::prec, Ug.data(), g.data(), parity,
-					                                 omega));
-				++it;
-				if (reunit_every > 0 && it % reunit_every == 0)
-				{
-					reunitize(g);
-					c.check(mesh_su3_gauge_transform(c.get(), U.grid().c(), G::prec, Ug.data(), U.data(), g.data()));
-				}
-			}
-			double d;
-			c.check(mesh_landau_divsum(c.get(), U.grid().c(), G::prec, Ug.data(), nullptr, &d));
-			last_theta = d / (double)U.grid().size();
-			if (verbose)
-				std::printf("iter = %d, theta = %.3e\n", it, last_theta);
-			if (last_theta < tol)
-				break;
-		}
-		return it;
+		ctx().check(mesh_su3_gauge_transform(ctx().get(), U.grid().c(), G::prec, Ug.data(), U.data(), g.data()));
+	}
+	void or_halfsweep(GaugeField<G> &Ug, int parity, double omega)
+	{
+		ctx().check(mesh_landau_or_halfsweep(ctx().get(), U.grid().c(), G::prec, Ug.data(), g.data(), parity, omega));
+	}
+	double divsum_of(GaugeField<G> const &Ug) const
+	{
+		double d;
+		ctx().check(mesh_landau_divsum(ctx().get(), U.grid().c(), G::prec, Ug.data(), nullptr, &d));
+		return d;
+	}
+};
+
+// Landau<SU2<T>>: g one quaternion per site
+template <class T> struct Landau<SU2<T>> : detail::LandauAscent<Landau<SU2<T>>>
+{
+	using F = SU2Field<T>;
+	using gauge_type = F;
+	using deriv_type = F;
+	static constexpr int prec = F::prec;
+	F const &U;
+	F g;
+	bool verbose = false;
+
+	void reset() { g.fill_unit(); }
+	explicit Landau(F const &U_) : U(U_), g(U_.grid(), 1) { reset(); }
+
+	double operator()(F const &h) const
+	{
+		double r;
+		ctx().check(mesh_su2_landau_functional(ctx().get(), U.grid().c(), prec, U.data(), h.data(), &r));
+		return r;
+	}
+	double operator()() const { return (*this)(g); }
+	double condition() const
+	{
+		double r;
+		ctx().check(mesh_su2_landau_divsum(ctx().get(), U.grid().c(), prec, U.data(), g.data(), &r));
+		return r / (U.grid().ndim() * (double)U.grid().size());
+	}
+	F deriv() const
+	{
+		F D(U.grid(), 1);
+		ctx().check(mesh_su2_landau_deriv(ctx().get(), U.grid().c(), prec, U.data(), g.data(), D.data()));
+		return D;
+	}
+	F gauge_fixed() const
+	{
+		F Ug(U.grid());
+		transform_into(Ug);
+		return Ug;
 	}
 
-	// abscissa of the vertex of the parabola through three points (landau.h:68-75)
-	static double solve_quadratic(double x1, double x2, double x3, double y1, double y2, double y3)
+	F new_gauge() const { return F(U.grid(), 1); }
+	F new_deriv() const { return F(U.grid(), 1); }
+	void assign(F &dst, F const &src) const { dst.copy_from(src); }
+	void step(F &h, F const &D, double e) const
 	{
-		return 0.5 * ((x2 * x2 - x3 * x3) * y1 - (x1 * x1 - x3 * x3) * y2 + (x1 * x1 - x2 * x2) * y3) /
-		       ((x2 - x3) * y1 - (x1 - x3) * y2 + (x1 - x2) * y3);
+		ctx().check(mesh_su2_link_update(ctx().get(), U.grid().c(), 1, prec, h.data(), h.data(), D.data(), e, 0));
+	}
+	void reunit(F &h) const { ctx().check(mesh_su2_reunit(ctx().get(), U.grid().c(), h.nf(), prec, h.data())); }
+	double dot(F const &A, F const &B) const
+	{
+		double r;
+		ctx().check(mesh_su2_dot(ctx().get(), U.grid().c(), A.nf(), prec, A.data(), B.data(), &r));
+		return r;
+	}
+	void axpy(F &Y, double a, F const &X) const
+	{
+		ctx().check(mesh_su2_axpy(ctx().get(), U.grid().c(), Y.nf(), prec, Y.data(), a, X.data()));
+	}
+	void transform_into(F &Ug) const
+	{
+		ctx().check(mesh_su2_gauge_transform(ctx().get(), U.grid().c(), prec, Ug.data(), U.data(), g.data()));
+	}
+	void or_halfsweep(F &Ug, int parity, double omega)
+	{
+		ctx().check(
+		    mesh_su2_landau_or_halfsweep(ctx().get(), U.grid().c(), prec, Ug.data(), g.data(), parity, omega));
+	}
+	double divsum_of(F const &Ug) const
+	{
+		double d;
+		ctx().check(mesh_su2_landau_divsum(ctx().get(), U.grid().c(), prec, Ug.data(), nullptr, &d));
+		return d;
+	}
+};
+
+// Landau<U1<T>>: g = exp(i phi), phi one FP32 angle per site
+template <class T> struct Landau<U1<T>> : detail::LandauAscent<Landau<U1<T>>>
+{
+	using gauge_type = U1Field;
+	using deriv_type = U1Field;
+	U1Field const &U;
+	U1Field g;
+	bool verbose = false;
+
+	void reset() { g.fill_unit(); }
+	explicit Landau(U1Field const &U_) : U(U_), g(U_.grid(), 1) { reset(); }
+
+	double operator()(U1Field const &h) const
+	{
+		double r;
+		ctx().check(mesh_u1_landau_functional(ctx().get(), U.grid().c(), U.data(), h.data(), &r));
+		return r;
+	}
+	double operator()() const { return (*this)(g); }
+	double condition() const
+	{
+		double r;
+		ctx().check(mesh_u1_landau_divsum(ctx().get(), U.grid().c(), U.data(), g.data(), &r));
+		return r / (U.grid().ndim() * (double)U.grid().size());
+	}
+	U1Field deriv() const
+	{
+		U1Field D(U.grid(), 1);
+		ctx().check(mesh_u1_landau_deriv(ctx().get(), U.grid().c(), U.data(), g.data(), D.data()));
+		return D;
+	}
+	U1Field gauge_fixed() const
+	{
+		U1Field Ug(U.grid());
+		transform_into(Ug);
+		return Ug;
 	}
 
-	// step length along F (landau.h:77-98)
-	double line_search(Lattice<G> const &F, double eps) const
+	U1Field new_gauge() const { return U1Field(U.grid(), 1); }
+	U1Field new_deriv() const { return U1Field(U.grid(), 1); }
+	void assign(U1Field &dst, U1Field const &src) const { dst.copy_from(src); }
+	void step(U1Field &h, U1Field const &D, double e) const
 	{
-		double base = (*this)();
-		Lattice<G> h(g);
-		auto trial = [&](double e) {
-			h = g;
-			evolve(h, F, e);
-			return (*this)(h);
-		};
-		double y1 = trial(eps);
-		while (y1 <= base)
-		{
-			eps *= 0.5;
-			y1 = trial(eps);
-			if (eps < 1e-10)
-				return 0.0;
-		}
-		double y2 = trial(eps * 1.2);
-		if (y2 > y1)
-			return 1.2 * eps;
-		double y3 = trial(eps * 0.8);
-		return solve_quadratic(eps, 1.2 * eps, 0.8 * eps, y1, y2, y3);
+		ctx().check(mesh_u1_gauge_step(ctx().get(), U.grid().c(), h.data(), h.data(), D.data(), e));
 	}
+	void reunit(U1Field &) const {} // angles stay on the group
+	double dot(U1Field const &A, U1Field const &B) const
+	{
+		double r;
+		ctx().check(mesh_u1_dot(ctx().get(), (long)(A.bytes() / sizeof(float)), A.data(), B.data(), &r));
+		return r;
+	}
+	void axpy(U1Field &Y, double a, U1Field const &X) const
+	{
+		ctx().check(mesh_u1_axpy(ctx().get(), (long)(Y.bytes() / sizeof(float)), Y.data(), a, X.data()));
+	}
+	void transform_into(U1Field &Ug) const
+	{
+		ctx().check(mesh_u1_gauge_transform(ctx().get(), U.grid().c(), Ug.data(), U.data(), g.data()));
+	}
+	void or_halfsweep(U1Field &Ug, int parity, double omega)
+	{
+		ctx().check(mesh_u1_landau_or_halfsweep(ctx().get(), U.grid().c(), Ug.data(), g.data(), parity, omega));
+	}
+	double divsum_of(U1Field const &Ug) const
+	{
+		double d;
+		ctx().check(mesh_u1_landau_divsum(ctx().get(), U.grid().c(), Ug.data(), nullptr, &d));
+		return d;
+	}
+};
 
-	// The reference's solver (landau.h:100-136): ascent along deriv() with the
-	// line search, Polak-Ribiere steps after iteration 20 (as written there,
-	// `last = F` precedes `F += last * beta`, so the step is (1 + beta) F), then
-	// g = reunitize(exp(eps F) g), until condition() < tol; returns iterations.
-	int run_ascent(int max_iter, double tol = 1e-10)
-	{
-		double eps = 3000.0, lastNorm = 0.0;
-		Lattice<G> last(U.grid());
-		for (int iter = 0; iter < max_iter; ++iter)
-		{
-			double cond = condition();
-			if (verbose)
-				std::printf("iter = %d, cond = %.6e, term = %.12f, eps = %.6e\n", iter + 1, cond, (*this)(), eps);
-			if (cond < tol)
-				return iter;
-			Lattice<G> F = deriv();
-			double norm = sum_real_trace_prod(F, F);
-			double beta = iter > 20 ? (norm - sum_real_trace_prod(F, last)) / lastNorm : 0.0;
-			last = F;
-			if (iter > 20 && beta > 0)
-				add_mul(F, beta, last);
-			eps = line_search(F, eps);
-			evolve(g, F, eps);
-			reunitize(g);
-			lastNorm = norm;
-		}
-		return max_iter;
-	}
+// the field type of a group's gauge configuration
+template <class G> struct FieldOf
+{
+	using type = GaugeField<G>;
+};
+template <class T> struct FieldOf<SU2<T>>
+{
+	using type = SU2Field<T>;
+};
+template <class T> struct FieldOf<U1<T>>
+{
+	using type = U1Field;
 };
 
 // ---- runtime dispatch (src/gauge/utils.h:155-186) ------------------------------
@@ -518,6 +872,13 @@ template <typename F> void dispatchByGroup(F f, std::string const &group, int pr
 			f.template operator()<SU3<float>>();
 		else
 			f.template operator()<SU3<double>>();
+	}
+	else if (group == "su2")
+	{
+		if (precision == 1)
+			f.template operator()<SU2<float>>();
+		else
+			f.template operator()<SU2<double>>();
 	}
 	else if (group == "u1")
 	{

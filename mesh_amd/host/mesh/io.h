@@ -478,6 +478,12 @@ inline void writeConfig(io::Ensemble const &e, std::string const &name, U1Field 
 	io::write_npy(e.config_path(name), "<f4", io::config_shape(U.grid(), 1), h.data());
 }
 
+template <class T> void writeConfig(io::Ensemble const &e, std::string const &name, SU2Field<T> const &U)
+{
+	auto h = U.download();
+	io::write_npy(e.config_path(name), sizeof(T) == 8 ? "<f8" : "<f4", io::config_shape(U.grid(), 4), h.data());
+}
+
 namespace io {
 // opens "dir/configs/<n>", checks the group and the geometry against the
 // ensemble's attributes and the array's shape (readConfig,
@@ -533,6 +539,22 @@ inline void readConfig(std::string const &configName, U1Field &U)
 	if (a.descr != "<f4")
 		throw std::runtime_error("U(1) configuration '" + configName + "' is not FP32 angles");
 	ctx().check(mesh_memcpy_from_host(ctx().get(), U.data(), a.data.data(), a.data.size()));
+}
+
+template <class T> void readConfig(std::string const &configName, SU2Field<T> &U)
+{
+	io::Npy a = io::open_config(configName, "su2", U.grid(), 4);
+	const size_t n = a.count();
+	std::vector<T> h(n);
+	if (a.descr == "<f8")
+		for (size_t i = 0; i < n; ++i)
+			h[i] = (T)((double const *)a.data.data())[i];
+	else if (a.descr == "<f4")
+		for (size_t i = 0; i < n; ++i)
+			h[i] = (T)((float const *)a.data.data())[i];
+	else
+		throw std::runtime_error("SU(2) configuration '" + configName + "' is neither <f8 nor <f4");
+	U.upload(h);
 }
 
 // geometry stored with a configuration ("dir/configs/<n>")

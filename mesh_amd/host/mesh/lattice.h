@@ -183,6 +183,18 @@ template <class T> struct U1
 	static constexpr int dim() { return 1; }
 };
 
+// SU(2): unit quaternions (w, x, y, z) = w - i (x, y, z).sigma, the
+// reference's util::quaternion<T> (re, im1, im2, im3)
+template <class T> struct SU2
+{
+	using real = T;
+	using element = std::array<T, 4>;
+	static constexpr int prec = sizeof(T) == 8 ? MESH_F64 : MESH_F32;
+	static std::string name() { return "su2"; }
+	static constexpr int Nc() { return 2; }
+	static constexpr int dim() { return 3; }
+};
+
 template <class G> struct GaugeTraits
 {
 	static std::string name() { return G::name(); }

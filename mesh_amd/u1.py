@@ -12,6 +12,7 @@ import time
 import numpy as np
 
 from ._lib import MeshError, lib
+from .gauge import LandauSolver
 from .lattice import Grid, default_context
 
 _d = C.c_double
@@ -321,3 +322,99 @@ def ess_per_second(res, obs="Q"):
     tau, err = tau_int(x)
     n = x.shape[0] * x.shape[1]
     return n / (2 * tau) / res["seconds"], tau, err
+
+
+class U1Field:
+    """One U(1) configuration as FP32 angles theta[f][x] (nf = Nd: links, nf
+    = 1: a gauge transformation g = exp(i phi)) on a grid of 2 to 4 dimensions."""
+
+    def __init__(self, grid, nf=None, ctx=None):
+        self.grid = grid
+        self.ctx = ctx or default_context()
+        self.nf = grid.ndim() if nf is None else nf
+        self.buf = Buffer(self.nf * grid.size(), np.float32, self.ctx)
+        self.ptr = self.buf.ptr
+
+    def upload(self, a):
+        self.buf.upload(a)
+        return self
+
+    def download(self):
+        return self.buf.download().reshape((self.nf,) + tuple(self.grid.shape))
+
+    def fill_unit(self):
+        self.buf.zero()
+        return self
+
+
+class Landau(LandauSolver):
+    """Landau gauge fixing of a U(1) field (Landau<U1<T>>, src/gauge/landau.h)
+    on angles: g = exp(i phi), phi one FP32 angle per site."""
+
+    def __init__(self, U):
+        self.U = U
+        self.g = U1Field(U.grid, 1, U.ctx)
+        self.verbose = False
+        self.reset()
+
+    def reset(self):
+        self.g.fill_unit()
+
+    def _call(self, name, *args):
+        self.U.ctx.call(name, self.U.grid.ref, *args)
+
+    def __call__(self, h=None):
+        """F[h] = sum cos(phi(x) + theta_mu(x) - phi(x+mu)) / (Nd V) (landau.h:23-34); default h = g"""
+        h = self.g if h is None else h
+        o = (C.c_double * 1)()
+        self._call("mesh_u1_landau_functional", self.U.ptr, h.ptr, o)
+        return o[0]
+
+    def divsum(self, Ug=None):
+        if Ug is not None:
+            return self._divsum_of(Ug)
+        o = (C.c_double * 1)()
+        self._call("mesh_u1_landau_divsum", self.U.ptr, self.g.ptr, o)
+        return o[0]
+
+    def deriv(self):
+        """-0.5 / (Nd V) Im(g d), the algebra element i F (landau.h:52-65)"""
+        F = U1Field(self.U.grid, 1, self.U.ctx)
+        self._call("mesh_u1_landau_deriv", self.U.ptr, self.g.ptr, F.ptr)
+        return F
+
+    def _links(self):
+        return U1Field(self.U.grid, None, self.U.ctx)
+
+    def _transform_into(self, Ug):
+        self._call("mesh_u1_gauge_transform", Ug.ptr, self.U.ptr, self.g.ptr)
+
+    def _or_half(self, Ug, parity, omega):
+        self._call("mesh_u1_landau_or_halfsweep", Ug.ptr, self.g.ptr, parity, _d(omega))
+
+    def _divsum_of(self, Ug):
+        o = (C.c_double * 1)()
+        self._call("mesh_u1_landau_divsum", Ug.ptr, None, o)
+        return o[0]
+
+    def _clone(self, h):
+        c = U1Field(h.grid, h.nf, h.ctx)
+        c.buf.copy_from(h.buf)
+        return c
+
+    def _assign(self, dst, src):
+        dst.buf.copy_from(src.buf)
+
+    def _step(self, h, F, e):
+        self._call("mesh_u1_gauge_step", h.ptr, h.ptr, F.ptr, _d(e))
+
+    def _reunit(self, h):
+        pass  # angles stay on the group
+
+    def _dot(self, A, B):
+        o = (C.c_double * 1)()
+        A.ctx.call("mesh_u1_dot", C.c_long(A.buf.n), A.ptr, B.ptr, o)
+        return o[0]
+
+    def _axpy(self, Y, a, X):
+        Y.ctx.call("mesh_u1_axpy", C.c_long(Y.buf.n), Y.ptr, _d(a), X.ptr)

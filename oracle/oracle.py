@@ -88,6 +88,26 @@ def _declare(L):
                                              C.c_uint32, C.c_int, _dp]
     L.orc_refpass_stage_times_f64.argtypes = [_dp, _ip, C.c_int, C.c_double, C.c_double, _dp]
     L.orc_set_threads.argtypes = [C.c_int]
+    for sfx, rp in (("f64", _dp), ("f32", _fp)):
+        g = lambda n: getattr(L, n + "_" + sfx)
+        g("orc_su2_random").argtypes = [rp, _ip, C.c_int, C.c_uint64]
+        g("orc_su2_momenta").argtypes = [rp, _dp, _ip, C.c_int, C.c_uint64, C.c_uint32]
+        g("orc_su2_force_update").argtypes = [rp, rp, _ip, C.c_int, C.c_double, C.c_double]
+        g("orc_su2_link_update").argtypes = [rp, rp, rp, _ip, C.c_int, C.c_double, C.c_int]
+        g("orc_su2_plaquette").argtypes = [rp, _ip, C.c_int]
+        g("orc_su2_plaquette").restype = C.c_double
+        g("orc_su2_staple").argtypes = [rp, _ip, C.c_int, C.c_int, rp]
+        g("orc_su2_hmc_trajectory").argtypes = [rp, _ip, C.c_int, C.c_double, _dp, C.c_int, C.c_uint64,
+                                                C.c_uint32, C.c_int, _dp]
+        g("orc_su2_landau_functional").argtypes = [rp, C.c_void_p, _ip, C.c_int]
+        g("orc_su2_landau_functional").restype = C.c_double
+        g("orc_su2_landau_divsum").argtypes = [rp, C.c_void_p, _ip, C.c_int]
+        g("orc_su2_landau_divsum").restype = C.c_double
+        g("orc_su2_landau_deriv").argtypes = [rp, C.c_void_p, _ip, C.c_int, rp]
+        g("orc_su2_gauge_transform").argtypes = [rp, rp, _ip, C.c_int, rp]
+        g("orc_su2_landau_or_half").argtypes = [rp, C.c_void_p, _ip, C.c_int, C.c_int, C.c_double]
+    L.orc_su2_qmul.argtypes = [_dp, _dp, _dp]
+    L.orc_su2_expm1.argtypes = [_dp, _dp]
     L.orc_max_threads.restype = C.c_int
 
 
@@ -430,3 +450,105 @@ def set_threads(n):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+# ---------------------------------------------------------------- SU(2)
+# quaternion fields U[Nd][V][4] (w, x, y, z), float64 or float32
+
+def _q(a):
+    return "f64" if a.dtype == np.float64 else "f32"
+
+
+def su2_random(dims, seed, prec="f64"):
+    U = np.zeros((len(dims), volume(dims), 4), np.float64 if prec == "f64" else np.float32)
+    getattr(lib(), "orc_su2_random_" + _q(U))(U, _dims(dims), len(dims), seed)
+    return U
+
+
+def su2_momenta(dims, seed, traj, prec="f64"):
+    P = np.zeros((len(dims), volume(dims), 4), np.float64 if prec == "f64" else np.float32)
+    K = np.zeros(volume(dims))
+    getattr(lib(), "orc_su2_momenta_" + _q(P))(P, K, _dims(dims), len(dims), seed, traj)
+    return P, K
+
+
+def su2_force_update(U, P, dims, beta, delta):
+    P = np.ascontiguousarray(P).copy()
+    getattr(lib(), "orc_su2_force_update_" + _q(U))(np.ascontiguousarray(U), P, _dims(dims), len(dims), beta, delta)
+    return P
+
+
+def su2_link_update(U, P, dims, t, reunit=False):
+    out = np.zeros_like(U)
+    getattr(lib(), "orc_su2_link_update_" + _q(U))(np.ascontiguousarray(U), out, np.ascontiguousarray(P),
+                                                   _dims(dims), len(dims), t, int(reunit))
+    return out
+
+
+def su2_plaquette(U, dims):
+    return getattr(lib(), "orc_su2_plaquette_" + _q(U))(np.ascontiguousarray(U), _dims(dims), len(dims))
+
+
+def su2_staple(U, dims, mu):
+    S = np.zeros((1, volume(dims), 4), U.dtype)
+    getattr(lib(), "orc_su2_staple_" + _q(U))(np.ascontiguousarray(U), _dims(dims), len(dims), mu, S)
+    return S[0]
+
+
+def su2_hmc_trajectory(U, dims, beta, deltas, seed, traj, metropolis=True):
+    """In place on U. Returns dict(dH, accepted, plaq, H_old, H_new, u)."""
+    out = np.zeros(6)
+    d = np.ascontiguousarray(deltas, np.float64)
+    getattr(lib(), "orc_su2_hmc_trajectory_" + _q(U))(U, _dims(dims), len(dims), beta, d, d.size, seed, traj,
+                                                      int(metropolis), out)
+    return dict(dH=out[0], accepted=bool(out[1]), plaq=out[2], H_old=out[3], H_new=out[4], u=out[5])
+
+
+def su2_qmul(a, b):
+    out = np.zeros(4)
+    lib().orc_su2_qmul(np.ascontiguousarray(a, np.float64), np.ascontiguousarray(b, np.float64), out)
+    return out
+
+
+def su2_expm1(v):
+    out = np.zeros(4)
+    lib().orc_su2_expm1(np.ascontiguousarray(v, np.float64), out)
+    return out
+
+
+# SU(2) Landau gauge: h[1][V][4] (or None for h = 1)
+
+def _opt(h, U):
+    """pointer of an optional array of U's dtype (None -> NULL); h must be C-contiguous"""
+    if h is None:
+        return None
+    if h.dtype != U.dtype or not h.flags.c_contiguous:
+        raise ValueError("h must be a C-contiguous array of the links' dtype")
+    return h.ctypes.data
+
+def su2_landau_functional(U, h, dims):
+    return getattr(lib(), "orc_su2_landau_functional_" + _q(U))(np.ascontiguousarray(U), _opt(h, U), _dims(dims),
+                                                                len(dims))
+
+
+def su2_landau_divsum(U, h, dims):
+    return getattr(lib(), "orc_su2_landau_divsum_" + _q(U))(np.ascontiguousarray(U), _opt(h, U), _dims(dims),
+                                                            len(dims))
+
+
+def su2_landau_deriv(U, h, dims):
+    F = np.zeros((1, volume(dims), 4), U.dtype)
+    getattr(lib(), "orc_su2_landau_deriv_" + _q(U))(np.ascontiguousarray(U), _opt(h, U), _dims(dims), len(dims), F)
+    return F
+
+
+def su2_gauge_transform(U, h, dims):
+    out = np.zeros_like(U)
+    getattr(lib(), "orc_su2_gauge_transform_" + _q(U))(np.ascontiguousarray(U), np.ascontiguousarray(h),
+                                                       _dims(dims), len(dims), out)
+    return out
+
+
+def su2_landau_or_half(U, h, dims, parity, omega):
+    """in place on U (transformed links) and h (may be None)"""
+    getattr(lib(), "orc_su2_landau_or_half_" + _q(U))(U, _opt(h, U), _dims(dims), len(dims), parity, omega)

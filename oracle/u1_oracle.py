@@ -437,3 +437,98 @@ def pflow_loss(params, u, beta):
     S = torch.sum(1 - torch.cos(th), dim=-1) + (1 - torch.cos(last))
     logq = torch.sum(nlg, dim=-1) - 2 * V * math.log(2 * math.pi)
     return logq + beta * S, logq, S
+
+
+# ---- Landau gauge on angles (src/gauge/landau.h for U1<T>): links
+# th[mu][x...] and the gauge field g = exp(i phi), phi[x...]; float64 from
+# the float32 angles, in the operation order of mesh_amd/csrc/u1.hip
+
+def wrap_d(a):
+    """to [-pi, pi) as a - 2 pi floor((a + pi) / (2 pi)), then the end fix-ups"""
+    tp = 6.283185307179586
+    a = np.asarray(a, np.float64)
+    a = a - tp * np.floor((a + 3.141592653589793) * (1.0 / tp))
+    a = np.where(a >= 3.141592653589793, a - tp, a)
+    return np.where(a < -3.141592653589793, a + tp, a)
+
+
+def landau_angles(th, phi):
+    """angle of g(x) U_mu(x) g(x+mu)^*: (phi(x) + theta_mu(x)) - phi(x+mu)"""
+    t = np.asarray(th, np.float32).astype(np.float64)
+    if phi is None:
+        return t
+    p = np.asarray(phi, np.float32).astype(np.float64)
+    return np.stack([(p + t[mu]) - np.roll(p, -1, axis=mu) for mu in range(t.shape[0])])
+
+
+def landau_functional(th, phi):
+    """sum cos(angle) / (Nd V) (landau.h:23-34 with Nc = 1)"""
+    a = landau_angles(th, phi)
+    return float(np.cos(a).sum() / a.size)
+
+
+def landau_gd(th, phi):
+    """Im(g d)(x) = sum_mu sin(angle_mu(x)) - sin(angle_mu(x - mu)) (landau.h:41-48)"""
+    a = landau_angles(th, phi)
+    s = np.zeros(a.shape[1:])
+    for mu in range(a.shape[0]):
+        sn = np.sin(a[mu])
+        s = s + sn
+        s = s - np.roll(sn, 1, axis=mu)
+    return s
+
+
+def landau_divsum(th, phi):
+    """sum_x Im(g d)^2; condition() = divsum / (Nd V)"""
+    return float(np.sum(landau_gd(th, phi) ** 2))
+
+
+def landau_deriv(th, phi):
+    """-0.5 / (Nd V) Im(g d), the algebra element i F (landau.h:52-65), as float32"""
+    nd = np.asarray(th).shape[0]
+    return ((-0.5 / (nd * np.asarray(th)[0].size)) * landau_gd(th, phi)).astype(np.float32)
+
+
+def gauge_transform(th, phi):
+    return wrap_d(landau_angles(th, phi)).astype(np.float32)
+
+
+def parity_mask(shape):
+    return np.indices(shape).sum(axis=0) % 2
+
+
+def landau_or_half(th, phi, parity, omega):
+    """overrelaxation of one parity on the transformed angles: K(x) = sum_mu
+    exp(i theta_mu(x)) + exp(-i theta_mu(x - mu)), turn by -omega arg K.
+    Sites of one parity share no link, so the vectorised update is exact.
+    Returns (th, phi) as new float32 arrays (phi None stays None)."""
+    t = np.asarray(th, np.float32).astype(np.float64)
+    nd = t.shape[0]
+    kr = np.zeros(t.shape[1:])
+    ki = np.zeros(t.shape[1:])
+    for mu in range(nd):
+        b = np.roll(t[mu], 1, axis=mu)
+        kr = kr + np.cos(t[mu])
+        ki = ki + np.sin(t[mu])
+        kr = kr + np.cos(b)
+        ki = ki - np.sin(b)
+    m = (parity_mask(t.shape[1:]) == parity) & ~((kr == 0.0) & (ki == 0.0))
+    al = np.where(m, -omega * np.arctan2(ki, kr), 0.0)
+    out = np.asarray(th, np.float32).copy()
+    for mu in range(nd):
+        fw = wrap_d(t[mu] + al).astype(np.float32)
+        out[mu] = np.where(m, fw, out[mu])
+        mb = np.roll(m, -1, axis=mu)
+        bw = wrap_d(t[mu] - np.roll(al, -1, axis=mu)).astype(np.float32)
+        out[mu] = np.where(mb, bw, out[mu])
+    ph = None
+    if phi is not None:
+        p = np.asarray(phi, np.float32)
+        ph = np.where(m, wrap_d(p.astype(np.float64) + al).astype(np.float32), p)
+    return out, ph
+
+
+def gauge_step(phi, F, t):
+    """wrap(phi + t F): g = exp(t i F) g (landau.h:85)"""
+    return wrap_d(t * np.asarray(F, np.float32).astype(np.float64) +
+                  np.asarray(phi, np.float32).astype(np.float64)).astype(np.float32)

@@ -150,6 +150,37 @@ def test_cpp_landau_app_on_stored_config(tmp_path):
     m = re.search(r"landau ascent: (\d+) iterations, condition = ([0-9.e+-]+)", r.stdout)
     assert m and float(m.group(2)) < 1e-10
     r = _app("mesh_landau", ["--group", "u1", cfg])
-    assert r.returncode != 0 and "SU(3)" in r.stderr
+    assert r.returncode != 0 and "group mismatch" in r.stderr
     r = _app("mesh_landau", [])
     assert r.returncode == 2
+
+
+@pytest.mark.parametrize("group,geom,beta", [("su2", ["4", "4", "4", "4"], "2.2"), ("u1", ["6", "6", "6", "6"], "1.0"),
+                                             ("u1", ["16", "16"], "2.0")])
+def test_cpp_landau_other_groups(tmp_path, group, geom, beta):
+    """Landau<SU2<T>> and Landau<U1<T>> (the instantiations of
+    src/tests/lattice_gauge.cpp:46-53) through mesh_hmc --landau and
+    mesh_landau on a stored configuration: overrelaxation to theta < 1e-12
+    (FP32 links: 1e-9), the reference's ascent to its own stopping condition,
+    and a gauge-invariant plaquette.  (On rough 3-d U(1) configurations the
+    reference's ascent stalls: the parabola vertex of its line search lands at
+    a negative step, after which every search returns 0; hence 4-d and 2-d.)"""
+    ens = str(tmp_path / "e")
+    r = _app("mesh_hmc", ["--group", group, "--geom", *geom, "--beta", beta, "--count", "6", "--scheme", "2lf",
+                          "--epsilon", "0.5", "--substeps", "8", "--seed", "5", "--filename", ens, "--landau"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "HMC summary" in r.stdout and "landau:" in r.stdout
+    cfg = os.path.join(ens, "configs", "6")
+    for precision in ("2", "1"):
+        r = _app("mesh_landau", ["--group", group, "--precision", precision, cfg])
+        assert r.returncode == 0, r.stdout + r.stderr
+        m = re.search(r"landau: (\d+) sweeps, theta = ([0-9.e+-]+), functional = ([0-9.]+)", r.stdout)
+        assert m and float(m.group(2)) < (1e-12 if precision == "2" or group == "u1" else 1e-9), r.stdout[-500:]
+        F0 = float(re.search(r"start: functional = (-?[0-9.]+)", r.stdout).group(1))
+        assert float(m.group(3)) > F0
+        pb, pa = map(float, re.search(r"plaquette: before (-?[0-9.]+), after (-?[0-9.]+)", r.stdout).groups())
+        assert abs(pb - pa) < (1e-12 if precision == "2" and group == "su2" else 2e-6)
+    r = _app("mesh_landau", ["--group", group, "--ascent", "--max-iter", "100000", cfg])
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"landau ascent: (\d+) iterations, condition = ([0-9.e+-]+)", r.stdout)
+    assert m and float(m.group(2)) < 1e-10, r.stdout[-800:]
