@@ -174,14 +174,21 @@ def same_config_compare(U64, seed, scheme, eps, substeps):
 
     deltas = gauge.makeDeltas(scheme, eps, substeps)
     out = {}
-    for prec in (F64, F32):
+    for key, prec, fs in ((F64, F64, 0), (F32, F32, 0), ("fsum", F32, 1)):
         h = gauge.Hmc(U64.grid, prec, seed=seed + 1000, ctx=U64.ctx)
         h.U = U64.convert(prec)
-        r = h.runHmcUpdate(6.0, deltas, metropolis=False)
-        out[prec] = (r["dH"], gauge.plaquette(h.U), r["H_old"], r["H_new"])
+        U64.ctx.set_option("float_sums", fs)
+        try:
+            r = h.runHmcUpdate(6.0, deltas, metropolis=False)
+        finally:
+            U64.ctx.set_option("float_sums", 0)
+        out[key] = (r["dH"], gauge.plaquette(h.U), r["H_old"], r["H_new"])
     return dict(dplaq=abs(out[F32][1] - out[F64][1]), ddH=abs(out[F32][0] - out[F64][0]),
                 dH_f64=out[F64][0], plaq_f64=out[F64][1],
-                H_old_f32_minus_f64=out[F32][2] - out[F64][2], H_new_f32_minus_f64=out[F32][3] - out[F64][3])
+                H_old_f32_minus_f64=out[F32][2] - out[F64][2], H_new_f32_minus_f64=out[F32][3] - out[F64][3],
+                # the reference's float build (--precision 1): H_old and H_new each
+                # from FP32 lattice sums, then differenced
+                reference_float_sums_ddH=abs(out["fsum"][0] - out[F64][0]))
 
 
 def cpu_model():

@@ -116,7 +116,12 @@ void *mesh_stream(mesh_ctx *ctx);
  *       in a compact antihermitian layout of 9 reals per link (upper
  *       off-diagonal entries and the imaginary diagonal); 0 keeps the
  *       18-real matrix layout.  The momenta are exactly antihermitian, so
- *       both give the same bits. */
+ *       both give the same bits;
+ *   "float_sums" = 0 (default): 1 makes FP32 trajectories form Delta H as
+ *       the reference's float build (--precision 1) does: H_old and H_new
+ *       each from FP32 sums over the lattice (mesh_su3_float_energy), then
+ *       differenced.  A comparison baseline: its error grows with the volume
+ *       (the default FP64 per-site differenced sums do not). */
 int mesh_set_option(mesh_ctx *ctx, const char *name, long value);
 /* Per-kernel timing (replaces the util::Stopwatch globals swStaples, swExp,
  * swPlaquette ... of src/gauge/utils.h:23, src/gauge/hmc.h:16, printed at
@@ -353,6 +358,16 @@ int mesh_flow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, fl
  * lattice (proposals of either flow). */
 int mesh_flow_mh(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, double *state, const float *prop_theta,
                  const double *prop, double beta, uint64_t seed, uint32_t step, double *acc_dev);
+
+/* out[2] = {wilsonAction(U, beta), norm2(P)} of FP32 fields as the reference's
+ * float build computes them (src/gauge/utils.h:94-108, 141-146, 27-41): each
+ * plane's Re tr P and each direction's ||P_mu||^2 summed in FP32 by a 256-wide
+ * pair tree per block and a sequential FP32 sum of the blocks
+ * (device_sum_apply, src/lattice/device.h:184-210), accumulated in FP64 over
+ * planes and directions.  P may be NULL (out[1] = 0); p_compact: P in the
+ * compact layout of the fused trajectory. */
+int mesh_su3_float_energy(mesh_ctx *ctx, const mesh_grid *g, const float *U, const float *P, int p_compact,
+                          double beta, double *out);
 
 /* ---- SU(2) pure-gauge HMC (Hmc<SU2<T>>, src/gauge/hmc.cpp:111-114) --------
  *

@@ -74,6 +74,7 @@ _SIGS = {
     "mesh_su2_delta_h": (i32, [vp, gp, i32, vp, vp, vp, vp, f64, dp]),
     "mesh_su2_hmc_trajectory": (i32, [vp, gp, i32, C.POINTER(vp), C.POINTER(vp), vp, vp, f64, dp, i32, u64, u32,
                                       i32, dp]),
+    "mesh_su3_float_energy": (i32, [vp, gp, vp, vp, i32, f64, dp]),
     "mesh_su2_landau_functional": (i32, [vp, gp, i32, vp, vp, dp]),
     "mesh_su2_landau_divsum": (i32, [vp, gp, i32, vp, vp, dp]),
     "mesh_su2_landau_deriv": (i32, [vp, gp, i32, vp, vp, vp]),

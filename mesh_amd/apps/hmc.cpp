@@ -15,7 +15,9 @@
 // Additions: --therm K (first K trajectories without Metropolis),
 //   --landau (Landau gauge fixing of the final configuration by
 //   overrelaxation), --landau-ascent (the same by the reference's line-search
-//   ascent, src/gauge/landau.h:100-136).
+//   ascent, src/gauge/landau.h:100-136), --float-sums (with --precision 1:
+//   Delta H from FP32 lattice sums of H_old and H_new, as the reference's
+//   float build forms it; a comparison baseline).
 #include "mesh/gauge.h"
 #include "mesh/io.h"
 
@@ -37,7 +39,7 @@ namespace {
 	             "                [--start random|cold|<ensemble>/configs/<n>] [--seed S] [--count N]\n"
 	             "                [--scheme 2lf|2mn|4mn] [--epsilon E] [--substeps N] [--precision 1|2]\n"
 	             "                [--filename <ensemble> [--force] [--spacing K]] [--therm K] [--landau]\n"
-	             "                [--landau-ascent]\n",
+	             "                [--landau-ascent] [--float-sums]\n",
 	             msg);
 	std::exit(2);
 }
@@ -55,7 +57,7 @@ int main(int argc, char **argv)
 	std::vector<int32_t> geom = {8, 8, 8, 8};
 	double beta = 6.0, epsilon = 1.0;
 	int substeps = 8, count = 100, precision = 2, therm = 0, spacing = 1;
-	bool landau = false, ascent = false, geom_given = false, allow_overwrite = false;
+	bool landau = false, ascent = false, geom_given = false, allow_overwrite = false, float_sums = false;
 	std::string filename;
 
 	for (int i = 1; i < argc; ++i)
@@ -97,6 +99,8 @@ int main(int argc, char **argv)
 			landau = true;
 		else if (a == "--landau-ascent")
 			landau = ascent = true;
+		else if (a == "--float-sums")
+			float_sums = true;
 		else if (a == "--plot")
 			std::fprintf(stderr, "mesh_hmc: --plot (gnuplot of the plaquette history) is not supported; ignored\n");
 		else if (a == "--filename")
@@ -124,6 +128,8 @@ int main(int argc, char **argv)
 			geom = configGeometry(start);
 		auto t_total = std::chrono::steady_clock::now();
 		mesh_profile_enable(ctx().get(), 1);
+		if (float_sums)
+			ctx().check(mesh_set_option(ctx().get(), "float_sums", 1));
 		dispatchByGroup(
 		    [&]<typename G>() {
 			    Grid g(geom);

@@ -49,6 +49,7 @@ struct mesh_ctx
 	void *md_spare = nullptr; // third gauge field for the fused trajectory
 	size_t md_spare_bytes = 0;
 	int u1_general = 0;       // U(1): site-parallel kernels also for small 2-d lattices
+	int float_sums = 0;       // FP32 trajectories: Delta H from the reference's FP32 energy sums
 	void *u1_ws = nullptr;    // workspace of the site-parallel U(1) kernels
 	size_t u1_ws_bytes = 0;
 	// device address of the static scratch array of the no-allocation entry

@@ -343,6 +343,8 @@ int mesh_set_option(mesh_ctx *ctx, const char *name, long value)
 		ctx->compact_p = value != 0;
 	else if (!std::strcmp(name, "u1_general"))
 		ctx->u1_general = value != 0;
+	else if (!std::strcmp(name, "float_sums"))
+		ctx->float_sums = value != 0;
 	else
 		return mesh_fail(ctx, MESH_E_ARG, std::string("unknown option ") + name);
 	return MESH_OK;

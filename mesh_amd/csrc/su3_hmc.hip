@@ -998,6 +998,41 @@ __global__ __launch_bounds__(256) void k_trace_sum(Geo g, int nf, const R *__res
 	block_sum_store<1>(v, partials, nb, blockIdx.x);
 }
 
+// ---- the reference's float build (precision 1) energies: a per-site FP32
+// value summed as device_sum_apply does (src/lattice/device.h:184-210), a
+// 256-wide shared-memory pair tree per workgroup in FP32 over the sites in
+// lexicographic order; the workgroup results are summed on the host in FP32.
+// mode 0: Re tr of the (mu, nu) plaquette, mode 1: ||P_mu||^2 (PC: compact P).
+template <bool PC>
+__global__ __launch_bounds__(256) void k_float_tree(Geo g, const float *__restrict__ A, int mode, int mu, int nu,
+                                                    float *__restrict__ partials)
+{
+	__shared__ float s[256];
+	const long gi = (long)blockIdx.x * 256 + threadIdx.x;
+	const int li = threadIdx.x;
+	if (gi < g.V)
+	{
+		if (mode == 0)
+		{
+			const long xpm = shift_c(g, gi, mu, coord(g, gi, mu), 1), xpn = shift_c(g, gi, nu, coord(g, gi, nu), 1);
+			m3<float> P1 = mul_nn(load(A, mu, g, gi), load(A, nu, g, xpm));
+			m3<float> P2 = mul_nn(load(A, nu, g, gi), load(A, mu, g, xpn));
+			s[li] = retr_nd(P1, P2);
+		}
+		else
+			s[li] = norm2_m3(PC ? load_alg<float, false>(A, mu, g, gi) : load(A, mu, g, gi));
+	}
+	__syncthreads();
+	for (int st = 128; st > 0; st >>= 1)
+	{
+		if (li < st && gi + st < g.V)
+			s[li] += s[li + st];
+		__syncthreads();
+	}
+	if (li == 0)
+		partials[blockIdx.x] = s[0];
+}
+
 // ---------------------------------------------------------------- host helpers
 
 struct Setup
@@ -1188,6 +1223,55 @@ int launch_momenta_link(mesh_ctx *ctx, Geo const &geo, int prec, void *P, double
 		compact ? launch(0.0f, PC1{}) : launch(0.0f, PC0{});
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+// out = {wilsonAction(U, beta), norm2(P)} of the reference's float build
+// (src/gauge/utils.h:94-108, 141-146, 27-41): each plane's and each
+// direction's FP32 sum, accumulated in FP64
+int float_energy(mesh_ctx *ctx, Geo const &geo, const float *U, const float *P, bool compact, double beta,
+                 double *out)
+{
+	const int nb = nblk(geo.V, 256);
+	if (int rc = mesh_reserve_partials(ctx, nb))
+		return rc;
+	float *dp = reinterpret_cast<float *>(ctx->partials);
+	std::vector<float> h(nb);
+	auto tree = [&](const float *A, int mode, int mu, int nu, bool pc, float *r) -> int {
+		if (pc)
+			k_float_tree<true><<<nb, 256, 0, ctx->stream>>>(geo, A, mode, mu, nu, dp);
+		else
+			k_float_tree<false><<<nb, 256, 0, ctx->stream>>>(geo, A, mode, mu, nu, dp);
+		MESH_LAUNCHED(ctx);
+		MESH_HIP(ctx, hipMemcpyAsync(h.data(), dp, nb * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+		MESH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+		float acc = h[0];
+		for (int i = 1; i < nb; ++i)
+			acc += h[i];
+		*r = acc;
+		return MESH_OK;
+	};
+	const int nd = geo.nd;
+	double sp = 0.0, sk = 0.0;
+	for (int mu = 0; mu < nd; ++mu)
+		for (int nu = mu + 1; nu < nd; ++nu)
+		{
+			float r;
+			if (int rc = tree(U, 0, mu, nu, false, &r))
+				return rc;
+			sp += r;
+		}
+	const double vol = (double)geo.V;
+	const double plaq = sp / (vol * nd * (nd - 1) * 0.5) / 3.0;
+	out[0] = (1.0 - plaq) * (beta * nd * (nd - 1) * 0.5 * vol);
+	for (int mu = 0; mu < nd && P; ++mu)
+	{
+		float r;
+		if (int rc = tree(P, 1, mu, 0, compact, &r))
+			return rc;
+		sk += r;
+	}
+	out[1] = sk;
 	return MESH_OK;
 }
 
@@ -1544,6 +1628,9 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 		return mesh_fail(ctx, MESH_E_ARG, "U and U_work must differ");
 	int rc;
 	const int nfused = (nd_ - 1) / 2;
+	// option "float_sums": Delta H as the reference's float build forms it
+	const bool fsum = ctx->float_sums && prec == MESH_F32;
+	double h_old[2] = {0.0, 0.0};
 	// the fused kernels keep P in the compact antihermitian layout (half the
 	// bytes of a matrix field, the same bits; option "compact_momenta" = 0
 	// keeps the 18-real matrix layout)
@@ -1573,6 +1660,9 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 		void *other = (nfused % 2) ? *U_work : ctx->md_spare;
 		if ((rc = launch_momenta_link(ctx, s.geo, prec, P, K, seed, traj, cur, *U, deltas[0], 0, compact)))
 			return rc;
+		// the link step only reads P: P is still the trajectory's initial momenta
+		if (fsum && (rc = float_energy(ctx, s.geo, (const float *)*U, (const float *)P, compact, beta, h_old)))
+			return rc;
 		for (int i = 1; i + 1 < nd_; i += 2)
 		{
 			if ((rc = launch_force_update(ctx, s.geo, prec, cur, P, beta, deltas[i], other, deltas[i + 1],
@@ -1586,6 +1676,9 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 	else
 		for (int i = 0; i < nd_; ++i)
 		{
+			if (i == 0 && fsum &&
+			    (rc = float_energy(ctx, s.geo, (const float *)*U, (const float *)P, false, beta, h_old)))
+				return rc;
 			if (i % 2 == 0)
 				rc = launch_link_update(ctx, s.geo, s.geo.nd, prec, *U_work, i == 0 ? *U : *U_work, P, deltas[i],
 				                        i == nd_ - 1);
@@ -1597,6 +1690,18 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 	double dh[5];
 	if ((rc = delta_h(ctx, s.geo, prec, *U, *U_work, P, K, beta, dh, compact)))
 		return rc;
+	double np = n_plaquettes(s.geo);
+	double H_old = beta * (np - dh[1] / 3.0) + dh[3], H_new = beta * (np - dh[2] / 3.0) + dh[4];
+	if (fsum)
+	{
+		// the reference's H_new - H_old of separately summed FP32 energies (hmc.cpp:85-92)
+		double h_new[2];
+		if ((rc = float_energy(ctx, s.geo, (const float *)*U_work, (const float *)P, compact, beta, h_new)))
+			return rc;
+		H_old = h_old[0] + h_old[1];
+		H_new = h_new[0] + h_new[1];
+		dh[0] = H_new - H_old;
+	}
 	double u = metropolis_uniform(seed, traj);
 	int acc = metropolis ? (u < std::exp(-dh[0])) : 1;
 	if (acc)
@@ -1605,14 +1710,25 @@ int mesh_su3_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 		*U = *U_work;
 		*U_work = t;
 	}
-	double np = n_plaquettes(s.geo);
 	out[0] = dh[0];
 	out[1] = acc;
 	out[2] = (acc ? dh[2] : dh[1]) / (np * 3.0);
-	out[3] = beta * (np - dh[1] / 3.0) + dh[3];
-	out[4] = beta * (np - dh[2] / 3.0) + dh[4];
+	out[3] = H_old;
+	out[4] = H_new;
 	out[5] = u;
 	return MESH_OK;
+}
+
+int mesh_su3_float_energy(mesh_ctx *ctx, const mesh_grid *g, const float *U, const float *P, int p_compact,
+                          double beta, double *out)
+{
+	MESH_CHECK_CTX(ctx);
+	Setup s = setup(ctx, g, MESH_F32);
+	if (s.rc)
+		return s.rc;
+	if (!U || !out)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to su3_float_energy");
+	return float_energy(ctx, s.geo, U, P, p_compact != 0, beta, out);
 }
 
 } // extern "C"

@@ -156,6 +156,16 @@ def deltaH(U_old, U_new, P, K_old, beta):
     return dict(dH=o[0], ps_old=o[1], ps_new=o[2], K_old=o[3], K_new=o[4])
 
 
+def float_energy(U, P=None, beta=6.0, p_compact=False):
+    """(wilsonAction(U, beta), norm2(P)) of FP32 fields summed as the
+    reference's float build does (mesh_su3_float_energy): a comparison
+    baseline, not the energies the library's trajectories use."""
+    o = _out(2)
+    U.ctx.call("mesh_su3_float_energy", U.grid.ref, U.ptr, None if P is None else P.ptr, int(bool(p_compact)),
+               _d(beta), o)
+    return o[0], o[1]
+
+
 class Hmc:
     """HMC state and observables (src/gauge/hmc.h:61-115, hmc.cpp:53-107).
 

@@ -88,6 +88,7 @@ def _declare(L):
                                              C.c_uint32, C.c_int, _dp]
     L.orc_refpass_stage_times_f64.argtypes = [_dp, _ip, C.c_int, C.c_double, C.c_double, _dp]
     L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_su3_float_energy_f32.argtypes = [_fp, C.c_void_p, _ip, C.c_int, C.c_double, _dp]
     for sfx, rp in (("f64", _dp), ("f32", _fp)):
         g = lambda n: getattr(L, n + "_" + sfx)
         g("orc_su2_random").argtypes = [rp, _ip, C.c_int, C.c_uint64]
@@ -552,3 +553,13 @@ def su2_gauge_transform(U, h, dims):
 def su2_landau_or_half(U, h, dims, parity, omega):
     """in place on U (transformed links) and h (may be None)"""
     getattr(lib(), "orc_su2_landau_or_half_" + _q(U))(U, _opt(h, U), _dims(dims), len(dims), parity, omega)
+
+
+def su3_float_energy(U, P, dims, beta):
+    """(wilsonAction, norm2(P)) of complex64 fields U, P [Nd][V][3][3] (P may
+    be None) as the reference's float build sums them"""
+    out = np.zeros(2)
+    Ur = _real(np.ascontiguousarray(U, np.complex64))
+    Pr = None if P is None else _real(np.ascontiguousarray(P, np.complex64))
+    lib().orc_su3_float_energy_f32(Ur, None if Pr is None else Pr.ctypes.data, _dims(dims), len(dims), beta, out)
+    return out

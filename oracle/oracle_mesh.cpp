@@ -513,6 +513,57 @@ int orc_make_deltas(char const *scheme, double epsilon, int substeps, double *ou
 	return make_deltas(scheme, epsilon, substeps, out, cap);
 }
 
+// {wilsonAction(U, beta), norm2(P)} of the reference's float build
+// (src/gauge/utils.h:94-108, 141-146, 27-41): per-site FP32 values summed as
+// device_sum_apply does (src/lattice/device.h:184-210), a 256-wide pair tree
+// per block in FP32 and a sequential FP32 sum of the block results; planes
+// and directions accumulated in FP64.  P may be NULL.
+void orc_su3_float_energy_f32(float const *U, float const *P, int const *dims, int nd, double beta, double *out)
+{
+	Geo g(dims, nd);
+	const int64_t nb = (g.V + 255) / 256;
+	std::vector<float> site(g.V);
+	auto tree = [&]() {
+		float acc = 0.0f;
+		for (int64_t b = 0; b < nb; ++b)
+		{
+			float s[256];
+			const int64_t base = b * 256;
+			for (int i = 0; i < 256; ++i)
+				s[i] = base + i < g.V ? site[base + i] : 0.0f;
+			for (int st = 128; st > 0; st >>= 1)
+				for (int i = 0; i < st; ++i)
+					if (base + i + st < g.V)
+						s[i] += s[i + st];
+			acc = b == 0 ? s[0] : acc + s[0];
+		}
+		return acc;
+	};
+	double sp = 0.0, sk = 0.0;
+	for (int mu = 0; mu < nd; ++mu)
+		for (int nu = mu + 1; nu < nd; ++nu)
+		{
+			for (int64_t x = 0; x < g.V; ++x)
+			{
+				int64_t xpm = g.shift(x, mu, 1), xpn = g.shift(x, nu, 1);
+				mat3<float> A = mul_nn(ld(U, g.V, mu, x), ld(U, g.V, nu, xpm));
+				mat3<float> B = mul_nn(ld(U, g.V, nu, x), ld(U, g.V, mu, xpn));
+				site[x] = retr_nd(A, B);
+			}
+			sp += tree();
+		}
+	const double vol = (double)g.V;
+	const double plaq = sp / (vol * nd * (nd - 1) * 0.5) / 3.0;
+	out[0] = (1.0 - plaq) * (beta * nd * (nd - 1) * 0.5 * vol);
+	for (int mu = 0; mu < nd && P; ++mu)
+	{
+		for (int64_t x = 0; x < g.V; ++x)
+			site[x] = norm2_3(ld(P, g.V, mu, x));
+		sk += tree();
+	}
+	out[1] = sk;
+}
+
 void orc_philox4x32_10(uint32_t const *ctr, uint32_t const *key, uint32_t *out)
 {
 	philox4x32_10(ctr, key, out);

@@ -30,7 +30,7 @@ def pytest_runtest_setup(item):
 
 # library defaults of the tuning options (mesh_set_option)
 OPTION_DEFAULTS = {"force_variant": -1, "xcd_remap": 3, "rb_tile": 4, "fuse_md": 1, "compact_momenta": 1,
-                   "u1_general": 0}
+                   "u1_general": 0, "float_sums": 0}
 
 
 @contextlib.contextmanager

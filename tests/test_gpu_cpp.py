@@ -184,3 +184,12 @@ def test_cpp_landau_other_groups(tmp_path, group, geom, beta):
     assert r.returncode == 0, r.stdout + r.stderr
     m = re.search(r"landau ascent: (\d+) iterations, condition = ([0-9.e+-]+)", r.stdout)
     assert m and float(m.group(2)) < 1e-10, r.stdout[-800:]
+
+
+def test_cpp_hmc_driver_float_sums():
+    """--precision 1 --float-sums: the reference's float-build Delta H runs
+    through the driver (the option reaches the library)."""
+    r = _app("mesh_hmc", ["--geom", "8", "8", "8", "8", "--count", "4", "--therm", "2", "--scheme", "2mn",
+                          "--epsilon", "0.5", "--substeps", "6", "--seed", "3", "--precision", "1", "--float-sums"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "HMC summary" in r.stdout

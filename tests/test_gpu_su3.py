@@ -607,3 +607,69 @@ def test_integrator_order():
     assert 1.58 < s < 2.33, s
     s = slope("4mn", 4)
     assert 3.3 < s < 4.5, s
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4, 4), (6, 4, 2, 8), (5, 3, 7), (16, 16, 8, 8)])
+def test_float_energy_bitexact(ctx, shape):
+    """mesh_su3_float_energy (the reference's float-build sums: FP32 pair
+    trees of 256 and a sequential FP32 sum of the blocks) against the oracle,
+    bit for bit, including a lattice smaller than one block."""
+    sh = list(shape)
+    g = mesh_amd.Grid(shape)
+    U = mesh_amd.GaugeField(g, F32, ctx)
+    gauge.randomGaugeField(U, 61)
+    P = mesh_amd.GaugeField(g, F32, ctx)
+    gauge.randomMomenta(P, 62, 0)
+    S, K = gauge.float_energy(U, P, 5.5)
+    So, Ko = O.su3_float_energy(U.download(), P.download(), sh, 5.5)
+    assert (S, K) == (So, Ko)
+    assert gauge.float_energy(U, None, 5.5) == (So, 0.0)
+
+
+def test_float_sums_trajectory_energies(ctx):
+    """option float_sums: H_old and H_new of an FP32 trajectory are the
+    float-build energies of (U, P_initial) and (U_new, P_final), Delta H their
+    difference; the fused (compact momenta) and unfused paths give the same bits."""
+    g, U64, d = _cold_thermalised(ctx, [8, 8, 8, 8], 10)
+    res = {}
+    for fuse in (0, 1):
+        h = gauge.Hmc(g, F32, seed=13, ctx=ctx)
+        h.U = U64.convert(F32)
+        U0 = h.U.download()
+        with options(ctx, float_sums=1, fuse_md=fuse):
+            r = h.runHmcUpdate(6.0, d, metropolis=False)
+        res[fuse] = (r, h.U.download())
+        if fuse == 0:
+            P0 = mesh_amd.GaugeField(g, F32, ctx)
+            gauge.randomMomenta(P0, 13, 0)
+            So, Ko = O.su3_float_energy(U0, P0.download(), [8] * 4, 6.0)
+            Sn, Kn = gauge.float_energy(h.U, h.P, 6.0)
+            assert r["H_old"] == So + Ko and r["H_new"] == Sn + Kn
+            assert r["dH"] == (Sn + Kn) - (So + Ko)
+    assert res[0][0]["dH"] == res[1][0]["dH"]
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
+def test_float_sums_error_grows_with_volume(ctx):
+    """The reference's float build against FP64 on the same thermalised
+    configurations: its Delta H error grows with the volume and is far above
+    that of the library's FP32-mixed trajectories (FP64 per-site differenced
+    sums)."""
+    err = {}
+    for L in (8, 16):
+        g, U64, d = _cold_thermalised(ctx, [L] * 4, 15)
+        e = {"fsum": [], "mixed": []}
+        for traj in range(3):
+            dh = {}
+            for key, prec, fs in (("f64", F64, 0), ("mixed", F32, 0), ("fsum", F32, 1)):
+                h = gauge.Hmc(g, prec, seed=17, ctx=ctx)
+                h.traj = traj
+                h.U = U64.convert(prec)
+                with options(ctx, float_sums=fs):
+                    dh[key] = h.runHmcUpdate(6.0, d, metropolis=False)["dH"]
+            e["fsum"].append(dh["fsum"] - dh["f64"])
+            e["mixed"].append(dh["mixed"] - dh["f64"])
+        err[L] = {k: float(np.sqrt(np.mean(np.square(v)))) for k, v in e.items()}
+        print(L, err[L])
+    assert err[16]["fsum"] > 10 * err[16]["mixed"]
+    assert err[16]["fsum"] > 4 * err[8]["fsum"]

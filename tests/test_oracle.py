@@ -358,3 +358,30 @@ def test_stage_times_report_every_stage():
     for refpass in (False, True):
         t = O.stage_times(U, dims, 6.0, 0.05, refpass=refpass)
         assert t.shape == (4,) and (t > 0).all()
+
+
+def test_float_energy_of_the_reference_float_build():
+    """orc_su3_float_energy_f32 (the reference's precision-1 sums): the unit
+    field is exact (3.0 per site, S = 0), and on random fields the FP32 sums
+    stay within their rounding bound of the FP64 sums of the same per-site
+    values while differing from them."""
+    for dims in ([4, 4, 4, 4], [6, 5, 3]):
+        V, nd = O.volume(dims), len(dims)
+        unit = np.zeros((nd, V, 3, 3), np.complex64)
+        unit[..., [0, 1, 2], [0, 1, 2]] = 1.0
+        S, K = O.su3_float_energy(unit, None, dims, 6.0)
+        assert S == 0.0 and K == 0.0
+    dims = [16, 16, 8, 8]
+    V, nd = O.volume(dims), len(dims)
+    U = O.random_gauge(dims, 3, "f32")
+    P, K64 = O.momenta(dims, 4, 0, "f32")
+    S, K = O.su3_float_energy(U, P, dims, 6.0)
+    S64 = 6.0 * (nd * (nd - 1) / 2 * V - O.plaq_sites(U, dims).sum() / 3.0)
+    K64 = K64.sum()
+    # per-site FP32 products differ from the FP64 ones by ~1e-6 relative; the
+    # FP32 block sums add a few ulp of each block and the host loop one ulp of
+    # the running sum per block
+    nb = V // 256
+    assert abs(S - S64) < 6.0 / 3.0 * (6 * V * 3 * 1e-6 + nb * 6 * 3 * V * 6e-8)
+    assert abs(K - K64) < nd * nb * K64 * 6e-8 + 1e-3
+    assert S != S64 and K != K64
