@@ -57,7 +57,8 @@ int main(int argc, char **argv)
 	std::vector<int32_t> geom = {8, 8, 8, 8};
 	double beta = 6.0, epsilon = 1.0;
 	int substeps = 8, count = 100, precision = 2, therm = 0, spacing = 1;
-	bool landau = false, ascent = false, geom_given = false, allow_overwrite = false, float_sums = false;
+	bool landau = false, ascent = false, geom_given = false, allow_overwrite = false, float_sums = false,
+	     therm_given = false;
 	std::string filename;
 
 	for (int i = 1; i < argc; ++i)
@@ -94,7 +95,10 @@ int main(int argc, char **argv)
 		else if (a == "--precision")
 			precision = std::atoi(next().c_str());
 		else if (a == "--therm")
+		{
 			therm = std::atoi(next().c_str());
+			therm_given = true;
+		}
 		else if (a == "--landau")
 			landau = true;
 		else if (a == "--landau-ascent")
@@ -149,7 +153,20 @@ int main(int argc, char **argv)
 					    hmc.seed = std::strtoull(at.get_string("philox_key").c_str(), nullptr, 10);
 					    hmc.traj = (uint32_t)(at.get_int("philox_traj0") + std::stoll(ename));
 					    std::printf("continuing the chain of %s at trajectory counter %u\n", edir.c_str(), hmc.traj);
+					    // a configuration stored inside the thermalisation: the
+					    // rest of it again without Metropolis, as the chain had it
+					    if (!therm_given && at.has("markov_discard"))
+					    {
+						    therm = (int)std::max(0LL, at.get_int("markov_discard") - std::stoll(ename));
+						    if (therm > 0)
+							    std::printf("%d thermalisation trajectories left in that chain\n", therm);
+					    }
 				    }
+				    if (at.has("precision") && at.get_int("precision") != precision)
+					    std::fprintf(stderr,
+					                 "warning: %s was written at --precision %lld, this run uses %d: the chain "
+					                 "does not continue bit for bit\n",
+					                 start.c_str(), at.get_int("precision"), precision);
 			    }
 			    io::Ensemble ens;
 			    if (!filename.empty())

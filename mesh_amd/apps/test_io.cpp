@@ -1,9 +1,11 @@
 // Ensemble file format of mesh/io.h, host only (no GPU call):
 //   test_io write <dir>   writes an ensemble with known attributes and arrays
 //   test_io read <dir>    prints what an ensemble holds, one "key = value" a line
+//   test_io overwrite <dir>  checks what create(..., allow_overwrite) replaces
 #include "mesh/io.h"
 
 #include <cstdio>
+#include <filesystem>
 #include <string>
 
 using namespace mesh;
@@ -12,12 +14,49 @@ int main(int argc, char **argv)
 {
 	if (argc != 3)
 	{
-		std::fprintf(stderr, "usage: test_io write|read <dir>\n");
+		std::fprintf(stderr, "usage: test_io write|read|overwrite <dir>\n");
 		return 2;
 	}
 	std::string mode = argv[1], dir = argv[2];
 	try
 	{
+		if (mode == "overwrite")
+		{
+			// allow_overwrite replaces an ensemble (its configurations and
+			// histories go) and refuses a directory that is not one
+			namespace fs = std::filesystem;
+			std::string ens = dir + "/ens", other = dir + "/other";
+			auto e = io::Ensemble::create(ens, false);
+			e.write_data("plaq_history", {1.0});
+			std::vector<double> u(2 * 2 * 2 * 18, 1.0);
+			io::write_npy(e.config_path("1"), "<f8", io::config_shape(Grid({2, 2}), 18), u.data());
+			io::Ensemble::create(ens, true);
+			if (fs::exists(ens + "/plaq_history.npy") || !fs::is_empty(ens + "/configs"))
+			{
+				std::printf("old ensemble files kept\n");
+				return 1;
+			}
+			fs::create_directories(other);
+			std::FILE *f = std::fopen((other + "/notes.txt").c_str(), "w");
+			std::fputs("x", f);
+			std::fclose(f);
+			try
+			{
+				io::Ensemble::create(other, true);
+				std::printf("overwrite of a non-ensemble not refused\n");
+				return 1;
+			}
+			catch (std::runtime_error const &)
+			{
+			}
+			if (!fs::exists(other + "/notes.txt"))
+			{
+				std::printf("non-ensemble touched\n");
+				return 1;
+			}
+			std::printf("overwrite ok\n");
+			return 0;
+		}
 		if (mode == "write")
 		{
 			auto e = io::Ensemble::create(dir, false);

@@ -23,6 +23,7 @@ philox_traj0 + n.
 """
 import json
 import os
+import shutil
 
 import numpy as np
 
@@ -40,16 +41,22 @@ class Ensemble:
 
     @classmethod
     def create(cls, path, allow_overwrite=False):
-        """New ensemble; an existing non-empty directory only with
-        allow_overwrite, which drops its configurations
-        (Hdf5File::create(filename, allow_overwrite))."""
+        """New ensemble (Hdf5File::create(filename, allow_overwrite)) in a
+        missing or empty directory, or -- with allow_overwrite -- in place of
+        an existing ensemble (a directory holding attributes.json), whose
+        configurations and data files are removed first.  Anything else is
+        refused, so that a stray path cannot be emptied."""
         path = str(path)
-        if os.path.exists(path) and not (os.path.isdir(path) and not os.listdir(path)) and not allow_overwrite:
-            raise FileExistsError(f"ensemble '{path}' exists (allow_overwrite to replace it)")
+        if os.path.exists(path) and not (os.path.isdir(path) and not os.listdir(path)):
+            if not allow_overwrite:
+                raise FileExistsError(f"ensemble '{path}' exists (allow_overwrite to replace it)")
+            if not os.path.isdir(path) or not os.path.isfile(os.path.join(path, _ATTR)):
+                raise FileExistsError(f"'{path}' is not an ensemble (no {_ATTR}); not overwritten")
+            shutil.rmtree(os.path.join(path, "configs"), ignore_errors=True)
+            for f in os.listdir(path):
+                if f.endswith(".npy") and os.path.isfile(os.path.join(path, f)):
+                    os.remove(os.path.join(path, f))
         cfg = os.path.join(path, "configs")
-        if os.path.isdir(cfg):
-            for f in os.listdir(cfg):
-                os.remove(os.path.join(cfg, f))
         os.makedirs(cfg, exist_ok=True)
         e = cls(path, {})
         e._flush()
@@ -137,6 +144,20 @@ def read_config(config_name, U, group="su3"):
     c = np.ascontiguousarray(a, real).view(U.cdtype).reshape(U.nf, U.grid.size(), 3, 3)
     U.upload(c)
     return U
+
+
+def resume_info(config_name):
+    """What continuing the chain that wrote "dir/configs/<n>" needs: dict of
+    key and traj (the Philox key and trajectory counter; None when the
+    ensemble does not record them), therm_left (thermalisation trajectories
+    of that chain still to run without Metropolis) and precision (1 or 2, or
+    None)."""
+    ens_dir, n = split_config_name(config_name)
+    at = Ensemble.open(ens_dir).attributes
+    st = resume_state(config_name)
+    left = max(0, int(at["markov_discard"]) - int(n)) if "markov_discard" in at and n.isdigit() else 0
+    return dict(key=None if st is None else st[0], traj=None if st is None else st[1], therm_left=left,
+                precision=at.get("precision"))
 
 
 def resume_state(config_name):

@@ -7,6 +7,7 @@ throws ``std::runtime_error``).
 """
 import ctypes as C
 import math
+import warnings
 import time
 
 import numpy as np
@@ -204,9 +205,18 @@ class Hmc:
         from . import configio
 
         configio.read_config(config_name, self.U)
-        state = configio.resume_state(config_name) if resume else None
-        if state is not None:
-            self.seed, self.traj = state
+        self.therm_left = 0
+        if resume:
+            info = configio.resume_info(config_name)
+            if info["key"] is not None:
+                self.seed, self.traj = info["key"], info["traj"]
+                # trajectories of the stored chain's thermalisation still to
+                # run with metropolis=False for the chain to continue bit for bit
+                self.therm_left = info["therm_left"]
+            want = 2 if self.prec == F64 else 1
+            if info["precision"] is not None and int(info["precision"]) != want:
+                warnings.warn(f"{config_name} was written at precision {info['precision']}, this chain runs at "
+                              f"{want}: it does not continue bit for bit", stacklevel=2)
         return self
 
     def runHmcUpdate(self, beta, deltas, metropolis=True):

@@ -390,15 +390,30 @@ class Ensemble
 	// a new ensemble directory; an existing non-empty one only with
 	// allow_overwrite, which drops its configurations (the reference's
 	// Hdf5File::create(filename, allow_overwrite))
+	// A new ensemble in `dir`: a missing or empty directory, or -- with
+	// allow_overwrite -- an existing ensemble (a directory holding
+	// attributes.json), whose configurations and data files are removed first.
+	// Anything else is refused, so that a stray path cannot be emptied.
 	static Ensemble create(std::string const &dir, bool allow_overwrite)
 	{
 		namespace fs = std::filesystem;
-		if (fs::exists(dir) && !(fs::is_directory(dir) && fs::is_empty(dir)) && !allow_overwrite)
-			throw std::runtime_error("ensemble '" + dir + "' exists (allow overwriting to replace it)");
+		const fs::path d(dir);
+		if (fs::exists(d) && !(fs::is_directory(d) && fs::is_empty(d)))
+		{
+			if (!allow_overwrite)
+				throw std::runtime_error("ensemble '" + dir + "' exists (allow overwriting to replace it)");
+			if (!fs::is_directory(d) || !fs::is_regular_file(d / "attributes.json"))
+				throw std::runtime_error("'" + dir + "' is not an ensemble (no attributes.json); not overwritten");
+			fs::remove_all(d / "configs");
+			std::vector<fs::path> data;
+			for (auto const &f : fs::directory_iterator(d))
+				if (f.is_regular_file() && f.path().extension() == ".npy")
+					data.push_back(f.path());
+			for (auto const &f : data)
+				fs::remove(f);
+		}
 		Ensemble e;
 		e.dir_ = dir;
-		if (fs::exists(e.dir_ / "configs"))
-			fs::remove_all(e.dir_ / "configs");
 		fs::create_directories(e.dir_ / "configs");
 		e.flush();
 		return e;

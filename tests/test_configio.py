@@ -83,6 +83,34 @@ def test_create_refuses_existing(tmp_path):
         configio.Ensemble.create(str(other))
     assert (other / "configs" / "keep.txt").exists()
     configio.Ensemble.create(str(tmp_path / "empty"))
+    # allow_overwrite replaces an ensemble, histories included, and still
+    # refuses a directory that is not one
+    e = configio.Ensemble.create(d, allow_overwrite=True)
+    e.write_data("plaq_history", [1.0])
+    configio._save(e.config_path("1"), np.zeros(3))
+    configio.Ensemble.create(d, allow_overwrite=True)
+    assert not os.path.exists(os.path.join(d, "plaq_history.npy")) and os.listdir(os.path.join(d, "configs")) == []
+    with pytest.raises(FileExistsError):
+        configio.Ensemble.create(str(other), allow_overwrite=True)
+    assert (other / "configs" / "keep.txt").exists()
+
+
+def test_cpp_create_overwrite(test_io, tmp_path):
+    r = subprocess.run([test_io, "overwrite", str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "overwrite ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_resume_info(tmp_path):
+    """a configuration stored inside the thermalisation leaves the rest of it
+    to run; the stored precision is reported"""
+    e = configio.Ensemble.create(str(tmp_path / "r"))
+    e.set_attribute("philox_key", "77")
+    e.set_attribute("philox_traj0", 3)
+    e.set_attribute("markov_discard", 10)
+    e.set_attribute("precision", 1)
+    i4 = configio.resume_info(os.path.join(e.path, "configs", "4"))
+    assert i4 == dict(key=77, traj=7, therm_left=6, precision=1)
+    assert configio.resume_info(os.path.join(e.path, "configs", "12"))["therm_left"] == 0
 
 
 def test_config_names(tmp_path):

@@ -134,3 +134,22 @@ def test_cpp_u1_chain_resumes_bit_exact(tmp_path):
     assert x.shape == (2, 8, 8, 1) and x.dtype == np.float32
     np.testing.assert_array_equal(x, np.load(C.config_path("3"), allow_pickle=False))
     np.testing.assert_array_equal(A.read_data("deltaH_history")[1:], C.read_data("deltaH_history"))
+
+
+def test_cpp_resume_inside_thermalisation(tmp_path):
+    """A configuration stored during --therm continues with the rest of the
+    thermalisation (no Metropolis) and reproduces the uninterrupted chain bit
+    for bit; a precision different from the stored one is reported."""
+    common = ["--beta", "6", "--scheme", "2mn", "--epsilon", "0.5", "--substeps", "4"]
+    a, c = str(tmp_path / "a"), str(tmp_path / "c")
+    r = _run(["--geom", "4", "4", "4", "4", "--count", "6", "--therm", "4", "--seed", "23", "--filename", a] + common)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = _run(["--start", os.path.join(a, "configs", "2"), "--count", "4", "--filename", c] + common)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "2 thermalisation trajectories left" in r.stdout
+    A, Cn = configio.Ensemble.open(a), configio.Ensemble.open(c)
+    np.testing.assert_array_equal(np.load(A.config_path("6")), np.load(Cn.config_path("4")))
+    # both histories start after the thermalisation: the chain's trajectories 5 and 6
+    np.testing.assert_array_equal(A.read_data("deltaH_history"), Cn.read_data("deltaH_history"))
+    r = _run(["--start", os.path.join(a, "configs", "2"), "--count", "1", "--precision", "1"] + common)
+    assert r.returncode == 0 and "does not continue bit for bit" in r.stderr

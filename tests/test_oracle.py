@@ -63,6 +63,20 @@ def test_make_deltas_known_answers():
         O.make_deltas("3xx", 0.5, 2)
 
 
+def test_make_deltas_2mn_lambda_closed_form():
+    """The 2mn table is (lambda, 1/2, 1 - 2 lambda, 1/2, lambda) tau with the
+    minimum-norm lambda of Omelyan, Mryglod and Folk (2003):
+    lambda = 1/2 - c/12 + 1/(6c), c = (2 sqrt(326) + 36)^(1/3)."""
+    c = (2 * math.sqrt(326) + 36) ** (1 / 3)
+    lam = 0.5 - c / 12 + 1 / (6 * c)
+    d = O.make_deltas("2mn", 0.5, 1)  # tau = sqrt(2 eps) = 1
+    np.testing.assert_allclose(d, [lam, 0.5, 1 - 2 * lam, 0.5, lam], rtol=0, atol=1e-15)
+    d = O.make_deltas("2mn", 2.0, 4)  # tau = 2, four merged substeps
+    h = 2.0 / 4
+    assert d[0] == pytest.approx(lam * h, abs=1e-15) and d[2] == pytest.approx((1 - 2 * lam) * h, abs=1e-15)
+    assert d[4] == pytest.approx(2 * lam * h, abs=1e-15)
+
+
 # ---------------------------------------------------------------- element algebra
 
 def test_exp_against_scipy_expm():
@@ -256,6 +270,18 @@ def test_landau_deriv_finite_difference():
     fd = (O.landau_functional(U, dims, gp) - O.landau_functional(U, dims, gm)) / 2
     expected = -2 * eps * np.einsum("xij,xji->", f, D).real
     assert abs(fd / expected - 1) < 1e-6
+
+
+def test_landau_divsum_is_the_derivative_norm():
+    """theta on a generic field: sum |TA(g d)|^2 = |deriv|^2 (2 Nc Nd V)^2,
+    with deriv pinned by the finite-difference test above."""
+    dims = [4, 4, 2, 4]
+    U = O.random_gauge(dims, 18)
+    g = O.random_transform(dims, 19)
+    D = O.landau_deriv(U, g, dims)
+    c = 2 * 3 * len(dims) * O.volume(dims)
+    assert O.landau_divsum(U, dims, g) == pytest.approx(np.sum(np.abs(D) ** 2) * c * c, rel=1e-12)
+    assert O.landau_divsum(U, dims, g) > 1.0
 
 
 def test_landau_or_pure_gauge_and_monotone():
