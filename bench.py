@@ -287,36 +287,48 @@ def kernel_only(name, prec, L, reps, variant=-1):
                       "launches": n, "copy_bytes_per_launch": 2 * U.bytes}), flush=True)
 
 
-def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pflow", "coupling")):
+def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pflow", "coupling"), substeps=20,
+           start="hot", n_therm=500):
     """U(1) 2-d at L x L: ESS/s of the topological charge Q for plain HMC
-    (2lf, tau = 1, 20 steps) and for HMC interleaved 1:1 with independence
-    MH steps proposed by each trained flow ("pflow": plaquette flow,
-    "coupling": 8 coupling layers).  Training time is reported apart from
-    the sampling time the ESS/s figures use."""
+    (2lf, tau = 1, `substeps` steps) and for HMC interleaved 1:1 with
+    independence MH steps proposed by each trained flow ("pflow": plaquette
+    flow, "coupling": 8 coupling layers).  start: "hot" (uniform angles) or
+    "cold" (unit field) for every chain.  Training time is reported apart
+    from the sampling time the ESS/s figures use."""
     import numpy as np
 
     from mesh_amd import gauge
     from mesh_amd import u1 as U1
 
-    deltas = gauge.makeDeltas("2lf", 0.5, 20)
+    deltas = gauge.makeDeltas("2lf", 0.5, substeps)
     out = {"metric": f"U(1) 2-d ESS/s of Q at L={L} beta={beta}, flow MH + HMC vs HMC",
-           "config": {"workload": f"U(1) {L}x{L} beta={beta} batch={batch}, HMC 2lf tau=1 20 steps, "
-                                  f"flow MH 1:1", "L": L, "beta": beta, "batch": batch},
+           "config": {"workload": f"U(1) {L}x{L} beta={beta} batch={batch}, HMC 2lf tau=1 {substeps} steps, "
+                                  f"{start} start, flow MH 1:1", "L": L, "beta": beta, "batch": batch,
+                      "hmc_substeps": substeps, "start": start},
            "dtype": "f32 angles, f64 energies and log-weights", "data": "synthetic: Philox-seeded starts"}
 
     def summary(r, extra):
         ess, tau, err = U1.ess_per_second(r)
         taup, _ = U1.tau_int(r["plaq"])
-        d = {"ess_per_s": ess, "tau_int_Q": tau, "tau_err": err, "tau_int_plaq": taup,
+        frozen = not math.isfinite(tau)
+        d = {"ess_per_s": ess, "tau_int_Q": None if frozen else tau, "Q_frozen": frozen, "tau_err": err,
+             "tau_int_plaq": taup,
              "acceptance_hmc": r["acc_hmc"], "seconds": r["seconds"], "steps": r["Q"].shape[1],
              "chi_t": float(np.mean(r["Q"] ** 2) / (L * L)), "mean_cos": float(r["plaq"].mean()),
-             "tau_window_capped": bool(2 * tau >= r["Q"].shape[1] / 2)}
+             "tau_window_capped": bool(frozen or 2 * tau >= r["Q"].shape[1] / 2)}
         d.update(extra)
         return d
 
-    ens = U1.U1Ensemble(L, L, batch)
-    ens.random_start(seed)
-    out["hmc"] = summary(U1.run_chains(ens, beta, deltas, hmc_steps, seed, n_therm=500), {})
+    def new_ensemble(s):
+        e = U1.U1Ensemble(L, L, batch)
+        if start == "cold":
+            e.cold_start()
+        else:
+            e.random_start(s)
+        return e
+
+    ens = new_ensemble(seed)
+    out["hmc"] = summary(U1.run_chains(ens, beta, deltas, hmc_steps, seed, n_therm=n_therm), {})
     for kind in flows:
         if kind == "pflow":
             flow = U1.PlaqFlow(L, L)
@@ -328,8 +340,7 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
         hist = flow.train(beta, train_steps, batch=batch, lr=lr, lr_final=lr_final, seed=seed + 1)
         flow.ctx.synchronize()
         t_train = time.perf_counter() - t0
-        ens2 = U1.U1Ensemble(L, L, batch)
-        ens2.random_start(seed + 2)
+        ens2 = new_ensemble(seed + 2)
         r2 = U1.run_chains(ens2, beta, deltas, flow_steps, seed + 3, flow=flow, n_therm=100)
         out[kind] = summary(r2, {"acceptance_flow": r2["acc_flow"], "train_steps": train_steps,
                                  "train_seconds": t_train, "loss_first": float(np.mean(hist[:10])),
@@ -453,6 +464,9 @@ def main():
     ap.add_argument("--u1-hmc-steps", type=int, default=4000)
     ap.add_argument("--u1-flow-steps", type=int, default=1000)
     ap.add_argument("--u1-flows", default="pflow,coupling", help="flows to measure (pflow, coupling)")
+    ap.add_argument("--u1-substeps", type=int, default=20, help="HMC steps per unit trajectory")
+    ap.add_argument("--u1-start", choices=("hot", "cold"), default="hot")
+    ap.add_argument("--u1-therm", type=int, default=500, help="HMC thermalisation trajectories")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -473,7 +487,8 @@ def main():
         return
     if a.u1:
         print(json.dumps(u1_ess(a.u1_L, a.u1_beta, a.u1_batch, a.u1_train, a.u1_hmc_steps, a.u1_flow_steps,
-                                a.seed, tuple(a.u1_flows.split(",")))), flush=True)
+                                a.seed, tuple(a.u1_flows.split(",")), a.u1_substeps, a.u1_start,
+                                a.u1_therm)), flush=True)
         return
     if a.kernel_only:
         kernel_only(a.kernel_only, F64 if a.prec == "f64" else F32, a.L, a.reps, a.force_variant)

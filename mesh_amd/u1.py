@@ -244,13 +244,15 @@ class PlaqFlow:
 def tau_int(series, S=1.5):
     """Integrated autocorrelation time by the Gamma method with Wolff's
     automatic window (U. Wolff, Comput. Phys. Commun. 156 (2004) 143).
-    series: (n,) or (replicas, n).  Returns (tau_int, error)."""
+    series: (n,) or (replicas, n).  Returns (tau_int, error); a series that
+    never changes (a charge frozen in every chain) carries no information:
+    tau_int = inf."""
     x = np.atleast_2d(np.asarray(series, np.float64))
     R, n = x.shape
     xm = x - x.mean()
     var = np.mean(xm * xm)
     if var == 0:
-        return 0.5, 0.0
+        return math.inf, 0.0
     csum = 0.0
     W = 0
     tau = 0.5

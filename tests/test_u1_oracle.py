@@ -149,6 +149,12 @@ def test_tau_int_ar1():
         assert abs(tau - exact) < max(4 * err, 0.05 * exact), (rho, tau, err, exact)
 
 
+def test_tau_int_frozen_series():
+    """a charge that never changes gives no effective samples"""
+    assert tau_int(np.zeros((4, 100)))[0] == math.inf
+
+
+
 def _pflow_params(seed, scale):
     rng = np.random.default_rng(seed)
     p = rng.normal(size=U.NPF) * scale
