@@ -288,11 +288,11 @@ def kernel_only(name, prec, L, reps, variant=-1):
 
 
 def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pflow", "coupling"), substeps=20,
-           start="hot", n_therm=500):
+           start="hot", n_therm=500, layers=8):
     """U(1) 2-d at L x L: ESS/s of the topological charge Q for plain HMC
     (2lf, tau = 1, `substeps` steps) and for HMC interleaved 1:1 with
     independence MH steps proposed by each trained flow ("pflow": plaquette
-    flow, "coupling": 8 coupling layers).  start: "hot" (uniform angles) or
+    flow, "coupling": `layers` coupling layers).  start: "hot" (uniform angles) or
     "cold" (unit field) for every chain.  Training time is reported apart
     from the sampling time the ESS/s figures use."""
     import numpy as np
@@ -334,7 +334,7 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
             flow = U1.PlaqFlow(L, L)
             lr, lr_final = 2e-2, 1e-3
         else:
-            flow = U1.Flow(L, L, seed=seed)
+            flow = U1.Flow(L, L, seed=seed, layers=layers)
             lr, lr_final = 1e-3, 1e-4
         t0 = time.perf_counter()
         hist = flow.train(beta, train_steps, batch=batch, lr=lr, lr_final=lr_final, seed=seed + 1)
@@ -343,6 +343,7 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
         ens2 = new_ensemble(seed + 2)
         r2 = U1.run_chains(ens2, beta, deltas, flow_steps, seed + 3, flow=flow, n_therm=100)
         out[kind] = summary(r2, {"acceptance_flow": r2["acc_flow"], "train_steps": train_steps,
+                                 **({"layers": layers} if kind == "coupling" else {}),
                                  "train_seconds": t_train, "loss_first": float(np.mean(hist[:10])),
                                  "loss_last": float(np.mean(hist[-50:]))})
         out[kind]["ratio_vs_hmc"] = out[kind]["ess_per_s"] / out["hmc"]["ess_per_s"]
@@ -467,6 +468,7 @@ def main():
     ap.add_argument("--u1-substeps", type=int, default=20, help="HMC steps per unit trajectory")
     ap.add_argument("--u1-start", choices=("hot", "cold"), default="hot")
     ap.add_argument("--u1-therm", type=int, default=500, help="HMC thermalisation trajectories")
+    ap.add_argument("--u1-layers", type=int, default=8, help="coupling layers of the coupling flow")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -488,7 +490,7 @@ def main():
     if a.u1:
         print(json.dumps(u1_ess(a.u1_L, a.u1_beta, a.u1_batch, a.u1_train, a.u1_hmc_steps, a.u1_flow_steps,
                                 a.seed, tuple(a.u1_flows.split(",")), a.u1_substeps, a.u1_start,
-                                a.u1_therm)), flush=True)
+                                a.u1_therm, a.u1_layers)), flush=True)
         return
     if a.kernel_only:
         kernel_only(a.kernel_only, F64 if a.prec == "f64" else F32, a.L, a.reps, a.force_variant)

@@ -327,31 +327,33 @@ int mesh_u1_hmc(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, cons
 
 /* ---- U(1) learned proposal (no counterpart in the reference) ---------------
  *
- * Gauge-equivariant coupling-layer flow: 8 layers, each moving one direction's
- * links on every fourth row/column by h(theta_P) - theta_P, with h a mixture of
- * non-compact projections whose parameters an MLP (12 -> 64 -> 64 -> 10, tanh)
- * computes from the frozen plaquettes around the active one.  Lattice extents
- * divisible by 4, V <= 4096.  params: mesh_flow_num_params() floats (device). */
-int mesh_flow_num_params(void);
+ * Gauge-equivariant coupling-layer flow of `layers` layers (8 in the
+ * specified design, 1 to 64), each moving one direction's links on every
+ * fourth row/column by h(theta_P) - theta_P, with h a mixture of non-compact
+ * projections whose parameters an MLP (12 -> 64 -> 64 -> 10, tanh) computes
+ * from the frozen plaquettes around the active one; layer l has direction
+ * l % 2 and row offset (l / 2) % 4.  Lattice extents divisible by 4,
+ * V <= 4096.  params: mesh_flow_num_params(layers) floats (device). */
+int mesh_flow_num_params(int layers);
 /* deterministic initial weights (near the identity transform: last layer
  * 1e-2 of the fan-in scale, bias of the identity) */
-int mesh_flow_init(mesh_ctx *ctx, float *params, uint64_t seed);
+int mesh_flow_init(mesh_ctx *ctx, int layers, float *params, uint64_t seed);
 /* batch samples U = f(z), z uniform from Philox(seed, step):
  * out_dev[2c + {0,1}] = {log q(U), S(U)/beta}; theta_layers may be NULL */
-int mesh_flow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, uint64_t seed, uint32_t step,
-                     float *theta_out, float *theta_layers, double *out_dev);
+int mesh_flow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params, uint64_t seed,
+                     uint32_t step, float *theta_out, float *theta_layers, double *out_dev);
 /* log q of given configurations by the inverse flow (z_out may be NULL):
  * out_dev[2c + {0,1}] = {log q(U), S(U)/beta} */
-int mesh_flow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, const float *theta, float *z_out,
-                   double *out_dev);
+int mesh_flow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params, const float *theta,
+                   float *z_out, double *out_dev);
 /* bytes of the workspace mesh_flow_gradient needs */
-size_t mesh_flow_workspace_bytes(const mesh_grid *g, int batch);
+size_t mesh_flow_workspace_bytes(const mesh_grid *g, int batch, int layers);
 /* reverse-KL gradient: grad = (1/batch) sum_c d/dparams [beta S(f(z_c)) - log det f'(z_c)]
  * over a fresh batch (Philox(seed, step)); out_dev as mesh_flow_sample */
-int mesh_flow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, uint64_t seed, uint32_t step,
-                       double beta, void *workspace, float *grad, double *out_dev);
+int mesh_flow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params, uint64_t seed,
+                       uint32_t step, double beta, void *workspace, float *grad, double *out_dev);
 /* Adam step t >= 1 (moments m, v zero-initialised by the caller) */
-int mesh_flow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, float *v, int t, double lr);
+int mesh_flow_adam(mesh_ctx *ctx, int layers, float *params, const float *grad, float *m, float *v, int t, double lr);
 /* independence Metropolis-Hastings: per chain accept the proposal with
  * probability min(1, exp[(-beta S' - log q') - (-beta S - log q)]); state and
  * prop hold {log q, S/beta} per chain; acc_dev[c] = 1 on acceptance.  Any 2-d

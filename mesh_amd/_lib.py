@@ -114,13 +114,13 @@ _SIGS = {
     "mesh_u1_random": (i32, [vp, gp, i32, vp, u64]),
     "mesh_u1_observables": (i32, [vp, gp, i32, vp, vp]),
     "mesh_u1_hmc": (i32, [vp, gp, i32, vp, dp, i32, f64, u64, u32, i32, vp]),
-    "mesh_flow_num_params": (i32, []),
-    "mesh_flow_init": (i32, [vp, vp, u64]),
-    "mesh_flow_sample": (i32, [vp, gp, i32, vp, u64, u32, vp, vp, vp]),
-    "mesh_flow_logq": (i32, [vp, gp, i32, vp, vp, vp, vp]),
-    "mesh_flow_workspace_bytes": (C.c_size_t, [gp, i32]),
-    "mesh_flow_gradient": (i32, [vp, gp, i32, vp, u64, u32, f64, vp, vp, vp]),
-    "mesh_flow_adam": (i32, [vp, vp, vp, vp, vp, i32, f64]),
+    "mesh_flow_num_params": (i32, [i32]),
+    "mesh_flow_init": (i32, [vp, i32, vp, u64]),
+    "mesh_flow_sample": (i32, [vp, gp, i32, i32, vp, u64, u32, vp, vp, vp]),
+    "mesh_flow_logq": (i32, [vp, gp, i32, i32, vp, vp, vp, vp]),
+    "mesh_flow_workspace_bytes": (C.c_size_t, [gp, i32, i32]),
+    "mesh_flow_gradient": (i32, [vp, gp, i32, i32, vp, u64, u32, f64, vp, vp, vp]),
+    "mesh_flow_adam": (i32, [vp, i32, vp, vp, vp, vp, i32, f64]),
     "mesh_flow_mh": (i32, [vp, gp, i32, vp, vp, vp, vp, f64, u64, u32, vp]),
     "mesh_pflow_num_params": (i32, []),
     "mesh_pflow_init": (i32, [vp, vp]),

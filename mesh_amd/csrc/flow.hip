@@ -2,7 +2,8 @@
 // independence proposal beside HMC (no counterpart in the reference; the
 // U(1) conventions are those of u1.hip).
 //
-// Layer l (l = 0..7): direction mu = l % 2, offset a = (l / 2) % 4, nu = 1 - mu.
+// Layer l (8 by default, any count up to 64): direction mu = l % 2, offset
+// a = (l / 2) % 4, nu = 1 - mu (the pattern repeats every 8 layers).
 //  * active plaquettes P(y) with y_nu = a (mod 4); each contains exactly one
 //    updated link: U_0(y) for mu = 0 (bottom edge, sign +), U_1(y + 0) for
 //    mu = 1 (right edge, sign +);
@@ -29,7 +30,7 @@ using namespace mesh_dev;
 
 namespace {
 
-constexpr int NL = 8;                                   // layers
+constexpr int MAX_LAYERS = 64;                          // layers: 8 by default, up to 64
 constexpr int NK = 4;                                   // NCP components
 constexpr int NI = 12;                                  // features
 constexpr int NH = 64;                                  // hidden width
@@ -278,9 +279,9 @@ __device__ void mlp_round(FGeo const &g, const float *th, int mu, int a, int it,
 
 // z ~ uniform -> U.  theta_layers (optional): input of every layer, for the
 // backward pass.  out[c*2 + {0,1}] = {log q(U), S(U)/beta = sum(1 - cos)}.
-__global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, const float *__restrict__ params, uint64_t seed,
-                                                          uint32_t step, float *theta_out, float *theta_layers,
-                                                          double *out)
+__global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, int nl, const float *__restrict__ params,
+                                                          uint64_t seed, uint32_t step, float *theta_out,
+                                                          float *theta_layers, double *out)
 {
 	extern __shared__ float th[]; // 2V
 	__shared__ FlowShared sh;
@@ -296,13 +297,13 @@ __global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, const float *_
 	}
 	double logdet = 0.0;
 	const int NT = (g.NA + 15) / 16;
-	for (int layer = 0; layer < NL; ++layer)
+	for (int layer = 0; layer < nl; ++layer)
 	{
 		const int mu = layer & 1, a = (layer >> 1) & 3;
 		__syncthreads();
 		if (theta_layers)
 			for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
-				theta_layers[((long)c * NL + layer) * 2 * g.V + i] = th[i];
+				theta_layers[((long)c * nl + layer) * 2 * g.V + i] = th[i];
 		load_weights(sh.lw, params + layer * NP_LAYER);
 		__syncthreads();
 		// within a layer the MLP inputs (frozen plaquettes) never change, so
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, const float *_
 
 // U -> z and log q(U), layers in reverse with g^-1 by safeguarded Newton.
 // out[c*2 + {0,1}] = {log q(U), S(U)/beta}
-__global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, const float *__restrict__ params,
+__global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, int nl, const float *__restrict__ params,
                                                           const float *__restrict__ theta_in, float *z_out, double *out)
 {
 	extern __shared__ float th[];
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, const float *_
 	for (int x = threadIdx.x; x < g.V; x += THREADS)
 		v[1] += 1.0 - cos((double)plaqf(g, th, x / g.L1, x % g.L1));
 	const int NT = (g.NA + 15) / 16;
-	for (int layer = NL - 1; layer >= 0; --layer)
+	for (int layer = nl - 1; layer >= 0; --layer)
 	{
 		const int mu = layer & 1, a = (layer >> 1) & 3;
 		__syncthreads();
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, const float *_
 // with respect to every weight; partial[c][layer * NP_LAYER + i].
 // theta_layers: input of each layer (from k_flow_forward); theta_final: output;
 // dxs: scratch of NA * NI floats per configuration.
-__global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, const float *__restrict__ params,
+__global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, int nl, const float *__restrict__ params,
                                                            const float *__restrict__ theta_layers,
                                                            const float *__restrict__ theta_final, float beta,
                                                            float *__restrict__ dxs, float *__restrict__ partial)
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, const float *
 	const int c = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
 	const int lc = l & 15, rq = (l >> 4) * 4;
 	float *dx = dxs + (long)c * g.NA * NI;
-	float *part = partial + (long)c * NL * NP_LAYER;
+	float *part = partial + (long)c * nl * NP_LAYER;
 	// adjoint of the final links: dS/dtheta
 	for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
 		th[i] = theta_final[(long)c * 2 * g.V + i];
@@ -478,12 +479,12 @@ __global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, const float *
 		ab[g.V + x] = beta * (gp[site(g, x0 - 1, x1)] - gp[x]);
 	}
 	const int NT = (g.NA + 15) / 16;
-	for (int layer = NL - 1; layer >= 0; --layer)
+	for (int layer = nl - 1; layer >= 0; --layer)
 	{
 		const int mu = layer & 1, a = (layer >> 1) & 3;
 		__syncthreads();
 		for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
-			th[i] = theta_layers[((long)c * NL + layer) * 2 * g.V + i];
+			th[i] = theta_layers[((long)c * nl + layer) * 2 * g.V + i];
 		load_weights(sh.lw, params + layer * NP_LAYER);
 		__syncthreads();
 		f4 aw1[4], aw2[16], aw3[4];
@@ -698,14 +699,15 @@ __global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, const float *
 }
 
 // grad[i] = scale * sum_c partial[c][i] (fixed order)
-__global__ void k_flow_reduce(const float *__restrict__ partial, int batch, float scale, float *__restrict__ grad)
+__global__ void k_flow_reduce(int n, const float *__restrict__ partial, int batch, float scale,
+                              float *__restrict__ grad)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= NL * NP_LAYER)
+	if (i >= n)
 		return;
 	double s = 0.0;
 	for (int c = 0; c < batch; ++c)
-		s += partial[(long)c * NL * NP_LAYER + i];
+		s += partial[(long)c * n + i];
 	grad[i] = (float)(s * scale);
 }
 
@@ -727,10 +729,10 @@ __global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ g
 // initial weights: W ~ N(0, 1/fan_in) for the hidden layers, the last layer
 // 1e-2 of that (identical components would receive identical gradients for
 // ever) with the bias of the identity transform (s_k = 1, w uniform, t = 0)
-__global__ void k_flow_init(float *p, uint64_t seed)
+__global__ void k_flow_init(int n, float *p, uint64_t seed)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= NL * NP_LAYER)
+	if (i >= n)
 		return;
 	int o = i % NP_LAYER;
 	float v = 0.f;
@@ -786,8 +788,10 @@ __global__ __launch_bounds__(THREADS) void k_flow_mh(FGeo g, float *theta, doubl
 			theta[(long)c * 2 * g.V + i] = prop_theta[(long)c * 2 * g.V + i];
 }
 
-int flow_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, FGeo *fg)
+int flow_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, FGeo *fg)
 {
+	if (layers < 1 || layers > MAX_LAYERS)
+		return mesh_fail(ctx, MESH_E_ARG, "the flow has 1 to 64 layers");
 	if (!g || g->nd != 2 || g->dims[0] % 4 || g->dims[1] % 4 || g->dims[0] < 4 || g->dims[1] < 4)
 		return mesh_fail(ctx, MESH_E_SIZE, "the flow needs a 2-d lattice with extents divisible by 4");
 	long V = (long)g->dims[0] * g->dims[1];
@@ -806,89 +810,90 @@ int flow_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, FGeo *fg)
 
 extern "C" {
 
-int mesh_flow_num_params(void) { return NL * NP_LAYER; }
+int mesh_flow_num_params(int layers) { return layers >= 1 && layers <= MAX_LAYERS ? layers * NP_LAYER : 0; }
 
-int mesh_flow_init(mesh_ctx *ctx, float *params, uint64_t seed)
+int mesh_flow_init(mesh_ctx *ctx, int layers, float *params, uint64_t seed)
 {
 	MESH_CHECK_CTX(ctx);
-	if (!params)
-		return mesh_fail(ctx, MESH_E_ARG, "params is NULL");
-	int n = NL * NP_LAYER;
-	k_flow_init<<<(n + 255) / 256, 256, 0, ctx->stream>>>(params, seed);
+	if (!params || layers < 1 || layers > MAX_LAYERS)
+		return mesh_fail(ctx, MESH_E_ARG, "bad flow_init arguments");
+	int n = layers * NP_LAYER;
+	k_flow_init<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, params, seed);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
 
-int mesh_flow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, uint64_t seed, uint32_t step,
-                     float *theta_out, float *theta_layers, double *out_dev)
+int mesh_flow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params, uint64_t seed,
+                     uint32_t step, float *theta_out, float *theta_layers, double *out_dev)
 {
 	MESH_CHECK_CTX(ctx);
 	FGeo fg;
-	if (int rc = flow_setup(ctx, g, batch, &fg))
+	if (int rc = flow_setup(ctx, g, batch, layers, &fg))
 		return rc;
 	if (!params || !theta_out || !out_dev)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to flow_sample");
 	MESH_HIP(ctx, hipFuncSetAttribute((const void *)k_flow_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
 	                                  (int)(2 * fg.V * sizeof(float))));
-	k_flow_forward<<<batch, THREADS, 2 * fg.V * sizeof(float), ctx->stream>>>(fg, params, seed, step, theta_out,
-	                                                                          theta_layers, out_dev);
+	k_flow_forward<<<batch, THREADS, 2 * fg.V * sizeof(float), ctx->stream>>>(fg, layers, params, seed, step,
+	                                                                          theta_out, theta_layers, out_dev);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
 
-int mesh_flow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, const float *theta, float *z_out,
-                   double *out_dev)
+int mesh_flow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params, const float *theta,
+                   float *z_out, double *out_dev)
 {
 	MESH_CHECK_CTX(ctx);
 	FGeo fg;
-	if (int rc = flow_setup(ctx, g, batch, &fg))
+	if (int rc = flow_setup(ctx, g, batch, layers, &fg))
 		return rc;
 	if (!params || !theta || !out_dev)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to flow_logq");
 	MESH_HIP(ctx, hipFuncSetAttribute((const void *)k_flow_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
 	                                  (int)(2 * fg.V * sizeof(float))));
-	k_flow_inverse<<<batch, THREADS, 2 * fg.V * sizeof(float), ctx->stream>>>(fg, params, theta, z_out, out_dev);
+	k_flow_inverse<<<batch, THREADS, 2 * fg.V * sizeof(float), ctx->stream>>>(fg, layers, params, theta, z_out,
+	                                                                          out_dev);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
 
-size_t mesh_flow_workspace_bytes(const mesh_grid *g, int batch)
+size_t mesh_flow_workspace_bytes(const mesh_grid *g, int batch, int layers)
 {
 	FGeo fg;
-	if (flow_setup(nullptr, g, batch, &fg))
+	if (flow_setup(nullptr, g, batch, layers, &fg))
 		return 0;
-	size_t n = (size_t)batch * NL * 2 * fg.V       // theta per layer
-	           + (size_t)batch * fg.NA * NI        // feature adjoints
-	           + (size_t)batch * NL * NP_LAYER     // per-configuration gradients
+	size_t n = (size_t)batch * layers * 2 * fg.V       // theta per layer
+	           + (size_t)batch * fg.NA * NI            // feature adjoints
+	           + (size_t)batch * layers * NP_LAYER     // per-configuration gradients
 	           + (size_t)batch * 2 * fg.V;         // samples
 	return n * sizeof(float) + (size_t)batch * 2 * sizeof(double);
 }
 
-int mesh_flow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, uint64_t seed, uint32_t step,
-                       double beta, void *workspace, float *grad, double *out_dev)
+int mesh_flow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params, uint64_t seed,
+                       uint32_t step, double beta, void *workspace, float *grad, double *out_dev)
 {
 	MESH_CHECK_CTX(ctx);
 	FGeo fg;
-	if (int rc = flow_setup(ctx, g, batch, &fg))
+	if (int rc = flow_setup(ctx, g, batch, layers, &fg))
 		return rc;
 	if (!params || !workspace || !grad || !out_dev)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to flow_gradient");
 	float *tl = (float *)workspace;
-	float *dxs = tl + (size_t)batch * NL * 2 * fg.V;
+	float *dxs = tl + (size_t)batch * layers * 2 * fg.V;
 	float *part = dxs + (size_t)batch * fg.NA * NI;
-	float *samp = part + (size_t)batch * NL * NP_LAYER;
+	float *samp = part + (size_t)batch * layers * NP_LAYER;
 	MESH_HIP(ctx, hipFuncSetAttribute((const void *)k_flow_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
 	                                  (int)(2 * fg.V * sizeof(float))));
-	k_flow_forward<<<batch, THREADS, 2 * fg.V * sizeof(float), ctx->stream>>>(fg, params, seed, step, samp, tl,
-	                                                                          out_dev);
+	k_flow_forward<<<batch, THREADS, 2 * fg.V * sizeof(float), ctx->stream>>>(fg, layers, params, seed, step, samp,
+	                                                                          tl, out_dev);
 	MESH_LAUNCHED(ctx);
 	size_t lds = (5 * (size_t)fg.V + fg.NA) * sizeof(float);
 	MESH_HIP(ctx, hipFuncSetAttribute((const void *)k_flow_backward, hipFuncAttributeMaxDynamicSharedMemorySize,
 	                                  (int)lds));
-	k_flow_backward<<<batch, THREADS, lds, ctx->stream>>>(fg, params, tl, samp, (float)beta, dxs, part);
+	k_flow_backward<<<batch, THREADS, lds, ctx->stream>>>(fg, layers, params, tl, samp, (float)beta, dxs, part);
 	MESH_LAUNCHED(ctx);
-	int n = NL * NP_LAYER;
-	k_flow_reduce<<<(n + 255) / 256, 256, 0, ctx->stream>>>(part, batch, 1.0f / batch, grad);
+	int n = layers * NP_LAYER;
+	k_flow_reduce<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, part, batch, 1.0f / batch, grad);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
@@ -913,12 +918,12 @@ int mesh_flow_mh(mesh_ctx *ctx, const mesh_grid *g, int batch, float *theta, dou
 	return MESH_OK;
 }
 
-int mesh_flow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, float *v, int t, double lr)
+int mesh_flow_adam(mesh_ctx *ctx, int layers, float *params, const float *grad, float *m, float *v, int t, double lr)
 {
 	MESH_CHECK_CTX(ctx);
-	if (!params || !grad || !m || !v || t < 1)
+	if (!params || !grad || !m || !v || t < 1 || layers < 1 || layers > MAX_LAYERS)
 		return mesh_fail(ctx, MESH_E_ARG, "bad flow_adam arguments");
-	return mesh_launch_adam(ctx, NL * NP_LAYER, params, grad, m, v, t, lr);
+	return mesh_launch_adam(ctx, layers * NP_LAYER, params, grad, m, v, t, lr);
 }
 
 } // extern "C"

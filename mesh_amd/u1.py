@@ -102,13 +102,17 @@ class U1Ensemble:
 
 
 class Flow:
-    """Gauge-equivariant coupling-layer flow on an L0 x L1 lattice."""
+    """Gauge-equivariant coupling-layer flow on an L0 x L1 lattice (8 layers
+    in the specified design; `layers` up to 64)."""
 
-    def __init__(self, L0, L1, ctx=None, seed=0):
+    def __init__(self, L0, L1, ctx=None, seed=0, layers=8):
         self.ctx = ctx or default_context()
         self.grid = Grid((L0, L1))
         self.L0, self.L1, self.V = L0, L1, L0 * L1
-        self.n = lib().mesh_flow_num_params()
+        self.layers = int(layers)
+        self.n = lib().mesh_flow_num_params(self.layers)
+        if self.n <= 0:
+            raise MeshError(-1, "the flow has 1 to 64 layers")
         self.params = Buffer(self.n, np.float32, self.ctx)
         self.m = Buffer(self.n, np.float32, self.ctx).zero()
         self.v = Buffer(self.n, np.float32, self.ctx).zero()
@@ -116,7 +120,7 @@ class Flow:
         self.t = 0
         self._ws = None
         self._ws_batch = 0
-        self.ctx.call("mesh_flow_init", self.params.ptr, C.c_uint64(seed))
+        self.ctx.call("mesh_flow_init", self.layers, self.params.ptr, C.c_uint64(seed))
 
     def get_params(self):
         return self.params.download()
@@ -126,7 +130,7 @@ class Flow:
 
     def _workspace(self, batch):
         if self._ws is None or self._ws_batch != batch:
-            nb = lib().mesh_flow_workspace_bytes(self.grid.ref, batch)
+            nb = lib().mesh_flow_workspace_bytes(self.grid.ref, batch, self.layers)
             if nb == 0:
                 raise MeshError(-2, "unsupported flow geometry")
             self._ws = Buffer(nb, np.uint8, self.ctx)
@@ -137,7 +141,7 @@ class Flow:
     def gradient(self, batch, beta, seed, step):
         """Reverse-KL gradient over a fresh batch; returns (loss, log q, S/beta)."""
         ws = self._workspace(batch)
-        self.ctx.call("mesh_flow_gradient", self.grid.ref, batch, self.params.ptr, C.c_uint64(seed),
+        self.ctx.call("mesh_flow_gradient", self.grid.ref, batch, self.layers, self.params.ptr, C.c_uint64(seed),
                       C.c_uint32(step), _d(beta), ws.ptr, self.grad.ptr, self._out.ptr)
         o = self._out.download().reshape(batch, 2)
         logq, S = o[:, 0], o[:, 1]
@@ -151,7 +155,7 @@ class Flow:
             loss, _, _ = self.gradient(batch, beta, seed, self.t)
             self.t += 1
             lr_i = lr if lr_final is None else lr * (lr_final / lr) ** (i / max(1, steps - 1))
-            self.ctx.call("mesh_flow_adam", self.params.ptr, self.grad.ptr, self.m.ptr, self.v.ptr, self.t,
+            self.ctx.call("mesh_flow_adam", self.layers, self.params.ptr, self.grad.ptr, self.m.ptr, self.v.ptr, self.t,
                           _d(lr_i))
             hist.append(loss)
             if log_every and (i % log_every == 0 or i == steps - 1):
@@ -163,13 +167,13 @@ class Flow:
         per-chain (log q, S/beta) as a device buffer of 2*batch doubles."""
         tgt = theta_out if theta_out is not None else ens.theta
         out = out or Buffer(2 * ens.batch, np.float64, self.ctx)
-        self.ctx.call("mesh_flow_sample", self.grid.ref, ens.batch, self.params.ptr, C.c_uint64(seed),
+        self.ctx.call("mesh_flow_sample", self.grid.ref, ens.batch, self.layers, self.params.ptr, C.c_uint64(seed),
                       C.c_uint32(step), tgt.ptr, None, out.ptr)
         return out
 
     def logq(self, ens, out=None, z_out=None):
         out = out or Buffer(2 * ens.batch, np.float64, self.ctx)
-        self.ctx.call("mesh_flow_logq", self.grid.ref, ens.batch, self.params.ptr, ens.theta.ptr,
+        self.ctx.call("mesh_flow_logq", self.grid.ref, ens.batch, self.layers, self.params.ptr, ens.theta.ptr,
                       None if z_out is None else z_out.ptr, out.ptr)
         return out
 

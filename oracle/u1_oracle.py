@@ -265,7 +265,8 @@ def flow_forward(params, z, inverse=False):
     x0 = torch.arange(L0).view(L0, 1).expand(L0, L1)
     x1 = torch.arange(L1).view(1, L1).expand(L0, L1)
     logdet = torch.zeros(B, dtype=th.dtype)
-    for layer in range(NL):
+    nl = params.numel() // NP_LAYER  # 8 in the specified design
+    for layer in range(nl):
         mu, a = layer & 1, (layer >> 1) & 3
         W1, b1, W2, b2, W3, b3 = _layer_params(params, layer)
         P = _tplaq(th)

@@ -27,10 +27,10 @@ def _wrapdiff(a, b):
     return np.abs(U.wrap(np.asarray(a, np.float64) - np.asarray(b, np.float64)))
 
 
-def _params(seed, scale):
+def _params(seed, scale, layers=U.NL):
     rng = np.random.default_rng(seed)
-    p = rng.normal(size=U.NL * U.NP_LAYER) * scale
-    for layer in range(U.NL):
+    p = rng.normal(size=layers * U.NP_LAYER) * scale
+    for layer in range(layers):
         p[layer * U.NP_LAYER + U.NP_LAYER - U.NO + 2 * U.NK] += 1.0
     return p.astype(np.float32)
 
@@ -193,6 +193,36 @@ def test_flow_forward_against_oracle(ctx, L0, L1):
         assert _wrapdiff(th[c], t_ref[c]).max() < 2e-3, c
     np.testing.assert_allclose(out[:, 0], U.log_prior(L0, L1) - ld, atol=tol_lq)
     np.testing.assert_allclose(out[:, 1], U.action_over_beta(th.astype(np.float64)), rtol=1e-4)
+
+
+def test_flow_deeper_stack_against_oracle(ctx):
+    """a 20-layer stack (the 8-layer pattern repeated): samples, log q by the
+    inverse flow and the reverse-KL gradient against the torch float64 oracle"""
+    torch = pytest.importorskip("torch")
+    L, B, seed, step, nl, beta = 8, 4, 51, 3, 20, 2.0
+    p = _params(52, 0.15, nl)
+    f = U1.Flow(L, L, ctx, layers=nl)
+    assert f.n == nl * U.NP_LAYER
+    f.set_params(p)
+    ens = U1.U1Ensemble(L, L, B, ctx)
+    out = f.sample(ens, seed, step).download().reshape(B, 2)
+    th = ens.angles()
+    z = torch.tensor(np.stack([U.flow_z(seed, step, c, L, L) for c in range(B)]).astype(np.float64))
+    t_ref, ld = U.flow_forward(torch.tensor(p.astype(np.float64)), z)
+    for c in range(B):
+        assert _wrapdiff(th[c], t_ref[c].numpy()).max() < 2e-3, c
+    np.testing.assert_allclose(out[:, 0], U.log_prior(L, L) - ld.numpy(), atol=1e-2)
+    inv = f.logq(ens).download().reshape(B, 2)
+    np.testing.assert_allclose(inv[:, 0], out[:, 0], atol=3e-3)
+    f.gradient(B, beta, seed, step)
+    g = f.grad.download().astype(np.float64)
+    pt = torch.tensor(p.astype(np.float64), requires_grad=True)
+    ref_loss, _, _, _ = U.flow_loss(pt, z, beta)
+    ref_loss.backward()
+    gr = pt.grad.numpy()
+    assert np.linalg.norm(g - gr) / np.linalg.norm(gr) < 2e-3
+    with pytest.raises(mesh_amd.MeshError):
+        U1.Flow(L, L, ctx, layers=65)
 
 
 def test_flow_inverse_roundtrip(ctx):
