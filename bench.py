@@ -353,6 +353,87 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
     return out
 
 
+# SU(2) algorithmic bytes per site (R bytes per real, 4 directions of 4-real
+# quaternions; momenta stored as quaternions with w = 0):
+#   force_update: read U, P; write P     48 R
+#   link_update:  read U, P; write U'    48 R
+#   momenta:      write P, K             16 R + 8
+#   delta_h:      read U, U', P, K       48 R + 8
+#   landau_or:    one half-sweep reads and writes every link once (each link
+#                 joins a site of each parity) and the transformation of
+#                 half the sites: 32 R + 4 R
+#   force_link:   the two fused (the trajectory's default): read U, P;
+#                 write P, U'                                       64 R
+SU2_BYTES = {"force_update": (48, 0), "link_update": (48, 0), "momenta": (16, 8), "delta_h": (48, 8),
+             "landau_or": (36, 0), "force_link": (64, 0)}
+
+
+def su2_bench(L, steps, warmup, therm, seed, beta=2.3, scheme="2mn", eps=0.5, substeps=12, landau_sweeps=2000):
+    """SU(2) pure-gauge HMC (Hmc<SU2<T>>) at L^4 in FP64 and FP32: trajectories
+    per second as site-updates/s, each kernel's average launch time (HIP events
+    on the library's stream) and its fraction of 8 TB/s on the algorithmic
+    bytes above; then Landau overrelaxation of the thermalised field to
+    theta < 1e-12 (or landau_sweeps sweeps)."""
+    import mesh_amd
+    from mesh_amd import F32, F64, gauge
+    from mesh_amd import su2 as S2
+
+    ctx = mesh_amd.default_context()
+    g = mesh_amd.Grid((L,) * 4)
+    V = g.size()
+    d = gauge.makeDeltas(scheme, eps, substeps)
+    out = {"metric": f"SU(2) HMC site-updates/s at {L}^4", "unit": "site-updates/s", "higher_is_better": True,
+           "config": {"workload": f"SU(2) {L}^4 beta={beta} {scheme} eps={eps} {substeps} substeps, "
+                                  f"{therm} thermalisation trajectories", "L": L, "beta": beta},
+           "data": "synthetic: Philox-seeded Haar start"}
+    for prec, name in ((F64, "f64"), (F32, "f32")):
+        R = 8 if prec == F64 else 4
+        h = S2.Hmc(g, prec, seed=seed, ctx=ctx)
+        h.randomizeGaugeField()
+        for _ in range(therm):
+            h.runHmcUpdate(beta, d, metropolis=False)
+        for _ in range(warmup):
+            h.runHmcUpdate(beta, d)
+        h.accept_history.clear()
+        h.plaq_history.clear()
+        ctx.synchronize()
+        ctx.profile(True)
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            h.runHmcUpdate(beta, d)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        kern = {}
+        for k in ("force_link", "force_update", "link_update", "momenta", "delta_h"):
+            ms, n = ctx.profile_read(k)
+            a, b = SU2_BYTES[k]
+            avg = ms / n if n else None
+            kern[k] = {"avg_ms": avg, "launches": n,
+                       "frac_of_hbm": (a * R + b) * V / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS if avg else None}
+        ctx.profile(False)
+        lan = S2.Landau(h.U)
+        ctx.profile(True)
+        ctx.profile_reset()
+        t1 = time.perf_counter()
+        sweeps, theta = lan.run(landau_sweeps, omega=1.7, tol=1e-12 if prec == F64 else 1e-9)
+        ctx.synchronize()
+        tl = time.perf_counter() - t1
+        ms, n = ctx.profile_read("landau_or")
+        ctx.profile(False)
+        half = ms / n if n else None
+        out[name] = {"value": V * steps / dt, "ms_per_step": 1e3 * dt / steps,
+                     "acceptance": float(sum(h.accept_history) / len(h.accept_history)),
+                     "plaquette": float(sum(h.plaq_history) / len(h.plaq_history)), "kernels": kern,
+                     "landau": {"sweeps": sweeps, "theta": theta, "seconds": tl,
+                                "site_sweeps_per_s": V * sweeps / tl, "half_sweep_ms": half,
+                                "frac_of_hbm": (SU2_BYTES["landau_or"][0] * R * V) / (half * 1e-3) / 1e9
+                                / HBM_PEAK_GBS if half else None}}
+    out["value"] = out["f64"]["value"]
+    out["dtype"] = "f64 (f32 beside it)"
+    return out
+
+
 def landau_bench(L, seed, therm, max_sweeps, omega=1.7, tol=1e-12, ascent_iters=0):
     """Landau gauge fixing of a thermalised SU(3) L^4 configuration (HMC as in
     the main workload, `therm` trajectories) by even/odd overrelaxation, in
@@ -456,6 +537,7 @@ def main():
     ap.add_argument("--landau-ascent", type=int, default=0, metavar="N",
                     help="with --landau: also the reference's line-search solver (at most N iterations) "
                          "against overrelaxation, both to condition() < 1e-10")
+    ap.add_argument("--su2", action="store_true", help="SU(2) HMC and Landau at --L (its own JSON line)")
     ap.add_argument("--u1", action="store_true",
                     help="instead: U(1) 2-d ESS/s of Q, flow-assisted vs plain HMC (one JSON line)")
     ap.add_argument("--u1-L", type=int, default=32)
@@ -486,6 +568,9 @@ def main():
     if a.landau:
         print(json.dumps(landau_bench(a.L, a.seed, a.therm, a.landau_sweeps, ascent_iters=a.landau_ascent)),
               flush=True)
+        return
+    if a.su2:
+        print(json.dumps(su2_bench(a.L, a.steps, a.warmup, a.therm, a.seed)), flush=True)
         return
     if a.u1:
         print(json.dumps(u1_ess(a.u1_L, a.u1_beta, a.u1_batch, a.u1_train, a.u1_hmc_steps, a.u1_flow_steps,

@@ -19,8 +19,10 @@
 // Boltzmann weight those momenta sample (the reference adds the quaternion's
 // own norm q1^2 + q2^2 + q3^2, half of it).
 //
-// Layout: structure of arrays, plane (f, c) = component c of direction f for
-// every site, base[(4 f + c) stride + x] with stride = V rounded up to 256.
+// Layout: per direction f the sites' quaternions in site order, component c
+// of site x at base[4 (f stride + x) + c], stride = V rounded up to 256 (a
+// quaternion is one 16-byte vector load in FP32, two in FP64; lanes of a
+// wavefront read consecutive quaternions).
 //
 // Every multiply-add is an explicit fma in a fixed order and cos / sin of the
 // exponential are power series in alpha^2 (no library transcendentals), so
@@ -118,26 +120,29 @@ struct G2
 	long V, stride;
 };
 
+// neighbour x + s mu^ (|s| <= L_mu); 32-bit arithmetic (V <= 2^26 sites): a
+// 64-bit division is a long emulated sequence on the GPU
 __device__ __forceinline__ long nbr(G2 const &g, long x, int mu, int s)
 {
-	int c = (int)((x / g.st[mu]) % g.L[mu]);
+	const unsigned xs = (unsigned)x;
+	int c = (int)((xs / (unsigned)g.st[mu]) % (unsigned)g.L[mu]);
 	int n = c + s;
 	n = n < 0 ? n + g.L[mu] : (n >= g.L[mu] ? n - g.L[mu] : n);
-	return x + (long)(n - c) * g.st[mu];
+	return (long)((int)xs + (n - c) * g.st[mu]);
 }
 
+// quaternion (w, x, y, z) of direction f at site x: four consecutive reals,
+// one vector load (two for FP64)
+template <class R> using v4 = R __attribute__((ext_vector_type(4)));
 template <class R> __device__ __forceinline__ qt<R> ldq(const R *__restrict__ b, G2 const &g, int f, long x)
 {
-	const R *p = b + (long)(4 * f) * g.stride + x;
-	return {p[0], p[g.stride], p[2 * g.stride], p[3 * g.stride]};
+	const v4<R> v = *reinterpret_cast<const v4<R> *>(b + 4 * ((long)f * g.stride + x));
+	return {v.x, v.y, v.z, v.w};
 }
 template <class R> __device__ __forceinline__ void stq(R *__restrict__ b, G2 const &g, int f, long x, qt<R> q)
 {
-	R *p = b + (long)(4 * f) * g.stride + x;
-	p[0] = q.w;
-	p[g.stride] = q.x;
-	p[2 * g.stride] = q.y;
-	p[3 * g.stride] = q.z;
+	v4<R> v = {q.w, q.x, q.y, q.z};
+	*reinterpret_cast<v4<R> *>(b + 4 * ((long)f * g.stride + x)) = v;
 }
 
 // staple sum of direction mu at x (src/gauge/utils.h:110-139), planes in
@@ -235,6 +240,31 @@ __global__ __launch_bounds__(256) void k_su2_force_update(G2 g, const R *__restr
 	stq(P, g, mu, x, qround<R>({p.w, fma(-k, M.x, p.x), fma(-k, M.y, p.y), fma(-k, M.z, p.z)}));
 }
 
+// the momentum step of link (mu, x) followed in the same thread by its link
+// step Unext_mu(x) = exp(t P_new) U_mu(x) (+ reunitarisation): the arithmetic
+// of k_su2_force_update and k_su2_link_update, one pass over the field
+// (Unext != U: neighbouring threads still read U)
+template <class R>
+__global__ __launch_bounds__(256) void k_su2_force_link(G2 g, const R *__restrict__ U, R *__restrict__ P, double k,
+                                                        R *__restrict__ Unext, R t, int reunit)
+{
+	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= g.V * g.nd)
+		return;
+	int mu = (int)((unsigned)i / (unsigned)g.V);
+	long x = i - (long)mu * g.V;
+	qt<R> u = ldq(U, g, mu, x);
+	qt<double> M = q64(qmul(u, staple(g, U, mu, x)));
+	qt<double> p = q64(ldq(P, g, mu, x));
+	qt<R> pn = qround<R>({p.w, fma(-k, M.x, p.x), fma(-k, M.y, p.y), fma(-k, M.z, p.z)});
+	stq(P, g, mu, x, pn);
+	qt<R> D = q_expm1(pn.x * t, pn.y * t, pn.z * t);
+	qt<R> un = qadd(u, qmul(D, u));
+	if (reunit)
+		un = q_reunit(un);
+	stq(Unext, g, mu, x, un);
+}
+
 // Uout = exp(t project_on_algebra(P)) Uin, as Uin + (exp - 1) Uin (+ reunitarisation)
 template <class R>
 __global__ __launch_bounds__(256) void k_su2_link_update(G2 g, int nf, R *Uout, const R *Uin, const R *__restrict__ P,
@@ -296,21 +326,24 @@ __global__ __launch_bounds__(256) void k_su2_delta_h(G2 g, const R *__restrict__
 	block_sum_store<5>(v, partials, nb, blockIdx.x);
 }
 
-// AoS host order [f][x][4] <-> planes
-template <class R> __global__ void k_su2_aos(const R *src, R *dst, G2 g, int nf, int to_soa)
+// host order [f][x][4] <-> device [f][stride][4] (directions padded to a
+// multiple of 256 sites)
+template <class R> __global__ void k_su2_aos(const R *src, R *dst, G2 g, int nf, int to_dev)
 {
 	long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= g.V * nf)
 		return;
-	int f = (int)(i / g.V);
+	int f = (int)((unsigned long)i / (unsigned long)g.V);
 	long x = i - (long)f * g.V;
-	for (int c = 0; c < 4; ++c)
+	if (to_dev)
+		stq(dst, g, f, x, qt<R>{src[4 * i], src[4 * i + 1], src[4 * i + 2], src[4 * i + 3]});
+	else
 	{
-		long a = 4 * i + c, s = (long)(4 * f + c) * g.stride + x;
-		if (to_soa)
-			dst[s] = src[a];
-		else
-			dst[a] = src[s];
+		qt<R> q = ldq(src, g, f, x);
+		dst[4 * i] = q.w;
+		dst[4 * i + 1] = q.x;
+		dst[4 * i + 2] = q.y;
+		dst[4 * i + 3] = q.z;
 	}
 }
 
@@ -515,10 +548,12 @@ int su2_setup(mesh_ctx *ctx, const mesh_grid *g, int prec, G2 *o)
 int launch_force(mesh_ctx *ctx, G2 const &g, int prec, const void *U, void *P, double beta, double delta)
 {
 	const double k = delta * (beta / 4.0);
+	int ph = mesh_prof_begin(ctx, PROF_FORCE_UPDATE);
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_force_update<R><<<nblk(g.V * g.nd), 256, 0, ctx->stream>>>(g, as<R>(U), as<R>(P), k);
 	});
+	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
@@ -526,21 +561,40 @@ int launch_force(mesh_ctx *ctx, G2 const &g, int prec, const void *U, void *P, d
 int launch_link(mesh_ctx *ctx, G2 const &g, int nf, int prec, void *Uout, const void *Uin, const void *P, double t,
                 int reunit)
 {
+	int ph = mesh_prof_begin(ctx, PROF_LINK_UPDATE);
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_link_update<R><<<nblk(g.V * nf), 256, 0, ctx->stream>>>(g, nf, as<R>(Uout), as<R>(Uin), as<R>(P), (R)t,
 		                                                             reunit);
 	});
+	mesh_prof_end(ctx, ph);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int launch_force_link(mesh_ctx *ctx, G2 const &g, int prec, const void *U, void *P, double beta, double delta,
+                      void *Unext, double t, int reunit)
+{
+	const double k = delta * (beta / 4.0);
+	int ph = mesh_prof_begin(ctx, PROF_FORCE_LINK);
+	by_prec(prec, [&](auto tag) {
+		using R = decltype(tag);
+		k_su2_force_link<R><<<nblk(g.V * g.nd), 256, 0, ctx->stream>>>(g, as<R>(U), as<R>(P), k, as<R>(Unext),
+		                                                               (R)t, reunit);
+	});
+	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
 
 int launch_momenta(mesh_ctx *ctx, G2 const &g, int prec, void *P, double *K, uint64_t seed, uint32_t traj)
 {
+	int ph = mesh_prof_begin(ctx, PROF_MOMENTA);
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_momenta<R><<<nblk(g.V), 256, 0, ctx->stream>>>(g, as<R>(P), K, seed, traj);
 	});
+	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
@@ -552,10 +606,12 @@ int su2_delta_h(mesh_ctx *ctx, G2 const &g, int prec, const void *Uo, const void
 	const int nb = nblk(g.V);
 	if (int rc = mesh_reserve_partials(ctx, 5 * (size_t)nb))
 		return rc;
+	int ph = mesh_prof_begin(ctx, PROF_DELTA_H);
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_delta_h<R><<<nb, 256, 0, ctx->stream>>>(g, as<R>(Uo), as<R>(Un), as<R>(P), Ko, beta, ctx->partials, nb);
 	});
+	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return mesh_finish_reduction(ctx, nb, 5, dh);
 }
@@ -766,15 +822,51 @@ int mesh_su2_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 	int rc;
 	if ((rc = launch_momenta(ctx, geo, prec, P, K, seed, traj)))
 		return rc;
-	for (int i = 0; i < nd_; ++i)
+	const int nfused = (nd_ - 1) / 2;
+	if (ctx->fuse_md && nfused > 0)
 	{
-		if (i % 2 == 0)
-			rc = launch_link(ctx, geo, geo.nd, prec, *U_work, i == 0 ? *U : *U_work, P, deltas[i], i == nd_ - 1);
-		else
-			rc = launch_force(ctx, geo, prec, *U_work, P, beta, deltas[i]);
-		if (rc)
+		// each momentum step fused with the link step after it, ping-pong
+		// between the caller's work field and the context's spare, the first
+		// link step writing the field that makes the last land in *U_work
+		const size_t fb = mesh_su2_field_bytes(g, geo.nd, prec);
+		if (fb > ctx->md_spare_bytes)
+		{
+			if (ctx->md_spare)
+			{
+				MESH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+				MESH_HIP(ctx, hipFree(ctx->md_spare));
+				ctx->md_spare = nullptr;
+				ctx->md_spare_bytes = 0;
+			}
+			if (hipMalloc(&ctx->md_spare, fb) != hipSuccess)
+				return mesh_fail(ctx, MESH_E_ALLOC, "cannot allocate the spare field of the fused trajectory");
+			ctx->md_spare_bytes = fb;
+		}
+		void *cur = (nfused % 2) ? ctx->md_spare : *U_work;
+		void *other = (nfused % 2) ? *U_work : ctx->md_spare;
+		if ((rc = launch_link(ctx, geo, geo.nd, prec, cur, *U, P, deltas[0], 0)))
 			return rc;
+		for (int i = 1; i + 1 < nd_; i += 2)
+		{
+			if ((rc = launch_force_link(ctx, geo, prec, cur, P, beta, deltas[i], other, deltas[i + 1],
+			                            i + 1 == nd_ - 1)))
+				return rc;
+			void *tmp = cur;
+			cur = other;
+			other = tmp;
+		}
 	}
+	else
+		for (int i = 0; i < nd_; ++i)
+		{
+			if (i % 2 == 0)
+				rc = launch_link(ctx, geo, geo.nd, prec, *U_work, i == 0 ? *U : *U_work, P, deltas[i],
+				                 i == nd_ - 1);
+			else
+				rc = launch_force(ctx, geo, prec, *U_work, P, beta, deltas[i]);
+			if (rc)
+				return rc;
+		}
 	double dh[5];
 	if ((rc = su2_delta_h(ctx, geo, prec, *U, *U_work, P, K, beta, dh)))
 		return rc;
@@ -888,10 +980,12 @@ int mesh_su2_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, int prec, vo
 	for (int i = 0; i < geo.nd; ++i)
 		if (geo.L[i] % 2)
 			return mesh_fail(ctx, MESH_E_SIZE, "checkerboard update needs even extents");
+	int ph = mesh_prof_begin(ctx, PROF_LANDAU_OR);
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_or_half<R><<<nblk(geo.V / 2), 256, 0, ctx->stream>>>(geo, as<R>(U), as<R>(h), parity, (R)omega);
 	});
+	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }

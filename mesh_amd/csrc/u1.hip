@@ -798,7 +798,9 @@ int mesh_u1_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, float *theta,
 	for (int i = 0; i < ng.nd; ++i)
 		if (ng.L[i] % 2)
 			return mesh_fail(ctx, MESH_E_SIZE, "checkerboard update needs even extents");
+	int ph = mesh_prof_begin(ctx, PROF_LANDAU_OR);
 	k_u1_or_half<<<(ng.V / 2 + 255) / 256, 256, 0, ctx->stream>>>(ng, theta, phi, parity, omega);
+	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }

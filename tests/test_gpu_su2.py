@@ -127,3 +127,23 @@ def test_su2_errors(ctx):
     h = S2.Hmc(g, F64, seed=1, ctx=ctx)
     with pytest.raises(mesh_amd.MeshError):
         h.runHmcUpdate(2.0, [0.1, 0.2])  # even number of steps
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("nsub", [1, 2, 5])
+def test_su2_fused_trajectory_same_bits(ctx, prec, nsub):
+    """each momentum step fused with the next link step (the default) gives the
+    unfused trajectory's bits, for both parities of the field ping-pong"""
+    from conftest import options
+
+    sh = (4, 4, 2, 6)
+    d = gauge.makeDeltas("2mn", 0.5, nsub)
+    res = []
+    for fuse in (0, 1):
+        h = S2.Hmc(mesh_amd.Grid(sh), prec, seed=31, ctx=ctx)
+        h.randomizeGaugeField()
+        with options(ctx, fuse_md=fuse):
+            rs = [h.runHmcUpdate(2.1, d, metropolis=t > 0) for t in range(3)]
+        res.append(([r["dH"] for r in rs], h.U.download()))
+    assert res[0][0] == res[1][0]
+    np.testing.assert_array_equal(res[0][1], res[1][1])
