@@ -434,6 +434,48 @@ def su2_bench(L, steps, warmup, therm, seed, beta=2.3, scheme="2mn", eps=0.5, su
     return out
 
 
+def u1_landau_bench(L, seed, therm, max_sweeps, beta=1.0, omega=1.8, tol=1e-12):
+    """Landau gauge fixing of a U(1) L^4 field (Landau<U1<T>>, FP32 angles)
+    thermalised by `therm` HMC trajectories: overrelaxation sweeps to
+    theta < tol, the half-sweep time from HIP events and its fraction of 8 TB/s
+    on 36 B per site (every link read and written once per half-sweep, the
+    phases of half the sites)."""
+    import mesh_amd
+    from mesh_amd import gauge
+    from mesh_amd import u1 as U1
+
+    ctx = mesh_amd.default_context()
+    dims = (L,) * 4
+    ens = U1.U1Ensemble(None, None, 1, ctx, dims=dims)
+    ens.random_start(seed)
+    d = gauge.makeDeltas("2lf", 0.5, 20)
+    for t in range(therm):
+        ens.hmc(beta, d, seed, t, metropolis=False)
+    g = mesh_amd.Grid(dims)
+    V = g.size()
+    U = U1.U1Field(g, None, ctx).upload(ens.angles()[0])
+    lan = U1.Landau(U)
+    F0 = lan()
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    sweeps, theta = lan.run(max_sweeps, omega=omega, tol=tol)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    ms, n = ctx.profile_read("landau_or")
+    ctx.profile(False)
+    half = ms / n if n else None
+    return {"metric": f"U(1) Landau gauge fixing site-sweeps/s at {L}^4", "unit": "site-sweeps/s",
+            "value": V * sweeps / dt, "higher_is_better": True,
+            "config": {"workload": f"U(1) {L}^4 beta={beta} after {therm} HMC trajectories; overrelaxation "
+                                   f"omega={omega} to theta < {tol:g}", "L": L},
+            "sweeps": sweeps, "theta": theta, "functional_start": F0, "functional_end": lan(),
+            "seconds": dt, "half_sweep_ms": half,
+            "frac_of_hbm": 36 * V / (half * 1e-3) / 1e9 / HBM_PEAK_GBS if half else None,
+            "dtype": "f32 angles, f64 sums", "data": "synthetic: Philox-seeded start, HMC-thermalised"}
+
+
 def landau_bench(L, seed, therm, max_sweeps, omega=1.7, tol=1e-12, ascent_iters=0):
     """Landau gauge fixing of a thermalised SU(3) L^4 configuration (HMC as in
     the main workload, `therm` trajectories) by even/odd overrelaxation, in
@@ -538,6 +580,7 @@ def main():
                     help="with --landau: also the reference's line-search solver (at most N iterations) "
                          "against overrelaxation, both to condition() < 1e-10")
     ap.add_argument("--su2", action="store_true", help="SU(2) HMC and Landau at --L (its own JSON line)")
+    ap.add_argument("--u1-landau", action="store_true", help="U(1) Landau gauge fixing at --L^4 (its own JSON line)")
     ap.add_argument("--u1", action="store_true",
                     help="instead: U(1) 2-d ESS/s of Q, flow-assisted vs plain HMC (one JSON line)")
     ap.add_argument("--u1-L", type=int, default=32)
@@ -568,6 +611,9 @@ def main():
     if a.landau:
         print(json.dumps(landau_bench(a.L, a.seed, a.therm, a.landau_sweeps, ascent_iters=a.landau_ascent)),
               flush=True)
+        return
+    if a.u1_landau:
+        print(json.dumps(u1_landau_bench(a.L, a.seed, a.therm, 20000)), flush=True)
         return
     if a.su2:
         print(json.dumps(su2_bench(a.L, a.steps, a.warmup, a.therm, a.seed)), flush=True)
