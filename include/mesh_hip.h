@@ -379,7 +379,10 @@ int mesh_su3_float_energy(mesh_ctx *ctx, const mesh_grid *g, const float *U, con
  * products; adj = conjugate, Re tr = 2 w, project_on_algebra keeps (0, x, y, z).
  * Device fields hold each direction's quaternions in site order, component c
  * of direction f at base[4 (f S + x) + c] with S = V rounded up to 256
- * (mesh_su2_field_bytes); host arrays are [nf][V][4] in that component order.  Momenta are (0, p/2)
+ * (mesh_su2_field_bytes); host arrays are [nf][V][4] in that component order.
+ * A quaternion is read with vector loads, so device field pointers must be
+ * aligned to 16 B (FP32) / 32 B (FP64) -- mesh_malloc's are; others are
+ * refused with MESH_E_ARG.  Momenta are (0, p/2)
  * with p ~ N(0,1) (random_algebra_element, groups.h:171-181) and the kinetic
  * energy is 2 (x^2 + y^2 + z^2), the 2x2 matrix norm, which the force
  * (beta/4) project_on_algebra(U Sigma) conserves (the reference adds the

@@ -34,6 +34,8 @@
 #include "rng_dev.h"
 
 #include <cmath>
+#include <cstdint>
+#include <initializer_list>
 
 using namespace mesh_dev;
 
@@ -625,6 +627,17 @@ double su2_metropolis_uniform(uint64_t seed, uint32_t traj)
 
 inline double su2_planes(G2 const &g) { return (double)g.V * g.nd * (g.nd - 1) / 2; }
 
+// fields are read with vector loads: every field pointer must be aligned to
+// one quaternion (16 B in FP32, 32 B in FP64; mesh_malloc gives 256 B)
+int su2_check_aligned(mesh_ctx *ctx, int prec, std::initializer_list<const void *> ps)
+{
+	const uintptr_t a = prec == MESH_F64 ? 32 : 16;
+	for (const void *p : ps)
+		if (p && (uintptr_t)p % a)
+			return mesh_fail(ctx, MESH_E_ARG, "SU(2) field pointers must be aligned to 16 B (FP32) / 32 B (FP64)");
+	return MESH_OK;
+}
+
 // one FP64 sum over sites of a kernel's per-site value
 template <class K> int su2_site_sum(mesh_ctx *ctx, G2 const &g, K launch, double *out)
 {
@@ -663,6 +676,8 @@ int mesh_su2_transfer(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void 
 		return rc;
 	if (nf < 1 || !host || !dev)
 		return mesh_fail(ctx, MESH_E_ARG, "bad SU(2) transfer arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {dev}))
+		return rc;
 	const size_t es = prec == MESH_F64 ? 8 : 4, bytes = (size_t)nf * geo.V * 4 * es;
 	void *tmp = nullptr;
 	if (hipMallocAsync(&tmp, bytes, ctx->stream) != hipSuccess)
@@ -699,6 +714,8 @@ int mesh_su2_random(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U
 		return rc;
 	if (!U || nf < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_random arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {U}))
+		return rc;
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_random<R><<<nblk(geo.V * nf), 256, 0, ctx->stream>>>(geo, nf, as<R>(U), seed);
@@ -715,6 +732,8 @@ int mesh_su2_unit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U)
 		return rc;
 	if (!U || nf < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_unit arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {U}))
+		return rc;
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_unit<R><<<nblk(geo.V * nf), 256, 0, ctx->stream>>>(geo, nf, as<R>(U));
@@ -731,6 +750,8 @@ int mesh_su2_momenta(mesh_ctx *ctx, const mesh_grid *g, int prec, void *P, doubl
 		return rc;
 	if (!P)
 		return mesh_fail(ctx, MESH_E_ARG, "P is NULL");
+	if (int rc = su2_check_aligned(ctx, prec, {P}))
+		return rc;
 	return launch_momenta(ctx, geo, prec, P, K, seed, traj);
 }
 
@@ -743,6 +764,8 @@ int mesh_su2_force_update(mesh_ctx *ctx, const mesh_grid *g, int prec, const voi
 		return rc;
 	if (!U || !P || U == P)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_force_update arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {U, P}))
+		return rc;
 	return launch_force(ctx, geo, prec, U, P, beta, delta);
 }
 
@@ -755,6 +778,8 @@ int mesh_su2_link_update(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, vo
 		return rc;
 	if (!Uout || !Uin || !P || nf < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_link_update arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {Uout, Uin, P}))
+		return rc;
 	return launch_link(ctx, geo, nf, prec, Uout, Uin, P, t, reunit);
 }
 
@@ -766,6 +791,8 @@ int mesh_su2_reunit(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *U
 		return rc;
 	if (!U || nf < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_reunit arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {U}))
+		return rc;
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_reunit<R><<<nblk(geo.V * nf), 256, 0, ctx->stream>>>(geo, nf, as<R>(U));
@@ -782,6 +809,8 @@ int mesh_su2_plaquette(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *
 		return rc;
 	if (!U || !out)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to su2_plaquette");
+	if (int rc = su2_check_aligned(ctx, prec, {U}))
+		return rc;
 	const int nb = nblk(geo.V);
 	if (int rc = mesh_reserve_partials(ctx, nb))
 		return rc;
@@ -806,6 +835,8 @@ int mesh_su2_delta_h(mesh_ctx *ctx, const mesh_grid *g, int prec, const void *U_
 		return rc;
 	if (!U_old || !U_new || !P || !K_old || !out)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to su2_delta_h");
+	if (int rc = su2_check_aligned(ctx, prec, {U_old, U_new, P}))
+		return rc;
 	return su2_delta_h(ctx, geo, prec, U_old, U_new, P, K_old, beta, out);
 }
 
@@ -819,6 +850,8 @@ int mesh_su2_hmc_trajectory(mesh_ctx *ctx, const mesh_grid *g, int prec, void **
 		return rc;
 	if (!U || !U_work || !*U || !*U_work || *U == *U_work || !P || !K || !deltas || !out || nd_ < 1 || nd_ % 2 != 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_hmc_trajectory arguments (deltas must have odd length)");
+	if (int rc = su2_check_aligned(ctx, prec, {*U, *U_work, P}))
+		return rc;
 	int rc;
 	if ((rc = launch_momenta(ctx, geo, prec, P, K, seed, traj)))
 		return rc;
@@ -899,6 +932,8 @@ int mesh_su2_landau_functional(mesh_ctx *ctx, const mesh_grid *g, int prec, cons
 		return rc;
 	if (!U || !out)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to su2_landau_functional");
+	if (int rc = su2_check_aligned(ctx, prec, {U, h}))
+		return rc;
 	double s;
 	int rc = su2_site_sum(
 	    ctx, geo,
@@ -923,6 +958,8 @@ int mesh_su2_landau_divsum(mesh_ctx *ctx, const mesh_grid *g, int prec, const vo
 		return rc;
 	if (!U || !out)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to su2_landau_divsum");
+	if (int rc = su2_check_aligned(ctx, prec, {U, h}))
+		return rc;
 	return su2_site_sum(
 	    ctx, geo,
 	    [&](int nb) {
@@ -942,6 +979,8 @@ int mesh_su2_landau_deriv(mesh_ctx *ctx, const mesh_grid *g, int prec, const voi
 		return rc;
 	if (!U || !F || F == U || F == h)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_landau_deriv arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {U, h, F}))
+		return rc;
 	const double c = -0.5 / (2.0 * geo.nd * (double)geo.V);
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
@@ -959,6 +998,8 @@ int mesh_su2_gauge_transform(mesh_ctx *ctx, const mesh_grid *g, int prec, void *
 		return rc;
 	if (!Uout || !U || !h || Uout == U || Uout == h)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_gauge_transform arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {Uout, U, h}))
+		return rc;
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_gauge_transform<R><<<nblk(geo.V * geo.nd), 256, 0, ctx->stream>>>(geo, as<R>(Uout), as<R>(U),
@@ -977,6 +1018,8 @@ int mesh_su2_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, int prec, vo
 		return rc;
 	if (!U || U == h || (parity != 0 && parity != 1))
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_landau_or_halfsweep arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {U, h}))
+		return rc;
 	for (int i = 0; i < geo.nd; ++i)
 		if (geo.L[i] % 2)
 			return mesh_fail(ctx, MESH_E_SIZE, "checkerboard update needs even extents");
@@ -998,6 +1041,8 @@ int mesh_su2_dot(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, const void
 		return rc;
 	if (!A || !B || !out || nf < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_dot arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {A, B}))
+		return rc;
 	return su2_site_sum(
 	    ctx, geo,
 	    [&](int nb) {
@@ -1017,6 +1062,8 @@ int mesh_su2_axpy(mesh_ctx *ctx, const mesh_grid *g, int nf, int prec, void *Y, 
 		return rc;
 	if (!Y || !X || nf < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "bad su2_axpy arguments");
+	if (int rc = su2_check_aligned(ctx, prec, {Y, X}))
+		return rc;
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
 		k_su2_axpy<R><<<nblk(geo.V * nf), 256, 0, ctx->stream>>>(geo, nf, as<R>(Y), (R)a, as<R>(X));

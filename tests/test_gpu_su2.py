@@ -147,3 +147,13 @@ def test_su2_fused_trajectory_same_bits(ctx, prec, nsub):
         res.append(([r["dH"] for r in rs], h.U.download()))
     assert res[0][0] == res[1][0]
     np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
+def test_su2_misaligned_field_refused(ctx):
+    g = mesh_amd.Grid([4, 4])
+    f = S2.SU2Field(g, None, F64, ctx)
+    import ctypes as C
+
+    bad = C.c_void_p(f.ptr.value + 8)
+    with pytest.raises(mesh_amd.MeshError):
+        ctx.call("mesh_su2_unit", g.ref, 2, F64, bad)
