@@ -167,3 +167,24 @@ def test_u1_landau_run_and_ascent(ctx, dims, beta):
     # both solvers find a maximum of the functional: a gauge copy with F close
     # to (at most a Gribov-copy spread away from) the overrelaxation result
     assert lan2() > F0 and abs(lan2() - lan()) < 0.05
+
+
+def test_u1_landau_full_size_half_sweep(ctx):
+    """32^4: a thermalised U(1) field, its gauge transform by random phases and
+    two overrelaxation half-sweeps against the numpy oracle (a few FP32 ulps)"""
+    dims = (32, 32, 32, 32)
+    th = _u1_thermalised(ctx, dims, 1.0, n=10)
+    rng = np.random.default_rng(61)
+    phi = rng.uniform(-math.pi, math.pi, size=dims).astype(np.float32)
+    fU = U1m.U1Field(mesh_amd.Grid(dims), None, ctx).upload(th)
+    lan = U1m.Landau(fU)
+    lan.g.upload(phi)
+    assert lan.divsum() == pytest.approx(U.landau_divsum(th, phi), rel=1e-10)
+    Ug = lan.gauge_fixed()
+    ref = U.gauge_transform(th, phi)
+    ph = phi.copy()
+    for parity in (0, 1):
+        lan._or_half(Ug, parity, 1.8)
+        ref, ph = U.landau_or_half(ref, ph, parity, 1.8)
+    assert _wd(Ug.download(), ref).max() < 4e-6
+    assert _wd(lan.g.download()[0], ph).max() < 4e-6

@@ -157,3 +157,23 @@ def test_su2_misaligned_field_refused(ctx):
     bad = C.c_void_p(f.ptr.value + 8)
     with pytest.raises(mesh_amd.MeshError):
         ctx.call("mesh_su2_unit", g.ref, 2, F64, bad)
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_su2_full_size_trajectory_bitexact(ctx, prec):
+    """32^4: one 2mn, 12-substep trajectory of the default fused path (momentum
+    steps fused with the link steps, ping-pong through the spare field) from a
+    field thermalised by 5 trajectories, against the oracle: the links bit for
+    bit, Delta H to 1e-12."""
+    sh = [32, 32, 32, 32]
+    O.set_threads(O.max_threads())
+    d = gauge.makeDeltas("2mn", 0.5, 12)
+    h = S2.Hmc(mesh_amd.Grid(sh), prec, seed=37, ctx=ctx)
+    h.randomizeGaugeField()
+    U = O.su2_random(sh, 37, _p(prec))
+    for t in range(6):
+        r = h.runHmcUpdate(2.3, d, metropolis=False)
+        o = O.su2_hmc_trajectory(U, sh, 2.3, d, 37, t, metropolis=False)
+        print(t, r["dH"], o["dH"], flush=True)
+        assert abs(r["dH"] - o["dH"]) <= 1e-12 * max(1.0, abs(o["dH"]))
+    np.testing.assert_array_equal(h.U.download(), U)
