@@ -673,3 +673,33 @@ def test_float_sums_error_grows_with_volume(ctx):
         print(L, err[L])
     assert err[16]["fsum"] > 10 * err[16]["mixed"]
     assert err[16]["fsum"] > 4 * err[8]["fsum"]
+
+
+def test_ensemble_f32_vs_f64_32(ctx):
+    """32^4 at beta 6 (2mn, 20 substeps: acceptance ~0.6), 400 Metropolis
+    trajectories in each precision from the same thermalised field:
+    acceptance, <exp(-dH)> (= 1) and the mean plaquette of the FP32-mixed chain
+    agree with the FP64 chain's within their binned errors."""
+    import math
+
+    g, U, _ = _cold_thermalised(ctx, [32] * 4, 60)
+    d = gauge.makeDeltas("2mn", 0.5, 20)
+    res = {}
+    for prec in (F64, F32):
+        h = gauge.Hmc(g, prec, seed=11, ctx=ctx)
+        h.U = U.convert(prec)
+        rows = [h.runHmcUpdate(6.0, d) for _ in range(400)]
+        res[prec] = dict(plaq=[r["plaq"] for r in rows], acc=[float(r["accepted"]) for r in rows],
+                         dH=np.array([r["dH"] for r in rows]))
+    a64, a32 = np.mean(res[F64]["acc"]), np.mean(res[F32]["acc"])
+    ea = math.sqrt(a64 * (1 - a64) / 400 + a32 * (1 - a32) / 400) + 1e-3
+    print("acceptance", a64, a32, "dH", res[F64]["dH"].mean(), res[F32]["dH"].mean(), flush=True)
+    assert abs(a64 - a32) < 4 * ea, (a64, a32)
+    for prec in (F64, F32):
+        c, ec = _binned(np.exp(-res[prec]["dH"]))
+        print("creutz", prec, c, ec, flush=True)
+        assert abs(c - 1) < 5 * ec + 0.02, (prec, c, ec)
+    p64, ep64 = _binned(res[F64]["plaq"])
+    p32, ep32 = _binned(res[F32]["plaq"])
+    print("plaquette", p64, ep64, p32, ep32, flush=True)
+    assert abs(p64 - p32) < 5 * math.hypot(ep64, ep32) + 1e-6, (p64, p32)
