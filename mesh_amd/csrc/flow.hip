@@ -23,46 +23,26 @@
 // Kernels: one workgroup (256 threads) per configuration, angles in LDS;
 // the dense layers of the MLP run on FP32 MFMA (v_mfma_f32_16x16x4_f32) in
 // 16-row tiles, one tile per wavefront.
+#include "flow_mlp.h"
 #include "mesh_internal.h"
 #include "rng_dev.h"
 
 using namespace mesh_dev;
+using namespace flow_mlp;
 
 namespace {
 
-constexpr int MAX_LAYERS = 64;                          // layers: 8 by default, up to 64
-constexpr int NK = 4;                                   // NCP components
-constexpr int NI = 12;                                  // features
-constexpr int NH = 64;                                  // hidden width
-constexpr int NO = 2 * NK + 2;                          // outputs
-constexpr int OW1 = 0, OB1 = OW1 + NI * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
-              OB3 = OW3 + NH * NO, NP_LAYER = OB3 + NO; // 5642 parameters per layer
+constexpr int MAX_LAYERS = 64; // layers: 8 by default, up to 64
+constexpr int NI = 12;         // features
+constexpr int NP_LAYER = Layout<NI>::NP; // 5642 parameters per layer
 constexpr int THREADS = 256;
-constexpr uint32_t TAG_FLOWZ = 0x464c575au;  // prior samples
-constexpr uint32_t TAG_FLOWI = 0x464c5749u;  // initial weights
-constexpr float PI_F = 3.14159265358979323846f;
-constexpr float TWO_PI_F = 6.28318530717958647692f;
-
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-// orders the LDS accesses of one wavefront (no cross-wavefront wait)
-__device__ __forceinline__ void wave_sync()
-{
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+constexpr uint32_t TAG_FLOWZ = 0x464c575au; // prior samples
+constexpr uint32_t TAG_FLOWI = 0x464c5749u; // initial weights
 
 struct FGeo
 {
 	int L0, L1, V, NA;
 };
-
-__device__ __forceinline__ float wrapf(float a)
-{
-	a = a - TWO_PI_F * floorf((a + PI_F) * (1.0f / TWO_PI_F));
-	return a >= PI_F ? a - TWO_PI_F : (a < -PI_F ? a + TWO_PI_F : a);
-}
 
 __device__ __forceinline__ int site(FGeo const &g, int x0, int x1)
 {
@@ -116,54 +96,11 @@ __device__ __forceinline__ int updated_link(FGeo const &g, int mu, int y0, int y
 	return mu == 0 ? site(g, y0, y1) : g.V + site(g, y0 + 1, y1);
 }
 
-// ---- MFMA tile product -----------------------------------------------------
-// D(16x16) = sum_k A(i,k) B(k,n), A(i,k) = A[i*sai + k*sak], B(k,n) = B[k*sbk + n*sbn],
-// k < K (multiple of 4).  Lane l supplies A(l%16, k0 + l/16) and
-// B(k0 + l/16, l%16); it receives D(4*(l/16) + j, l%16), j < 4.
-__device__ __forceinline__ f4 tile_mm(const float *A, int sai, int sak, const float *B, int sbk, int sbn, int K)
-{
-	int l = threadIdx.x & 63, i = l & 15, kk = l >> 4;
-	f4 acc = {0.f, 0.f, 0.f, 0.f};
-	for (int k0 = 0; k0 < K; k0 += 4)
-		acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[i * sai + (k0 + kk) * sak], B[(k0 + kk) * sbk + i * sbn], acc,
-		                                           0, 0, 0);
-	return acc;
-}
-
-// LDS image of one layer's weights (W3/b3 padded to 16 columns)
-struct LW
-{
-	float w1[NI * NH], b1[NH], w2[NH * NH], b2[NH], w3[NH * 16], b3[16];
-};
-
-__device__ void load_weights(LW &lw, const float *__restrict__ p)
-{
-	for (int i = threadIdx.x; i < NI * NH; i += THREADS)
-		lw.w1[i] = p[OW1 + i];
-	for (int i = threadIdx.x; i < NH * NH; i += THREADS)
-		lw.w2[i] = p[OW2 + i];
-	for (int i = threadIdx.x; i < NH * 16; i += THREADS)
-	{
-		int k = i / 16, n = i % 16;
-		lw.w3[i] = n < NO ? p[OW3 + k * NO + n] : 0.f;
-	}
-	for (int i = threadIdx.x; i < NH; i += THREADS)
-	{
-		lw.b1[i] = p[OB1 + i];
-		lw.b2[i] = p[OB2 + i];
-	}
-	for (int i = threadIdx.x; i < 16; i += THREADS)
-		lw.b3[i] = i < NO ? p[OB3 + i] : 0.f;
-}
-
-// per-wavefront tile scratch: features, two hidden layers, outputs
-struct Tile
-{
-	float x[16 * 16], h1[16 * NH], h2[16 * NH], y[16 * 16];
-};
+using LWF = LW<NI>;
+using TileF = Tile<NI>;
 
 // features of the 16 active plaquettes r0..r0+15 (rows beyond NA are zero)
-__device__ void features(FGeo const &g, const float *t, int mu, int a, int r0, Tile &T)
+__device__ void features(FGeo const &g, const float *t, int mu, int a, int r0, TileF &T)
 {
 	int l = threadIdx.x & 63;
 	for (int e = l; e < 16 * 16; e += 64)
@@ -183,77 +120,12 @@ __device__ void features(FGeo const &g, const float *t, int mu, int a, int r0, T
 	}
 }
 
-// MLP forward of one tile: h1 = tanh(x W1 + b1), h2 = tanh(h1 W2 + b2), y = h2 W3 + b3
-__device__ void mlp_forward(LW const &lw, Tile &T)
-{
-	int l = threadIdx.x & 63, c = l & 15, rq = (l >> 4) * 4;
-	for (int n0 = 0; n0 < NH; n0 += 16)
-	{
-		f4 d = tile_mm(T.x, 16, 1, lw.w1 + n0, NH, 1, NI);
-		for (int j = 0; j < 4; ++j)
-			T.h1[(rq + j) * NH + n0 + c] = tanhf(d[j] + lw.b1[n0 + c]);
-	}
-	wave_sync();
-	for (int n0 = 0; n0 < NH; n0 += 16)
-	{
-		f4 d = tile_mm(T.h1, NH, 1, lw.w2 + n0, NH, 1, NH);
-		for (int j = 0; j < 4; ++j)
-			T.h2[(rq + j) * NH + n0 + c] = tanhf(d[j] + lw.b2[n0 + c]);
-	}
-	wave_sync();
-	f4 d = tile_mm(T.h2, NH, 1, lw.w3, 16, 1, NH);
-	for (int j = 0; j < 4; ++j)
-		T.y[(rq + j) * 16 + c] = d[j] + lw.b3[c];
-	wave_sync();
-}
-
-// transform parameters from the MLP outputs
-struct NCP
-{
-	float s[NK], w[NK], t, ta, tb;
-};
-
-__device__ __forceinline__ NCP ncp_params(const float *y)
-{
-	NCP p;
-	float m = y[NK];
-	for (int k = 1; k < NK; ++k)
-		m = fmaxf(m, y[NK + k]);
-	float z = 0.f;
-	for (int k = 0; k < NK; ++k)
-	{
-		p.s[k] = expf(fminf(fmaxf(y[k], -8.f), 8.f));
-		p.w[k] = expf(y[NK + k] - m);
-		z += p.w[k];
-	}
-	for (int k = 0; k < NK; ++k)
-		p.w[k] /= z;
-	p.ta = y[2 * NK];
-	p.tb = y[2 * NK + 1];
-	p.t = atan2f(p.tb, p.ta);
-	return p;
-}
-
-// g(u), g'(u) of the mixture
-__device__ __forceinline__ void ncp_eval(NCP const &p, float u, float &G, float &Gp)
-{
-	float c = cosf(0.5f * u), sn = sinf(0.5f * u);
-	G = 0.f;
-	Gp = 0.f;
-	for (int k = 0; k < NK; ++k)
-	{
-		float d = fmaf(c, c, p.s[k] * p.s[k] * sn * sn);
-		G = fmaf(p.w[k], 2.f * atan2f(p.s[k] * sn, c), G);
-		Gp = fmaf(p.w[k], p.s[k] / d, Gp);
-	}
-}
-
 // ---- kernels ------------------------------------------------------------------
 
 struct FlowShared
 {
-	LW lw;
-	Tile tile[4];
+	LWF lw;
+	TileF tile[4];
 	double red[4][4];
 };
 
@@ -262,7 +134,7 @@ __device__ void mlp_round(FGeo const &g, const float *th, int mu, int a, int it,
                           float (*Ys)[256])
 {
 	const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-	Tile &T = sh.tile[w];
+	TileF &T = sh.tile[w];
 	for (int q = 0; q < 4; ++q)
 	{
 		const int tl = q * 4 + w, tix = it + tl;
@@ -270,7 +142,7 @@ __device__ void mlp_round(FGeo const &g, const float *th, int mu, int a, int it,
 			break;
 		features(g, th, mu, a, tix * 16, T);
 		wave_sync();
-		mlp_forward(sh.lw, T);
+		mlp_forward<NI>(sh.lw, T);
 		for (int e = l; e < 256; e += 64)
 			Ys[tl][e] = T.y[e];
 		wave_sync();
@@ -304,7 +176,7 @@ __global__ __launch_bounds__(THREADS) void k_flow_forward(FGeo g, int nl, const 
 		if (theta_layers)
 			for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
 				theta_layers[((long)c * nl + layer) * 2 * g.V + i] = th[i];
-		load_weights(sh.lw, params + layer * NP_LAYER);
+		load_weights<NI>(sh.lw, params + layer * NP_LAYER);
 		__syncthreads();
 		// within a layer the MLP inputs (frozen plaquettes) never change, so
 		// all MLPs of a round run first and then every thread moves one link
@@ -372,7 +244,7 @@ __global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, int nl, const 
 	{
 		const int mu = layer & 1, a = (layer >> 1) & 3;
 		__syncthreads();
-		load_weights(sh.lw, params + layer * NP_LAYER);
+		load_weights<NI>(sh.lw, params + layer * NP_LAYER);
 		__syncthreads();
 		for (int it = 0; it < NT; it += 16)
 		{
@@ -385,38 +257,8 @@ __global__ __launch_bounds__(THREADS) void k_flow_inverse(FGeo g, int nl, const 
 				active_corner(g, mu, a, r, y0, y1);
 				NCP p = ncp_params(&Ys[threadIdx.x >> 4][(threadIdx.x & 15) * 16]);
 				float tpn = plaqf(g, th, y0, y1);
-				float target = wrapf(tpn - p.t);
-				// g is increasing on [-pi, pi] with g(+-pi) = +-pi: Newton steps
-				// inside a bracket that every evaluation narrows, bisection when a
-				// step leaves it, until a step or the bracket is below
-				// 2e-7 max(1, |u|) (an FP32 ulp of u is up to 2.4e-7 here, so an
-				// absolute bound could not stop an iterate that alternates
-				// between neighbouring floats; at most 40 evaluations)
-				float lo = -PI_F, hi = PI_F, u = target, G, Gp;
-				for (int b = 0; b < 40; ++b)
-				{
-					ncp_eval(p, u, G, Gp);
-					float f = G - target;
-					if (f == 0.f)
-						break;
-					if (f < 0.f)
-						lo = u;
-					else
-						hi = u;
-					if (hi - lo <= 2e-7f * fmaxf(1.f, fabsf(u)))
-						break;
-					float un = u - f / Gp;
-					if (un > lo && un < hi)
-					{
-						bool done = fabsf(un - u) <= 2e-7f * fmaxf(1.f, fabsf(un));
-						u = un;
-						if (done)
-							break;
-					}
-					else
-						u = 0.5f * (lo + hi);
-				}
-				ncp_eval(p, u, G, Gp);
+				float Gp;
+				float u = ncp_inverse(p, wrapf(tpn - p.t), Gp);
 				int ul = updated_link(g, mu, y0, y1);
 				th[ul] = wrapf(th[ul] - tpn + u + p.t);
 				v[0] += (double)logf(Gp);
@@ -462,7 +304,6 @@ __global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, int nl, const
 	float *gact = dyn + 5 * g.V;    // NA: dL/dtheta_P of the active plaquettes
 	__shared__ FlowShared sh;
 	const int c = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
-	const int lc = l & 15, rq = (l >> 4) * 4;
 	float *dx = dxs + (long)c * g.NA * NI;
 	float *part = partial + (long)c * nl * NP_LAYER;
 	// adjoint of the final links: dS/dtheta
@@ -485,25 +326,22 @@ __global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, int nl, const
 		__syncthreads();
 		for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
 			th[i] = theta_layers[((long)c * nl + layer) * 2 * g.V + i];
-		load_weights(sh.lw, params + layer * NP_LAYER);
+		load_weights<NI>(sh.lw, params + layer * NP_LAYER);
 		__syncthreads();
-		f4 aw1[4], aw2[16], aw3[4];
-		for (int q = 0; q < 4; ++q)
-			aw1[q] = aw3[q] = f4{0.f, 0.f, 0.f, 0.f};
-		for (int q = 0; q < 16; ++q)
-			aw2[q] = f4{0.f, 0.f, 0.f, 0.f};
-		float ab1 = 0.f, ab2 = 0.f, ab3 = 0.f; // bias gradients, lane = column
+		Grad<NI> gr;
+		gr.zero();
 		for (int it = 0; it < (NT + 3) / 4; ++it)
 		{
 			const int tix = it * 4 + w;
-			Tile &T = sh.tile[w];
-			const bool on = tix < NT;
-			if (on)
+			TileF &T = sh.tile[w];
+			if (tix < NT)
 			{
 				features(g, th, mu, a, tix * 16, T);
 				wave_sync();
-				mlp_forward(sh.lw, T);
-				// output adjoints of the 16 rows (lanes 0..15 own one row each)
+				mlp_forward<NI>(sh.lw, T);
+				// output adjoints of the 16 rows (lanes 0..15 own one row each):
+				// the updated link moves by g(u) + t - theta_P, so its adjoint A
+				// reaches theta_P as A (g' - 1) plus the log-det term
 				if (l < 16)
 				{
 					float dy[16];
@@ -515,138 +353,20 @@ __global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, int nl, const
 						int y0, y1;
 						active_corner(g, mu, a, r, y0, y1);
 						const float *y = T.y + l * 16;
-						NCP p = ncp_params(y);
 						float tp = plaqf(g, th, y0, y1);
-						float u = wrapf(tp - p.t);
-						float cc = cosf(0.5f * u), sn = sinf(0.5f * u);
-						float G = 0.f, Gp = 0.f, Gpp = 0.f, gk[NK], gpk[NK], dgs[NK], dgps[NK];
-						for (int k = 0; k < NK; ++k)
-						{
-							float s2 = p.s[k] * p.s[k];
-							float d = fmaf(cc, cc, s2 * sn * sn);
-							gk[k] = 2.f * atan2f(p.s[k] * sn, cc);
-							gpk[k] = p.s[k] / d;
-							dgs[k] = 2.f * sn * cc / d;
-							dgps[k] = (cc * cc - s2 * sn * sn) / (d * d);
-							G = fmaf(p.w[k], gk[k], G);
-							Gp = fmaf(p.w[k], gpk[k], Gp);
-							Gpp = fmaf(p.w[k], -p.s[k] * (s2 - 1.f) * sn * cc / (d * d), Gpp);
-						}
 						float A = ab[updated_link(g, mu, y0, y1)];
-						float iG = 1.f / Gp;
-						gact[r] = A * (Gp - 1.f) - Gpp * iG;
-						float dLdt = A * (1.f - Gp) + Gpp * iG;
-						float dLdw[NK], sw = 0.f;
-						for (int k = 0; k < NK; ++k)
-						{
-							dLdw[k] = A * gk[k] - gpk[k] * iG;
-							sw = fmaf(p.w[k], dLdw[k], sw);
-							float dLds = p.w[k] * (A * dgs[k] - dgps[k] * iG);
-							dy[k] = (y[k] > -8.f && y[k] < 8.f) ? p.s[k] * dLds : 0.f;
-						}
-						for (int k = 0; k < NK; ++k)
-							dy[NK + k] = p.w[k] * (dLdw[k] - sw);
-						float r2 = fmaf(p.ta, p.ta, p.tb * p.tb) + 1e-20f;
-						dy[2 * NK] = dLdt * (-p.tb / r2);
-						dy[2 * NK + 1] = dLdt * (p.ta / r2);
+						gact[r] = ncp_adjoint(y, wrapf(tp - ncp_params(y).t), A, dy) - A;
 					}
 					for (int k = 0; k < 16; ++k)
 						T.y[l * 16 + k] = dy[k];
 				}
 				wave_sync();
-				// dW3 += H2^T dY, db3
-				for (int i0 = 0; i0 < NH; i0 += 16)
-				{
-					f4 d = tile_mm(T.h2 + i0, 1, NH, T.y, 16, 1, 16);
-					aw3[i0 / 16] += d;
-				}
-				if (l < 16)
-					for (int rr = 0; rr < 16; ++rr)
-						ab3 += T.y[rr * 16 + l];
-				// dA2 = (dY W3^T) * (1 - h2^2), into h2
-				f4 dh2[4];
-				for (int n0 = 0; n0 < NH; n0 += 16)
-					dh2[n0 / 16] = tile_mm(T.y, 16, 1, sh.lw.w3 + n0 * 16, 1, 16, 16);
-				wave_sync();
-				for (int n0 = 0; n0 < NH; n0 += 16)
-					for (int j = 0; j < 4; ++j)
-					{
-						float hv = T.h2[(rq + j) * NH + n0 + lc];
-						T.h2[(rq + j) * NH + n0 + lc] = dh2[n0 / 16][j] * (1.f - hv * hv);
-					}
-				wave_sync();
-				// dW2 += H1^T dA2, db2
-				for (int i0 = 0; i0 < NH; i0 += 16)
-					for (int n0 = 0; n0 < NH; n0 += 16)
-						aw2[(i0 / 16) * 4 + n0 / 16] += tile_mm(T.h1 + i0, 1, NH, T.h2 + n0, NH, 1, 16);
-				for (int rr = 0; rr < 16; ++rr)
-					ab2 += T.h2[rr * NH + l];
-				// dA1 = (dA2 W2^T) * (1 - h1^2), into h1
-				f4 dh1[4];
-				for (int n0 = 0; n0 < NH; n0 += 16)
-					dh1[n0 / 16] = tile_mm(T.h2, NH, 1, sh.lw.w2 + n0 * NH, 1, NH, NH);
-				wave_sync();
-				for (int n0 = 0; n0 < NH; n0 += 16)
-					for (int j = 0; j < 4; ++j)
-					{
-						float hv = T.h1[(rq + j) * NH + n0 + lc];
-						T.h1[(rq + j) * NH + n0 + lc] = dh1[n0 / 16][j] * (1.f - hv * hv);
-					}
-				wave_sync();
-				// dW1 += X^T dA1, db1
-				for (int n0 = 0; n0 < NH; n0 += 16)
-					aw1[n0 / 16] += tile_mm(T.x, 1, 16, T.h1 + n0, NH, 1, 16);
-				for (int rr = 0; rr < 16; ++rr)
-					ab1 += T.h1[rr * NH + l];
-				// dX = dA1 W1^T (feature columns 0..11)
-				f4 dxt = tile_mm(T.h1, NH, 1, sh.lw.w1, 1, NH, NH);
-				for (int j = 0; j < 4; ++j)
-				{
-					int r = tix * 16 + rq + j;
-					if (lc < NI && r < g.NA)
-						dx[(long)r * NI + lc] = dxt[j];
-				}
-				wave_sync();
+				mlp_backward<NI>(sh.lw, T, gr, dx, tix * 16, g.NA);
 			}
 			__syncthreads();
 		}
 		// per-configuration weight gradients of this layer, waves in order
-		float *pl = part + layer * NP_LAYER;
-		for (int ww = 0; ww < 4; ++ww)
-		{
-			if (w == ww)
-			{
-				for (int n0 = 0; n0 < NH; n0 += 16)
-					for (int j = 0; j < 4; ++j)
-					{
-						int k = rq + j, n = n0 + lc;
-						if (k < NI)
-						{
-							float *dst = pl + OW1 + k * NH + n;
-							*dst = (ww ? *dst : 0.f) + aw1[n0 / 16][j];
-						}
-					}
-				for (int i0 = 0; i0 < NH; i0 += 16)
-					for (int n0 = 0; n0 < NH; n0 += 16)
-						for (int j = 0; j < 4; ++j)
-						{
-							float *dst = pl + OW2 + (i0 + rq + j) * NH + n0 + lc;
-							*dst = (ww ? *dst : 0.f) + aw2[(i0 / 16) * 4 + n0 / 16][j];
-						}
-				for (int i0 = 0; i0 < NH; i0 += 16)
-					for (int j = 0; j < 4; ++j)
-						if (lc < NO)
-						{
-							float *dst = pl + OW3 + (i0 + rq + j) * NO + lc;
-							*dst = (ww ? *dst : 0.f) + aw3[i0 / 16][j];
-						}
-				pl[OB1 + l] = (ww ? pl[OB1 + l] : 0.f) + ab1;
-				pl[OB2 + l] = (ww ? pl[OB2 + l] : 0.f) + ab2;
-				if (l < NO)
-					pl[OB3 + l] = (ww ? pl[OB3 + l] : 0.f) + ab3;
-			}
-			__syncthreads();
-		}
+		store_grad<NI>(gr, part + layer * NP_LAYER);
 		// plaquette gradients: active from gact, frozen from the feature adjoints
 		for (int x = threadIdx.x; x < g.V; x += THREADS)
 		{
@@ -726,34 +446,12 @@ __global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ g
 	p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
-// initial weights: W ~ N(0, 1/fan_in) for the hidden layers, the last layer
-// 1e-2 of that (identical components would receive identical gradients for
-// ever) with the bias of the identity transform (s_k = 1, w uniform, t = 0)
+// initial weights (flow_mlp::init_param: near the identity transform)
 __global__ void k_flow_init(int n, float *p, uint64_t seed)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	int o = i % NP_LAYER;
-	float v = 0.f;
-	double sd = 0.0;
-	if (o >= OW1 && o < OB1)
-		sd = 1.0 / sqrt((double)NI);
-	else if (o >= OW2 && o < OB2)
-		sd = 1.0 / sqrt((double)NH);
-	else if (o >= OW3 && o < OB3)
-		sd = 1e-2 / sqrt((double)NH); // small: breaks the symmetry of the NK components
-	if (sd > 0)
-	{
-		u4 ctr = {(uint32_t)i, 0u, 0u, TAG_FLOWI};
-		u4 r = philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
-		double z0, z1;
-		normal_pair(r, &z0, &z1);
-		v = (float)(z0 * sd);
-	}
-	if (o == OB3 + 2 * NK)
-		v = 1.f;
-	p[i] = v;
+	if (i < n)
+		init_param<NI>(i, p, seed, TAG_FLOWI);
 }
 
 // Metropolis-Hastings with an independence proposal, per chain c:

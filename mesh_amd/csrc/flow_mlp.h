@@ -1,0 +1,394 @@
+// Dense layers and circle maps shared by the U(1) learned flows (flow.hip:
+// coupling layers on links, pcflow.hip: coupling layers on plaquettes).
+//
+// Per layer an MLP NI -> 64 -> 64 -> 10 (tanh) turns NI context features of
+// an active variable into the parameters of a circle diffeomorphism
+//   h(a) = g(wrap(a - t)) + t,  g(u) = sum_k w_k 2 atan(s_k tan(u / 2))
+// (a mixture of NK = 4 non-compact projections; s_k = exp(clamp(y_k, -8, 8)),
+// w = softmax(y_{4..7}), t = atan2(y_9, y_8)).  The dense layers run on FP32
+// MFMA (v_mfma_f32_16x16x4_f32) in 16-row tiles, one tile per wavefront; the
+// backward pass forms the weight gradients as MFMA tile products.
+//
+// Parameters of one layer, in order: W1 [NI][64], b1 [64], W2 [64][64],
+// b2 [64], W3 [64][10], b3 [10].
+#pragma once
+
+#include "mesh_internal.h"
+#include "rng_dev.h"
+
+namespace flow_mlp {
+
+constexpr int NK = 4;          // mixture components
+constexpr int NH = 64;         // hidden width
+constexpr int NO = 2 * NK + 2; // outputs
+constexpr float PI_F = 3.14159265358979323846f;
+constexpr float TWO_PI_F = 6.28318530717958647692f;
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <int NI> struct Layout
+{
+	static_assert(NI % 4 == 0 && NI <= 32, "features: a multiple of 4, at most 32");
+	static constexpr int XS = NI <= 16 ? 16 : 32; // feature row stride of a tile
+	static constexpr int OW1 = 0, OB1 = OW1 + NI * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
+	                     OB3 = OW3 + NH * NO, NP = OB3 + NO;
+};
+
+// orders the LDS accesses of one wavefront (no cross-wavefront wait)
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float wrapf(float a)
+{
+	a = a - TWO_PI_F * floorf((a + PI_F) * (1.0f / TWO_PI_F));
+	return a >= PI_F ? a - TWO_PI_F : (a < -PI_F ? a + TWO_PI_F : a);
+}
+
+// D(16x16) = sum_k A(i,k) B(k,n), A(i,k) = A[i*sai + k*sak], B(k,n) = B[k*sbk + n*sbn],
+// k < K (multiple of 4).  Lane l supplies A(l%16, k0 + l/16) and
+// B(k0 + l/16, l%16); it receives D(4*(l/16) + j, l%16), j < 4.
+__device__ __forceinline__ f4 tile_mm(const float *A, int sai, int sak, const float *B, int sbk, int sbn, int K)
+{
+	int l = threadIdx.x & 63, i = l & 15, kk = l >> 4;
+	f4 acc = {0.f, 0.f, 0.f, 0.f};
+	for (int k0 = 0; k0 < K; k0 += 4)
+		acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[i * sai + (k0 + kk) * sak], B[(k0 + kk) * sbk + i * sbn], acc,
+		                                           0, 0, 0);
+	return acc;
+}
+
+// LDS image of one layer's weights (W3/b3 padded to 16 columns)
+template <int NI> struct LW
+{
+	float w1[NI * NH], b1[NH], w2[NH * NH], b2[NH], w3[NH * 16], b3[16];
+};
+
+template <int NI> __device__ void load_weights(LW<NI> &lw, const float *__restrict__ p)
+{
+	using Y = Layout<NI>;
+	for (int i = threadIdx.x; i < NI * NH; i += blockDim.x)
+		lw.w1[i] = p[Y::OW1 + i];
+	for (int i = threadIdx.x; i < NH * NH; i += blockDim.x)
+		lw.w2[i] = p[Y::OW2 + i];
+	for (int i = threadIdx.x; i < NH * 16; i += blockDim.x)
+	{
+		int k = i / 16, n = i % 16;
+		lw.w3[i] = n < NO ? p[Y::OW3 + k * NO + n] : 0.f;
+	}
+	for (int i = threadIdx.x; i < NH; i += blockDim.x)
+	{
+		lw.b1[i] = p[Y::OB1 + i];
+		lw.b2[i] = p[Y::OB2 + i];
+	}
+	for (int i = threadIdx.x; i < 16; i += blockDim.x)
+		lw.b3[i] = i < NO ? p[Y::OB3 + i] : 0.f;
+}
+
+// per-wavefront tile scratch: features (row stride XS), two hidden layers, outputs
+template <int NI> struct Tile
+{
+	float x[16 * Layout<NI>::XS], h1[16 * NH], h2[16 * NH], y[16 * 16];
+};
+
+// MLP forward of one tile: h1 = tanh(x W1 + b1), h2 = tanh(h1 W2 + b2), y = h2 W3 + b3
+template <int NI> __device__ void mlp_forward(LW<NI> const &lw, Tile<NI> &T)
+{
+	constexpr int XS = Layout<NI>::XS;
+	int l = threadIdx.x & 63, c = l & 15, rq = (l >> 4) * 4;
+	for (int n0 = 0; n0 < NH; n0 += 16)
+	{
+		f4 d = tile_mm(T.x, XS, 1, lw.w1 + n0, NH, 1, NI);
+		for (int j = 0; j < 4; ++j)
+			T.h1[(rq + j) * NH + n0 + c] = tanhf(d[j] + lw.b1[n0 + c]);
+	}
+	wave_sync();
+	for (int n0 = 0; n0 < NH; n0 += 16)
+	{
+		f4 d = tile_mm(T.h1, NH, 1, lw.w2 + n0, NH, 1, NH);
+		for (int j = 0; j < 4; ++j)
+			T.h2[(rq + j) * NH + n0 + c] = tanhf(d[j] + lw.b2[n0 + c]);
+	}
+	wave_sync();
+	f4 d = tile_mm(T.h2, NH, 1, lw.w3, 16, 1, NH);
+	for (int j = 0; j < 4; ++j)
+		T.y[(rq + j) * 16 + c] = d[j] + lw.b3[c];
+	wave_sync();
+}
+
+// transform parameters from the MLP outputs
+struct NCP
+{
+	float s[NK], w[NK], t, ta, tb;
+};
+
+__device__ __forceinline__ NCP ncp_params(const float *y)
+{
+	NCP p;
+	float m = y[NK];
+	for (int k = 1; k < NK; ++k)
+		m = fmaxf(m, y[NK + k]);
+	float z = 0.f;
+	for (int k = 0; k < NK; ++k)
+	{
+		p.s[k] = expf(fminf(fmaxf(y[k], -8.f), 8.f));
+		p.w[k] = expf(y[NK + k] - m);
+		z += p.w[k];
+	}
+	for (int k = 0; k < NK; ++k)
+		p.w[k] /= z;
+	p.ta = y[2 * NK];
+	p.tb = y[2 * NK + 1];
+	p.t = atan2f(p.tb, p.ta);
+	return p;
+}
+
+// g(u), g'(u) of the mixture
+__device__ __forceinline__ void ncp_eval(NCP const &p, float u, float &G, float &Gp)
+{
+	float c = cosf(0.5f * u), sn = sinf(0.5f * u);
+	G = 0.f;
+	Gp = 0.f;
+	for (int k = 0; k < NK; ++k)
+	{
+		float d = fmaf(c, c, p.s[k] * p.s[k] * sn * sn);
+		G = fmaf(p.w[k], 2.f * atan2f(p.s[k] * sn, c), G);
+		Gp = fmaf(p.w[k], p.s[k] / d, Gp);
+	}
+}
+
+// u = g^-1(target): g is increasing on [-pi, pi] with g(+-pi) = +-pi, so
+// Newton steps inside a bracket that every evaluation narrows, bisection when
+// a step leaves it, until a step or the bracket is below 2e-7 max(1, |u|)
+// (an FP32 ulp of u is up to 2.4e-7 here, so an absolute bound could not stop
+// an iterate that alternates between neighbouring floats; at most 40
+// evaluations).  Returns u, and g'(u) in Gp.
+__device__ __forceinline__ float ncp_inverse(NCP const &p, float target, float &Gp)
+{
+	float lo = -PI_F, hi = PI_F, u = target, G;
+	for (int b = 0; b < 40; ++b)
+	{
+		ncp_eval(p, u, G, Gp);
+		float f = G - target;
+		if (f == 0.f)
+			break;
+		if (f < 0.f)
+			lo = u;
+		else
+			hi = u;
+		if (hi - lo <= 2e-7f * fmaxf(1.f, fabsf(u)))
+			break;
+		float un = u - f / Gp;
+		if (un > lo && un < hi)
+		{
+			bool done = fabsf(un - u) <= 2e-7f * fmaxf(1.f, fabsf(un));
+			u = un;
+			if (done)
+				break;
+		}
+		else
+			u = 0.5f * (lo + hi);
+	}
+	ncp_eval(p, u, G, Gp);
+	return u;
+}
+
+// Adjoints of L = A (g(u) + t) - log g'(u), u = wrap(a - t), for one active
+// variable a with MLP outputs y: dy[16] = dL/dy (through s, w and t), and
+// the return value dL/da.
+__device__ __forceinline__ float ncp_adjoint(const float *y, float u, float A, float (&dy)[16])
+{
+	NCP p = ncp_params(y);
+	float cc = cosf(0.5f * u), sn = sinf(0.5f * u);
+	float Gp = 0.f, Gpp = 0.f, gk[NK], gpk[NK], dgs[NK], dgps[NK];
+	for (int k = 0; k < NK; ++k)
+	{
+		float s2 = p.s[k] * p.s[k];
+		float d = fmaf(cc, cc, s2 * sn * sn);
+		gk[k] = 2.f * atan2f(p.s[k] * sn, cc);
+		gpk[k] = p.s[k] / d;
+		dgs[k] = 2.f * sn * cc / d;
+		dgps[k] = (cc * cc - s2 * sn * sn) / (d * d);
+		Gp = fmaf(p.w[k], gpk[k], Gp);
+		Gpp = fmaf(p.w[k], -p.s[k] * (s2 - 1.f) * sn * cc / (d * d), Gpp);
+	}
+	float iG = 1.f / Gp;
+	float dLdt = A * (1.f - Gp) + Gpp * iG;
+	float dLdw[NK], sw = 0.f;
+	for (int k = 0; k < 16; ++k)
+		dy[k] = 0.f;
+	for (int k = 0; k < NK; ++k)
+	{
+		dLdw[k] = A * gk[k] - gpk[k] * iG;
+		sw = fmaf(p.w[k], dLdw[k], sw);
+		float dLds = p.w[k] * (A * dgs[k] - dgps[k] * iG);
+		dy[k] = (y[k] > -8.f && y[k] < 8.f) ? p.s[k] * dLds : 0.f;
+	}
+	for (int k = 0; k < NK; ++k)
+		dy[NK + k] = p.w[k] * (dLdw[k] - sw);
+	float r2 = fmaf(p.ta, p.ta, p.tb * p.tb) + 1e-20f;
+	dy[2 * NK] = dLdt * (-p.tb / r2);
+	dy[2 * NK + 1] = dLdt * (p.ta / r2);
+	return A * Gp - Gpp * iG;
+}
+
+// Weight-gradient accumulators of one wavefront (lane-owned tile fragments)
+template <int NI> struct Grad
+{
+	static constexpr int NX = Layout<NI>::XS / 16;
+	f4 w1[NX * 4], w2[16], w3[4];
+	float b1, b2, b3;
+	__device__ void zero()
+	{
+		for (int q = 0; q < NX * 4; ++q)
+			w1[q] = f4{0.f, 0.f, 0.f, 0.f};
+		for (int q = 0; q < 16; ++q)
+			w2[q] = f4{0.f, 0.f, 0.f, 0.f};
+		for (int q = 0; q < 4; ++q)
+			w3[q] = f4{0.f, 0.f, 0.f, 0.f};
+		b1 = b2 = b3 = 0.f;
+	}
+};
+
+// MLP backward of one tile whose T.y holds dL/dy (and T.x, T.h1, T.h2 the
+// forward values): accumulates the weight gradients into G and writes
+// dL/dx of rows r0 .. r0+15 (r < nrows) to dx[r * NI + feature]
+template <int NI>
+__device__ void mlp_backward(LW<NI> const &lw, Tile<NI> &T, Grad<NI> &G, float *__restrict__ dx, int r0, int nrows)
+{
+	constexpr int XS = Layout<NI>::XS;
+	const int l = threadIdx.x & 63, lc = l & 15, rq = (l >> 4) * 4;
+	// dW3 += H2^T dY, db3
+	for (int i0 = 0; i0 < NH; i0 += 16)
+		G.w3[i0 / 16] += tile_mm(T.h2 + i0, 1, NH, T.y, 16, 1, 16);
+	if (l < 16)
+		for (int rr = 0; rr < 16; ++rr)
+			G.b3 += T.y[rr * 16 + l];
+	// dA2 = (dY W3^T) * (1 - h2^2), into h2
+	f4 dh2[4];
+	for (int n0 = 0; n0 < NH; n0 += 16)
+		dh2[n0 / 16] = tile_mm(T.y, 16, 1, lw.w3 + n0 * 16, 1, 16, 16);
+	wave_sync();
+	for (int n0 = 0; n0 < NH; n0 += 16)
+		for (int j = 0; j < 4; ++j)
+		{
+			float hv = T.h2[(rq + j) * NH + n0 + lc];
+			T.h2[(rq + j) * NH + n0 + lc] = dh2[n0 / 16][j] * (1.f - hv * hv);
+		}
+	wave_sync();
+	// dW2 += H1^T dA2, db2
+	for (int i0 = 0; i0 < NH; i0 += 16)
+		for (int n0 = 0; n0 < NH; n0 += 16)
+			G.w2[(i0 / 16) * 4 + n0 / 16] += tile_mm(T.h1 + i0, 1, NH, T.h2 + n0, NH, 1, 16);
+	for (int rr = 0; rr < 16; ++rr)
+		G.b2 += T.h2[rr * NH + l];
+	// dA1 = (dA2 W2^T) * (1 - h1^2), into h1
+	f4 dh1[4];
+	for (int n0 = 0; n0 < NH; n0 += 16)
+		dh1[n0 / 16] = tile_mm(T.h2, NH, 1, lw.w2 + n0 * NH, 1, NH, NH);
+	wave_sync();
+	for (int n0 = 0; n0 < NH; n0 += 16)
+		for (int j = 0; j < 4; ++j)
+		{
+			float hv = T.h1[(rq + j) * NH + n0 + lc];
+			T.h1[(rq + j) * NH + n0 + lc] = dh1[n0 / 16][j] * (1.f - hv * hv);
+		}
+	wave_sync();
+	// dW1 += X^T dA1, db1
+	for (int i0 = 0; i0 < XS; i0 += 16)
+		for (int n0 = 0; n0 < NH; n0 += 16)
+			G.w1[(i0 / 16) * 4 + n0 / 16] += tile_mm(T.x + i0, 1, XS, T.h1 + n0, NH, 1, 16);
+	for (int rr = 0; rr < 16; ++rr)
+		G.b1 += T.h1[rr * NH + l];
+	// dX = dA1 W1^T (feature columns < NI)
+	for (int n0 = 0; n0 < XS; n0 += 16)
+	{
+		f4 dxt = tile_mm(T.h1, NH, 1, lw.w1 + n0 * NH, 1, NH, NH);
+		for (int j = 0; j < 4; ++j)
+		{
+			int r = r0 + rq + j, f = n0 + lc;
+			if (f < NI && r < nrows)
+				dx[(long)r * NI + f] = dxt[j];
+		}
+	}
+	wave_sync();
+}
+
+// the four wavefronts' accumulators, added in wavefront order, into the
+// layer's per-configuration gradient pl[0 .. NP)
+template <int NI> __device__ void store_grad(Grad<NI> const &G, float *__restrict__ pl)
+{
+	using Y = Layout<NI>;
+	constexpr int XS = Y::XS;
+	const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lc = l & 15, rq = (l >> 4) * 4;
+	for (int ww = 0; ww < 4; ++ww)
+	{
+		if (w == ww)
+		{
+			for (int i0 = 0; i0 < XS; i0 += 16)
+				for (int n0 = 0; n0 < NH; n0 += 16)
+					for (int j = 0; j < 4; ++j)
+					{
+						int k = i0 + rq + j, n = n0 + lc;
+						if (k < NI)
+						{
+							float *dst = pl + Y::OW1 + k * NH + n;
+							*dst = (ww ? *dst : 0.f) + G.w1[(i0 / 16) * 4 + n0 / 16][j];
+						}
+					}
+			for (int i0 = 0; i0 < NH; i0 += 16)
+				for (int n0 = 0; n0 < NH; n0 += 16)
+					for (int j = 0; j < 4; ++j)
+					{
+						float *dst = pl + Y::OW2 + (i0 + rq + j) * NH + n0 + lc;
+						*dst = (ww ? *dst : 0.f) + G.w2[(i0 / 16) * 4 + n0 / 16][j];
+					}
+			for (int i0 = 0; i0 < NH; i0 += 16)
+				for (int j = 0; j < 4; ++j)
+					if (lc < NO)
+					{
+						float *dst = pl + Y::OW3 + (i0 + rq + j) * NO + lc;
+						*dst = (ww ? *dst : 0.f) + G.w3[i0 / 16][j];
+					}
+			pl[Y::OB1 + l] = (ww ? pl[Y::OB1 + l] : 0.f) + G.b1;
+			pl[Y::OB2 + l] = (ww ? pl[Y::OB2 + l] : 0.f) + G.b2;
+			if (l < NO)
+				pl[Y::OB3 + l] = (ww ? pl[Y::OB3 + l] : 0.f) + G.b3;
+		}
+		__syncthreads();
+	}
+}
+
+// initial weights of n parameters (layers of NP each): W ~ N(0, 1/fan_in) for
+// the hidden layers, the last layer 1e-2 of that (identical components would
+// receive identical gradients for ever) with the bias of the identity
+// transform (s_k = 1, w uniform, t = 0)
+template <int NI> __device__ void init_param(int i, float *p, uint64_t seed, uint32_t tag)
+{
+	using Y = Layout<NI>;
+	int o = i % Y::NP;
+	float v = 0.f;
+	double sd = 0.0;
+	if (o >= Y::OW1 && o < Y::OB1)
+		sd = 1.0 / sqrt((double)NI);
+	else if (o >= Y::OW2 && o < Y::OB2)
+		sd = 1.0 / sqrt((double)NH);
+	else if (o >= Y::OW3 && o < Y::OB3)
+		sd = 1e-2 / sqrt((double)NH);
+	if (sd > 0)
+	{
+		mesh_dev::u4 ctr = {(uint32_t)i, 0u, 0u, tag};
+		mesh_dev::u4 r = mesh_dev::philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+		double z0, z1;
+		mesh_dev::normal_pair(r, &z0, &z1);
+		v = (float)(z0 * sd);
+	}
+	if (o == Y::OB3 + 2 * NK)
+		v = 1.f;
+	p[i] = v;
+}
+
+} // namespace flow_mlp
