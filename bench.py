@@ -292,7 +292,8 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
     """U(1) 2-d at L x L: ESS/s of the topological charge Q for plain HMC
     (2lf, tau = 1, `substeps` steps) and for HMC interleaved 1:1 with
     independence MH steps proposed by each trained flow ("pflow": plaquette
-    flow, "coupling": `layers` coupling layers).  start: "hot" (uniform angles) or
+    flow, "coupling": `layers` coupling layers on links, "pcflow": `layers`
+    coupling layers on plaquettes).  start: "hot" (uniform angles) or
     "cold" (unit field) for every chain.  Training time is reported apart
     from the sampling time the ESS/s figures use."""
     import numpy as np
@@ -333,6 +334,9 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
         if kind == "pflow":
             flow = U1.PlaqFlow(L, L)
             lr, lr_final = 2e-2, 1e-3
+        elif kind == "pcflow":
+            flow = U1.PlaqCouplingFlow(L, L, seed=seed, layers=layers)
+            lr, lr_final = 1e-3, 1e-4
         else:
             flow = U1.Flow(L, L, seed=seed, layers=layers)
             lr, lr_final = 1e-3, 1e-4
@@ -343,7 +347,7 @@ def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pfl
         ens2 = new_ensemble(seed + 2)
         r2 = U1.run_chains(ens2, beta, deltas, flow_steps, seed + 3, flow=flow, n_therm=100)
         out[kind] = summary(r2, {"acceptance_flow": r2["acc_flow"], "train_steps": train_steps,
-                                 **({"layers": layers} if kind == "coupling" else {}),
+                                 **({"layers": layers} if kind in ("coupling", "pcflow") else {}),
                                  "train_seconds": t_train, "loss_first": float(np.mean(hist[:10])),
                                  "loss_last": float(np.mean(hist[-50:]))})
         out[kind]["ratio_vs_hmc"] = out[kind]["ess_per_s"] / out["hmc"]["ess_per_s"]
@@ -589,7 +593,7 @@ def main():
     ap.add_argument("--u1-train", type=int, default=4000)
     ap.add_argument("--u1-hmc-steps", type=int, default=4000)
     ap.add_argument("--u1-flow-steps", type=int, default=1000)
-    ap.add_argument("--u1-flows", default="pflow,coupling", help="flows to measure (pflow, coupling)")
+    ap.add_argument("--u1-flows", default="pflow,coupling", help="flows to measure (pflow, coupling, pcflow)")
     ap.add_argument("--u1-substeps", type=int, default=20, help="HMC steps per unit trajectory")
     ap.add_argument("--u1-start", choices=("hot", "cold"), default="hot")
     ap.add_argument("--u1-therm", type=int, default=500, help="HMC thermalisation trajectories")

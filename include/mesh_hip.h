@@ -490,6 +490,41 @@ int mesh_pflow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, const floa
 /* Adam step t >= 1 (moments zero-initialised by the caller) */
 int mesh_pflow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, float *v, int t, double lr);
 
+/* ---- U(1) plaquette coupling flow (no counterpart in the reference) --------
+ *
+ * Coupling layers on the V - 1 free plaquettes of a 2-d torus (all but the
+ * last, (L0-1, L1-1)), which start uniform; the links follow as for
+ * mesh_pflow_* (maximal-tree gauge, two uniform holonomies, uniform gauge
+ * transformation).  Layer l moves the plaquettes of parity class l % 4,
+ * (x0 mod 2, x1 mod 2) = (m % 2, m / 2), by h(a) = g(wrap(a - t)) + t, g a
+ * mixture of 4 non-compact projections whose parameters an MLP
+ * 32 -> 64 -> 64 -> 10 (tanh) computes from (cos, sin) of the 8 surrounding
+ * plaquettes and of the 8 1x2 loops formed by neighbouring pairs of them (the
+ * last plaquette enters as 0).  Exact log-density:
+ *   log q(U) = -sum_{layers, active} log g' - 2V log 2 pi.
+ * Even extents >= 4, V <= 4096, 1 <= layers <= 64; weights as mesh_flow_*
+ * with 32 input features (mesh_pcflow_num_params(layers) floats, device). */
+int mesh_pcflow_num_params(int layers);
+/* initial weights (Philox(seed)): the identity map plus 1e-2 noise */
+int mesh_pcflow_init(mesh_ctx *ctx, int layers, float *params, uint64_t seed);
+/* batch proposals theta_out (links, [batch][2][L0][L1]) from Philox(seed,
+ * step); out_dev[2c + {0,1}] = {log q(U), S(U)/beta} */
+int mesh_pcflow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params, uint64_t seed,
+                       uint32_t step, float *theta_out, double *out_dev);
+/* out_dev[2c + {0,1}] = {log q(U), S(U)/beta} of configurations theta;
+ * z_out (optional, [batch][V]): the prior plaquettes the inverse recovers */
+int mesh_pcflow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params,
+                     const float *theta, float *z_out, double *out_dev);
+/* bytes of the workspace mesh_pcflow_gradient needs */
+size_t mesh_pcflow_workspace_bytes(const mesh_grid *g, int batch, int layers);
+/* reverse-KL gradient grad = (1/batch) sum_c d/dparams [log q + beta S] over a
+ * fresh batch of draws (Philox(seed, step)); out_dev as mesh_pcflow_sample */
+int mesh_pcflow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params, uint64_t seed,
+                         uint32_t step, double beta, void *workspace, float *grad, double *out_dev);
+/* Adam step t >= 1 (moments zero-initialised by the caller) */
+int mesh_pcflow_adam(mesh_ctx *ctx, int layers, float *params, const float *grad, float *m, float *v, int t,
+                     double lr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -129,6 +129,13 @@ _SIGS = {
     "mesh_pflow_workspace_bytes": (C.c_size_t, [gp, i32]),
     "mesh_pflow_gradient": (i32, [vp, gp, i32, vp, u64, u32, f64, vp, vp, vp]),
     "mesh_pflow_adam": (i32, [vp, vp, vp, vp, vp, i32, f64]),
+    "mesh_pcflow_num_params": (i32, [i32]),
+    "mesh_pcflow_init": (i32, [vp, i32, vp, u64]),
+    "mesh_pcflow_sample": (i32, [vp, gp, i32, i32, vp, u64, u32, vp, vp]),
+    "mesh_pcflow_logq": (i32, [vp, gp, i32, i32, vp, vp, vp, vp]),
+    "mesh_pcflow_workspace_bytes": (C.c_size_t, [gp, i32, i32]),
+    "mesh_pcflow_gradient": (i32, [vp, gp, i32, i32, vp, u64, u32, f64, vp, vp, vp]),
+    "mesh_pcflow_adam": (i32, [vp, i32, vp, vp, vp, vp, i32, f64]),
 }
 
 

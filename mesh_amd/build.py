@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmesh_hip.so")
-SOURCES = ["runtime.hip", "su3_hmc.hip", "landau.hip", "u1.hip", "flow.hip", "pflow.hip", "su2.hip"]
+SOURCES = ["runtime.hip", "su3_hmc.hip", "landau.hip", "u1.hip", "flow.hip", "pflow.hip", "pcflow.hip", "su2.hip"]
 HEADERS = ["mesh_internal.h", "su3_dev.h", "rng_dev.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -27,6 +27,7 @@
 // the others; its Boltzmann weight is what the Metropolis step corrects.
 #include "mesh_internal.h"
 #include "rng_dev.h"
+#include "u1_plaq.h"
 
 // log q: table-guided inverse of the circle map (0: Newton from the target)
 #ifndef MESH_PFLOW_TABLE
@@ -34,6 +35,11 @@
 #endif
 
 using namespace mesh_dev;
+using u1_plaq::block_sum;
+using u1_plaq::PGeo;
+using u1_plaq::plaqp;
+using u1_plaq::site;
+using u1_plaq::wrapd;
 
 namespace {
 
@@ -45,35 +51,10 @@ constexpr float PI_F = 3.14159265358979323846f;
 constexpr float TWO_PI_F = 6.28318530717958647692f;
 constexpr double LOG_2PI = 1.8378770664093454836;
 
-struct PGeo
-{
-	int L0, L1, V;
-};
-
 __device__ __forceinline__ float wrapf(float a)
 {
 	a = a - TWO_PI_F * floorf((a + PI_F) * (1.0f / TWO_PI_F));
 	return a >= PI_F ? a - TWO_PI_F : (a < -PI_F ? a + TWO_PI_F : a);
-}
-
-__device__ __forceinline__ double wrapd(double a)
-{
-	const double pi = 3.141592653589793238, tpi = 6.283185307179586477;
-	a = a - tpi * floor((a + pi) * (1.0 / tpi));
-	return a >= pi ? a - tpi : (a < -pi ? a + tpi : a);
-}
-
-__device__ __forceinline__ int site(PGeo const &g, int x0, int x1)
-{
-	x0 = x0 == g.L0 ? 0 : x0;
-	x1 = x1 == g.L1 ? 0 : x1;
-	return x0 * g.L1 + x1;
-}
-
-// plaquette at (x0, x1) of an LDS configuration t[2V], unwrapped
-__device__ __forceinline__ float plaqp(PGeo const &g, const float *t, int x0, int x1)
-{
-	return t[site(g, x0, x1)] + t[g.V + site(g, x0 + 1, x1)] - t[site(g, x0, x1 + 1)] - t[g.V + site(g, x0, x1)];
 }
 
 struct Map
@@ -189,30 +170,9 @@ __device__ __forceinline__ float g_inverse_tab(Map const &p, const float *tab, f
 	return g_inverse(p, target, Gp, lo, hi, u);
 }
 
-// uniform on [-pi, pi) for counter word (a, b) of configuration c
 __device__ __forceinline__ float uniform_angle(uint64_t seed, uint32_t step, uint32_t a, uint32_t b)
 {
-	u4 ctr = {a, b, step << 5, TAG_PFLOW};
-	u4 r = philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
-	double u = u01_from_bits(r.x, r.y) - 0x1p-53;
-	return (float)(6.283185307179586477 * u - 3.141592653589793238);
-}
-
-template <int NQ> __device__ __forceinline__ void block_sum(double (&v)[NQ], double (*red)[4])
-{
-	int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-	for (int q = 0; q < NQ; ++q)
-	{
-		double s = v[q];
-		for (int o = 32; o >= 1; o >>= 1)
-			s += __shfl_xor(s, o, 64);
-		if (lane == 0)
-			red[q][w] = s;
-	}
-	__syncthreads();
-	for (int q = 0; q < NQ; ++q)
-		v[q] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
-	__syncthreads();
+	return u1_plaq::uniform_angle(seed, step, a, b, TAG_PFLOW);
 }
 
 // Proposals: links of configuration c into theta_out[c][2V].
@@ -241,41 +201,7 @@ __global__ __launch_bounds__(THREADS) void k_pflow_sample(PGeo g, const float *_
 	__syncthreads();
 	const float h0 = uniform_angle(seed, step, (uint32_t)g.V, cc);
 	const float h1 = uniform_angle(seed, step, (uint32_t)g.V + 1u, cc);
-	// direction 0: suffix sums along each row (one thread per row)
-	for (int x0 = threadIdx.x; x0 < g.L0; x0 += THREADS)
-	{
-		double acc = x0 == g.L0 - 1 ? (double)h0 : 0.0;
-		t[x0 * g.L1 + g.L1 - 1] = (float)wrapd(acc);
-		for (int x1 = g.L1 - 2; x1 >= 0; --x1)
-		{
-			acc += (double)P[x0 * g.L1 + x1];
-			t[x0 * g.L1 + x1] = (float)wrapd(acc);
-		}
-		for (int x1 = 0; x1 < g.L1 - 1; ++x1)
-			t[g.V + x0 * g.L1 + x1] = 0.f;
-	}
-	__syncthreads();
-	// direction 1, last column: prefix sum over rows (one thread)
-	if (threadIdx.x == 0)
-	{
-		double acc = (double)h1;
-		t[g.V + g.L1 - 1] = (float)wrapd(acc);
-		for (int x0 = 0; x0 + 1 < g.L0; ++x0)
-		{
-			int r = x0 * g.L1;
-			acc += (double)P[r + g.L1 - 1] - (double)t[r + g.L1 - 1] + (double)t[r];
-			t[g.V + r + g.L1 + g.L1 - 1] = (float)wrapd(acc);
-		}
-	}
-	__syncthreads();
-	// gauge transformation into the output
-	float *out = theta_out + (long)c * 2 * g.V;
-	for (int x = threadIdx.x; x < g.V; x += THREADS)
-	{
-		int x0 = x / g.L1, x1 = x - x0 * g.L1;
-		out[x] = wrapf(t[x] + al[x] - al[site(g, x0 + 1, x1)]);
-		out[g.V + x] = wrapf(t[g.V + x] + al[x] - al[site(g, x0, x1 + 1)]);
-	}
+	u1_plaq::links_from_plaquettes(g, P, h0, h1, al, t, theta_out + (long)c * 2 * g.V, THREADS);
 }
 
 // out[2c + {0,1}] = {log q(U), S(U)/beta} of configurations theta[c][2V]

@@ -105,12 +105,14 @@ class Flow:
     """Gauge-equivariant coupling-layer flow on an L0 x L1 lattice (8 layers
     in the specified design; `layers` up to 64)."""
 
+    _api = "mesh_flow"
+
     def __init__(self, L0, L1, ctx=None, seed=0, layers=8):
         self.ctx = ctx or default_context()
         self.grid = Grid((L0, L1))
         self.L0, self.L1, self.V = L0, L1, L0 * L1
         self.layers = int(layers)
-        self.n = lib().mesh_flow_num_params(self.layers)
+        self.n = getattr(lib(), f"{self._api}_num_params")(self.layers)
         if self.n <= 0:
             raise MeshError(-1, "the flow has 1 to 64 layers")
         self.params = Buffer(self.n, np.float32, self.ctx)
@@ -120,7 +122,7 @@ class Flow:
         self.t = 0
         self._ws = None
         self._ws_batch = 0
-        self.ctx.call("mesh_flow_init", self.layers, self.params.ptr, C.c_uint64(seed))
+        self.ctx.call(f"{self._api}_init", self.layers, self.params.ptr, C.c_uint64(seed))
 
     def get_params(self):
         return self.params.download()
@@ -130,7 +132,7 @@ class Flow:
 
     def _workspace(self, batch):
         if self._ws is None or self._ws_batch != batch:
-            nb = lib().mesh_flow_workspace_bytes(self.grid.ref, batch, self.layers)
+            nb = getattr(lib(), f"{self._api}_workspace_bytes")(self.grid.ref, batch, self.layers)
             if nb == 0:
                 raise MeshError(-2, "unsupported flow geometry")
             self._ws = Buffer(nb, np.uint8, self.ctx)
@@ -141,7 +143,7 @@ class Flow:
     def gradient(self, batch, beta, seed, step):
         """Reverse-KL gradient over a fresh batch; returns (loss, log q, S/beta)."""
         ws = self._workspace(batch)
-        self.ctx.call("mesh_flow_gradient", self.grid.ref, batch, self.layers, self.params.ptr, C.c_uint64(seed),
+        self.ctx.call(f"{self._api}_gradient", self.grid.ref, batch, self.layers, self.params.ptr, C.c_uint64(seed),
                       C.c_uint32(step), _d(beta), ws.ptr, self.grad.ptr, self._out.ptr)
         o = self._out.download().reshape(batch, 2)
         logq, S = o[:, 0], o[:, 1]
@@ -155,8 +157,8 @@ class Flow:
             loss, _, _ = self.gradient(batch, beta, seed, self.t)
             self.t += 1
             lr_i = lr if lr_final is None else lr * (lr_final / lr) ** (i / max(1, steps - 1))
-            self.ctx.call("mesh_flow_adam", self.layers, self.params.ptr, self.grad.ptr, self.m.ptr, self.v.ptr, self.t,
-                          _d(lr_i))
+            self.ctx.call(f"{self._api}_adam", self.layers, self.params.ptr, self.grad.ptr, self.m.ptr, self.v.ptr,
+                          self.t, _d(lr_i))
             hist.append(loss)
             if log_every and (i % log_every == 0 or i == steps - 1):
                 print(f"flow step {i}: loss {loss:.4f}", flush=True)
@@ -173,8 +175,25 @@ class Flow:
 
     def logq(self, ens, out=None, z_out=None):
         out = out or Buffer(2 * ens.batch, np.float64, self.ctx)
-        self.ctx.call("mesh_flow_logq", self.grid.ref, ens.batch, self.layers, self.params.ptr, ens.theta.ptr,
+        self.ctx.call(f"{self._api}_logq", self.grid.ref, ens.batch, self.layers, self.params.ptr, ens.theta.ptr,
                       None if z_out is None else z_out.ptr, out.ptr)
+        return out
+
+
+class PlaqCouplingFlow(Flow):
+    """Coupling layers on the plaquettes of an L0 x L1 lattice (pcflow.hip):
+    layer l moves the plaquettes of parity class l % 4 through a circle map
+    whose parameters an MLP computes from the surrounding plaquettes and 1x2
+    loops; the links follow by a maximal-tree gauge with uniform holonomies
+    and gauge.  Same interface as Flow (even extents >= 4)."""
+
+    _api = "mesh_pcflow"
+
+    def sample(self, ens, seed, step, theta_out=None, out=None):
+        tgt = theta_out if theta_out is not None else ens.theta
+        out = out or Buffer(2 * ens.batch, np.float64, self.ctx)
+        self.ctx.call("mesh_pcflow_sample", self.grid.ref, ens.batch, self.layers, self.params.ptr, C.c_uint64(seed),
+                      C.c_uint32(step), tgt.ptr, out.ptr)
         return out
 
 

@@ -440,6 +440,151 @@ def pflow_loss(params, u, beta):
     return logq + beta * S, logq, S
 
 
+# ---------------------------------------------------------------- plaquette coupling flow
+#
+# Restatement of mesh_amd/csrc/pcflow.hip: the V - 1 free plaquettes (the last,
+# (L0-1, L1-1), enters contexts as 0) start uniform; layer l moves the
+# plaquettes of parity class m = l % 4, (x0 % 2, x1 % 2) = (m % 2, m // 2), by
+# h(a) = g(wrap(a - t)) + t, the parameters from an MLP of (cos, sin) of the 8
+# ring plaquettes and the 8 1x2 loops of neighbouring ring pairs; the links
+# follow as for the plaquette flow (pflow_links).
+
+PC_NI = 32
+PC_NP_LAYER = PC_NI * NH + NH + NH * NH + NH + NH * NO + NO
+TAG_PCFZ = 0x5043465A
+# ring offsets (d0, d1), in order around the plaquette
+PC_RING = ((-1, -1), (-1, 0), (-1, 1), (0, 1), (1, 1), (1, 0), (1, -1), (0, -1))
+
+
+def pcflow_draws(seed, step, chain, L0, L1):
+    """(z[V-1], a[V], h0, h1) the GPU draws for (seed, step, chain), float32."""
+    V = L0 * L1
+    cc = chain << 1
+
+    def ang(a, b):
+        u = O.philox_draw(seed, a, b, step << 5, TAG_PCFZ)[0] - 2.0 ** -53
+        return np.float32(6.283185307179586477 * u - 3.141592653589793238)
+
+    z = np.array([ang(x, cc) for x in range(V - 1)], np.float32)
+    a = np.array([ang(x, cc | 1) for x in range(V)], np.float32)
+    return z, a, ang(V, cc), ang(V + 1, cc)
+
+
+def _pc_layer_params(p, layer):
+    o = layer * PC_NP_LAYER
+    sizes = [(PC_NI, NH), (NH,), (NH, NH), (NH,), (NH, NO), (NO,)]
+    out = []
+    for sh in sizes:
+        n = int(np.prod(sh))
+        out.append(p[o:o + n].reshape(sh))
+        o += n
+    return out
+
+
+def _pc_class_mask(L0, L1, m):
+    x0 = np.arange(L0).reshape(L0, 1)
+    x1 = np.arange(L1).reshape(1, L1)
+    mask = ((x0 % 2) == (m % 2)) & ((x1 % 2) == (m // 2))
+    mask[L0 - 1, L1 - 1] = False
+    return mask
+
+
+def _pc_mlp_y(W, phi, mask):
+    """MLP outputs (B, NA, NO) of the active plaquettes for contexts phi (B, L0, L1)
+    whose last plaquette is 0."""
+    import torch
+
+    W1, b1, W2, b2, W3, b3 = W
+    ring = [torch.roll(phi, shifts=(-d0, -d1), dims=(1, 2))[:, mask] for d0, d1 in PC_RING]
+    feats = []
+    for k in range(8):
+        feats += [torch.cos(ring[k]), torch.sin(ring[k])]
+    for k in range(8):
+        lp = ring[k] + ring[(k + 1) % 8]
+        feats += [torch.cos(lp), torch.sin(lp)]
+    X = torch.stack(feats, dim=-1)
+    H1 = torch.tanh(X @ W1 + b1)
+    H2 = torch.tanh(H1 @ W2 + b2)
+    return H2 @ W3 + b3
+
+
+def _pc_ncp(Y):
+    import torch
+
+    s = torch.exp(torch.clamp(Y[..., :NK], -8.0, 8.0))
+    w = torch.softmax(Y[..., NK:2 * NK], dim=-1)
+    t = torch.atan2(Y[..., 2 * NK + 1], Y[..., 2 * NK])
+    return s, w, t
+
+
+def pcflow_forward(params, z, L0, L1, inverse=False):
+    """torch float64 flow on free plaquettes z (B, V-1).  Forward: (phi, logdet)
+    with phi the plaquettes after the last layer and logdet = sum of log g'(u);
+    inverse: z given the flow's output (the same logdet, at the inverse's
+    points, by bisection to float64 precision)."""
+    import torch
+
+    B = z.shape[0]
+    V = L0 * L1
+    nl = params.numel() // PC_NP_LAYER
+    phi = torch.cat([z, torch.zeros(B, 1, dtype=z.dtype)], dim=1).reshape(B, L0, L1)
+    logdet = torch.zeros(B, dtype=z.dtype)
+    layers = range(nl - 1, -1, -1) if inverse else range(nl)
+    for layer in layers:
+        m = layer % 4
+        mask = torch.as_tensor(_pc_class_mask(L0, L1, m))
+        s, w, t = _pc_ncp(_pc_mlp_y(_pc_layer_params(params, layer), phi, mask))
+        a = phi[:, mask]
+        if inverse:
+            target = _twrap(a - t)
+            lo = torch.full_like(target, -math.pi)
+            hi = torch.full_like(target, math.pi)
+            for _ in range(80):
+                mid = 0.5 * (lo + hi)
+                G, _ = _pflow_g(s, w, mid)
+                less = G < target
+                lo = torch.where(less, mid, lo)
+                hi = torch.where(less, hi, mid)
+            u = 0.5 * (lo + hi)
+            _, Gp = _pflow_g(s, w, u)
+            new = _twrap(u + t)
+        else:
+            u = _twrap(a - t)
+            G, Gp = _pflow_g(s, w, u)
+            new = _twrap(G + t)
+        logdet = logdet + torch.sum(torch.log(Gp), dim=-1)
+        phi = phi.clone()
+        phi[:, mask] = new
+    return phi.reshape(B, V)[:, :V - 1], logdet
+
+
+def pcflow_loss(params, z, L0, L1, beta):
+    """per-configuration log q + beta S for prior draws z (B, V-1) (torch
+    float64), the last plaquette = -sum of the others; returns (loss, log q,
+    S, phi)."""
+    import torch
+
+    phi, ld = pcflow_forward(params, z, L0, L1)
+    V = L0 * L1
+    S = torch.sum(1 - torch.cos(phi), dim=-1) + (1 - torch.cos(torch.sum(phi, dim=-1)))
+    logq = -ld - 2 * V * math.log(2 * math.pi)
+    return logq + beta * S, logq, S, phi
+
+
+def pcflow_logq(params, th):
+    """(log q, S/beta, z) of configurations th (B, 2, L0, L1) (numpy in, numpy out)."""
+    import torch
+
+    th = np.asarray(th, np.float64)
+    B, _, L0, L1 = th.shape
+    V = L0 * L1
+    P = wrap(plaq(th)).reshape(B, V)
+    S = np.sum(1 - np.cos(P), axis=-1)
+    p = torch.as_tensor(np.asarray(params, np.float64))
+    z, ld = pcflow_forward(p, torch.as_tensor(P[:, :V - 1]), L0, L1, inverse=True)
+    return -ld.numpy() - 2 * V * math.log(2 * math.pi), S, z.numpy()
+
+
 # ---- Landau gauge on angles (src/gauge/landau.h for U1<T>): links
 # th[mu][x...] and the gauge field g = exp(i phi), phi[x...]; float64 from
 # the float32 angles, in the operation order of mesh_amd/csrc/u1.hip

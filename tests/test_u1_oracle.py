@@ -235,6 +235,121 @@ def test_pflow_loss_gradient_finite_difference():
         assert abs(fd - g[i]) < 1e-6 * (1 + abs(g[i])), (i, fd, g[i])
 
 
+def _pc_params(seed, scale, layers=8):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(size=layers * U.PC_NP_LAYER) * scale
+    for layer in range(layers):
+        p[(layer + 1) * U.PC_NP_LAYER - U.NO + 2 * U.NK] += 1.0
+    return p
+
+
+def _pc_jacobian(p, z, L0, L1):
+    import torch
+
+    return torch.autograd.functional.jacobian(
+        lambda v: U.pcflow_forward(p, v.unsqueeze(0), L0, L1)[0][0], z).numpy()
+
+
+def test_pcflow_logdet_against_jacobian():
+    """log det of the plaquette coupling flow z -> phi (the V - 1 free
+    plaquettes) equals the sum of log g' that the flow reports, for a brute
+    force Jacobian of the whole 8-layer map on 4x4 and 4x6."""
+    torch = pytest.importorskip("torch")
+    for L0, L1, seed in ((4, 4, 30), (4, 6, 31)):
+        V = L0 * L1
+        p = torch.tensor(_pc_params(seed, 0.3))
+        z = torch.tensor(np.random.default_rng(seed).uniform(-np.pi, np.pi, V - 1))
+        _, ld = U.pcflow_forward(p, z.unsqueeze(0), L0, L1)
+        J = _pc_jacobian(p, z, L0, L1)
+        sign, logabs = np.linalg.slogdet(J)
+        assert sign > 0
+        assert abs(logabs - ld.item()) < 1e-9, (logabs, ld.item())
+        assert abs(ld.item()) > 0.5  # a non-trivial map
+        assert np.abs(J - np.diag(np.diag(J))).max() > 1e-2  # with couplings
+
+
+def test_pcflow_layer_structure():
+    """One layer of class m: the plaquettes of the other classes do not move
+    (unit rows), and an active plaquette depends on itself and on its 8
+    surrounding plaquettes only; the last plaquette is never a variable. The
+    four classes cover every free plaquette once."""
+    torch = pytest.importorskip("torch")
+    L0, L1 = 6, 4
+    V = L0 * L1
+    z = torch.tensor(np.random.default_rng(32).uniform(-np.pi, np.pi, V - 1))
+    seen = np.zeros(V - 1, int)
+    for m in range(4):
+        p = torch.tensor(_pc_params(33 + m, 0.4, layers=m + 1))
+        # layers 0 .. m - 1 with zero weights (identity), layer m non-trivial
+        p[:m * U.PC_NP_LAYER] = 0.0
+        for layer in range(m):
+            p[(layer + 1) * U.PC_NP_LAYER - U.NO + 2 * U.NK] = 1.0
+        J = _pc_jacobian(p, z, L0, L1)
+        mask = U._pc_class_mask(L0, L1, m).reshape(-1)[:V - 1]
+        seen += mask
+        for x in range(V - 1):
+            row = J[x]
+            if not mask[x]:
+                np.testing.assert_allclose(row, np.eye(V - 1)[x], atol=1e-14)
+                continue
+            x0, x1 = divmod(x, L1)
+            ring = {((x0 + d0) % L0) * L1 + (x1 + d1) % L1 for d0, d1 in U.PC_RING}
+            allowed = (ring | {x}) - {V - 1}
+            for y in range(V - 1):
+                if y not in allowed:
+                    assert row[y] == 0.0, (m, x, y)
+            assert row[x] > 0
+            assert max(abs(row[y]) for y in allowed - {x}) > 1e-4
+    np.testing.assert_array_equal(seen, np.ones(V - 1, int))
+
+
+def test_pcflow_identity_inverse_and_gauge_invariance():
+    """Zero weights with the identity bias are the identity (log q = the
+    uniform density); the inverse recovers the prior draw and the same log
+    det; log q of links built from a sample (any holonomies and gauge) is the
+    sample's own log q."""
+    torch = pytest.importorskip("torch")
+    L0, L1 = 4, 6
+    V = L0 * L1
+    rng = np.random.default_rng(34)
+    z = torch.tensor(rng.uniform(-np.pi, np.pi, size=(3, V - 1)))
+    ident = torch.zeros(8 * U.PC_NP_LAYER, dtype=torch.float64)
+    for layer in range(8):
+        ident[(layer + 1) * U.PC_NP_LAYER - U.NO + 2 * U.NK] = 1.0
+    phi, ld = U.pcflow_forward(ident, z, L0, L1)
+    assert torch.abs(U._twrap(phi - z)).max().item() < 1e-12
+    assert torch.abs(ld).max().item() < 1e-12
+    p = _pc_params(35, 0.3)
+    phi, ld = U.pcflow_forward(torch.tensor(p), z, L0, L1)
+    zi, ldi = U.pcflow_forward(torch.tensor(p), phi, L0, L1, inverse=True)
+    assert torch.abs(U._twrap(zi - z)).max().item() < 1e-10
+    assert torch.abs(ldi - ld).max().item() < 1e-9
+    for c in range(3):
+        th = U.pflow_links(phi[c].numpy(), rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(-3, 3, V), L0, L1)
+        lq, S, zz = U.pcflow_logq(p, th[None])
+        assert abs(lq[0] - (-ld[c].item() - 2 * V * math.log(2 * math.pi))) < 1e-9
+        assert np.abs(U.wrap(zz[0] - z[c].numpy())).max() < 1e-9
+        assert abs(S[0] - U.action_over_beta(th)) < 1e-9
+
+
+def test_pcflow_loss_gradient_finite_difference():
+    torch = pytest.importorskip("torch")
+    L0, L1 = 4, 4
+    rng = np.random.default_rng(36)
+    p = torch.tensor(_pc_params(37, 0.3, layers=4), requires_grad=True)
+    z = torch.tensor(rng.uniform(-np.pi, np.pi, size=(3, 15)))
+    U.pcflow_loss(p, z, L0, L1, 2.0)[0].mean().backward()
+    g = p.grad.numpy()
+    h = 1e-6
+    for i in rng.choice(p.numel(), 40, replace=False):
+        pp, pm = p.detach().clone(), p.detach().clone()
+        pp[i] += h
+        pm[i] -= h
+        fd = (U.pcflow_loss(pp, z, L0, L1, 2.0)[0].mean() - U.pcflow_loss(pm, z, L0, L1, 2.0)[0].mean()).item()
+        fd /= 2 * h
+        assert abs(fd - g[i]) < 1e-6 * (1 + abs(g[i])), (i, fd, g[i])
+
+
 def test_u1_nd_oracle_reduces_to_2d():
     """The any-dimension oracle on a 2-d lattice is the 2-d oracle: the same
     random angles and momenta (the same Philox streams), plaquettes, force and
