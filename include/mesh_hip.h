@@ -515,6 +515,14 @@ int mesh_pcflow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers,
  * z_out (optional, [batch][V]): the prior plaquettes the inverse recovers */
 int mesh_pcflow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params,
                      const float *theta, float *z_out, double *out_dev);
+/* one MH proposal round in one launch: state_dev[2c + {0,1}] = {log q, S/beta}
+ * of the current configurations theta (as mesh_pcflow_logq) and proposals
+ * theta_prop with prop_dev (as mesh_pcflow_sample); theta_prop must not alias
+ * theta.  The two halves of the launch run side by side where two workgroups
+ * fit a CU (V <= 256). */
+int mesh_pcflow_propose(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params,
+                        const float *theta, double *state_dev, uint64_t seed, uint32_t step, float *theta_prop,
+                        double *prop_dev);
 /* bytes of the workspace mesh_pcflow_gradient needs */
 size_t mesh_pcflow_workspace_bytes(const mesh_grid *g, int batch, int layers);
 /* reverse-KL gradient grad = (1/batch) sum_c d/dparams [log q + beta S] over a

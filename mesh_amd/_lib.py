@@ -133,6 +133,7 @@ _SIGS = {
     "mesh_pcflow_init": (i32, [vp, i32, vp, u64]),
     "mesh_pcflow_sample": (i32, [vp, gp, i32, i32, vp, u64, u32, vp, vp]),
     "mesh_pcflow_logq": (i32, [vp, gp, i32, i32, vp, vp, vp, vp]),
+    "mesh_pcflow_propose": (i32, [vp, gp, i32, i32, vp, vp, vp, u64, u32, vp, vp]),
     "mesh_pcflow_workspace_bytes": (C.c_size_t, [gp, i32, i32]),
     "mesh_pcflow_gradient": (i32, [vp, gp, i32, i32, vp, u64, u32, f64, vp, vp, vp]),
     "mesh_pcflow_adam": (i32, [vp, i32, vp, vp, vp, vp, i32, f64]),

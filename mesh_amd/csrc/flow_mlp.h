@@ -13,6 +13,8 @@
 // b2 [64], W3 [64][10], b3 [10].
 #pragma once
 
+#include <cstddef>
+
 #include "mesh_internal.h"
 #include "rng_dev.h"
 
@@ -94,8 +96,74 @@ template <int NI> struct Tile
 	float x[16 * Layout<NI>::XS], h1[16 * NH], h2[16 * NH], y[16 * 16];
 };
 
+// One layer's parameters held in registers (two floats a lane per chunk) so
+// that the next layer's global loads overlap the current layer's work; NP is
+// even for every NI, so each layer starts 8-byte aligned.  store() writes the
+// LDS image: w1, b1, w2, b2 in the parameters' own order, W3/b3 into their
+// 16-column padding (the pad stays as zero_pad() left it).
+template <int NI, int THREADS> struct WeightRegs
+{
+	using Y = Layout<NI>;
+	static_assert(Y::NP % 2 == 0 && Y::OW3 % 2 == 0 && Y::OB3 % 2 == 0 && NO % 2 == 0, "pairs of parameters");
+	static constexpr int NC = Y::NP / 2;
+	static constexpr int PER = (NC + THREADS - 1) / THREADS;
+	float2 v[PER];
+
+	__device__ void load(const float *__restrict__ p)
+	{
+		const float2 *q = reinterpret_cast<const float2 *>(p);
+		for (int i = 0; i < PER; ++i)
+		{
+			const int c = threadIdx.x + i * THREADS;
+			if (c < NC)
+				v[i] = q[c];
+		}
+	}
+	__device__ void store(LW<NI> &lw) const
+	{
+		static_assert(offsetof(LW<NI>, w3) == Y::OW3 * sizeof(float), "w1, b1, w2, b2 as in the parameters");
+		float2 *head = reinterpret_cast<float2 *>(&lw);
+		for (int i = 0; i < PER; ++i)
+		{
+			const int c = threadIdx.x + i * THREADS, idx = 2 * c;
+			if (c >= NC)
+				break;
+			if (idx < Y::OW3)
+				head[c] = v[i];
+			else if (idx < Y::OB3)
+			{
+				const int k = (idx - Y::OW3) / NO, n = idx - Y::OW3 - k * NO;
+				*reinterpret_cast<float2 *>(&lw.w3[k * 16 + n]) = v[i];
+			}
+			else
+				*reinterpret_cast<float2 *>(&lw.b3[idx - Y::OB3]) = v[i];
+		}
+	}
+};
+
+// zero the W3/b3 padding columns NO..15 once per kernel (store() never writes them)
+template <int NI> __device__ void zero_pad(LW<NI> &lw)
+{
+	for (int i = threadIdx.x; i < NH * 16; i += blockDim.x)
+		if (i % 16 >= NO)
+			lw.w3[i] = 0.f;
+	if (threadIdx.x < 16 && threadIdx.x >= NO)
+		lw.b3[threadIdx.x] = 0.f;
+}
+
+// tanh from the hardware exponential and reciprocal: absolute error a few
+// 1e-7, saturating to +-1 (e = inf or 0)
+__device__ __forceinline__ float tanh_fast(float x)
+{
+	const float e = __expf(2.f * x);
+	return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + e);
+}
+
+template <bool FAST> __device__ __forceinline__ float act(float x) { return FAST ? tanh_fast(x) : tanhf(x); }
+
 // MLP forward of one tile: h1 = tanh(x W1 + b1), h2 = tanh(h1 W2 + b2), y = h2 W3 + b3
-template <int NI> __device__ void mlp_forward(LW<NI> const &lw, Tile<NI> &T)
+// (FAST: tanh_fast; a flow uses one choice in all of its kernels)
+template <int NI, bool FAST = false> __device__ void mlp_forward(LW<NI> const &lw, Tile<NI> &T)
 {
 	constexpr int XS = Layout<NI>::XS;
 	int l = threadIdx.x & 63, c = l & 15, rq = (l >> 4) * 4;
@@ -103,14 +171,14 @@ template <int NI> __device__ void mlp_forward(LW<NI> const &lw, Tile<NI> &T)
 	{
 		f4 d = tile_mm(T.x, XS, 1, lw.w1 + n0, NH, 1, NI);
 		for (int j = 0; j < 4; ++j)
-			T.h1[(rq + j) * NH + n0 + c] = tanhf(d[j] + lw.b1[n0 + c]);
+			T.h1[(rq + j) * NH + n0 + c] = act<FAST>(d[j] + lw.b1[n0 + c]);
 	}
 	wave_sync();
 	for (int n0 = 0; n0 < NH; n0 += 16)
 	{
 		f4 d = tile_mm(T.h1, NH, 1, lw.w2 + n0, NH, 1, NH);
 		for (int j = 0; j < 4; ++j)
-			T.h2[(rq + j) * NH + n0 + c] = tanhf(d[j] + lw.b2[n0 + c]);
+			T.h2[(rq + j) * NH + n0 + c] = act<FAST>(d[j] + lw.b2[n0 + c]);
 	}
 	wave_sync();
 	f4 d = tile_mm(T.h2, NH, 1, lw.w3, 16, 1, NH);
@@ -193,6 +261,110 @@ __device__ __forceinline__ float ncp_inverse(NCP const &p, float target, float &
 			u = 0.5f * (lo + hi);
 	}
 	ncp_eval(p, u, G, Gp);
+	return u;
+}
+
+// ---- the mixture on a quad of lanes: lane k (= lane % 4) of a quad holds
+// component k; sums over the quad pair lanes (0,1)(2,3) and then the pairs,
+// with commutative adds, so that every lane of the quad holds the same bits.
+// A quad is active or inactive as a whole.
+static_assert(NK == 4, "one mixture component per lane of a quad");
+
+template <int CTRL> __device__ __forceinline__ float quad_swap(float v)
+{
+	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float quad_sum(float v)
+{
+	v += quad_swap<0xB1>(v); // quad_perm [1, 0, 3, 2]
+	return v + quad_swap<0x4E>(v); // quad_perm [2, 3, 0, 1]
+}
+
+struct NCPq
+{
+	float s, w, t;
+};
+
+// component lane % 4 of the transform with MLP outputs y
+__device__ __forceinline__ NCPq ncpq_params(const float *y)
+{
+	const int k = threadIdx.x & 3;
+	const float m = fmaxf(fmaxf(y[NK], y[NK + 1]), fmaxf(y[NK + 2], y[NK + 3]));
+	const float e = expf(y[NK + k] - m);
+	NCPq p;
+	p.w = e / quad_sum(e);
+	p.s = expf(fminf(fmaxf(y[k], -8.f), 8.f));
+	p.t = atan2f(y[2 * NK + 1], y[2 * NK]);
+	return p;
+}
+
+// g(u), g'(u) of the mixture, every lane of the quad
+__device__ __forceinline__ void ncpq_eval(NCPq const &p, float u, float &G, float &Gp)
+{
+	float sn, c;
+	sincosf(0.5f * u, &sn, &c);
+	const float d = fmaf(c, c, p.s * p.s * sn * sn);
+	G = quad_sum(p.w * (2.f * atan2f(p.s * sn, c)));
+	Gp = quad_sum(p.w * (p.s / d));
+}
+
+__device__ __forceinline__ float quad_min(float v)
+{
+	v = fminf(v, quad_swap<0xB1>(v));
+	return fminf(v, quad_swap<0x4E>(v));
+}
+__device__ __forceinline__ float quad_max(float v)
+{
+	v = fmaxf(v, quad_swap<0xB1>(v));
+	return fmaxf(v, quad_swap<0x4E>(v));
+}
+
+// u = g^-1(target), the evaluations on the quad (the bracket and the iterates
+// are uniform over it).  g is a convex combination of the increasing maps
+// g_k, so the root lies between the smallest and the largest closed-form
+// component inverse g_k^-1(target) = 2 atan2(sin(target/2), s_k cos(target/2))
+// (widened by a few ulps against their rounding); Newton starts from their
+// w-weighted mean and continues as ncp_inverse.  It also stops once
+// |g(u) - target| is at the rounding level of g (2.5e-7 max(1, |target|)):
+// where g' is small the residual is noise and further steps would only wander
+// inside the noise band (u is then as accurate as the output determines it).
+__device__ __forceinline__ float ncpq_inverse(NCPq const &p, float target, float &Gp)
+{
+	float st, ct;
+	sincosf(0.5f * target, &st, &ct);
+	const float uk = fminf(fmaxf(2.f * atan2f(st, p.s * ct), -PI_F), PI_F);
+	float lo = quad_min(uk), hi = quad_max(uk), u = quad_sum(p.w * uk), G;
+	lo = fmaxf(-PI_F, lo - 5e-7f * fmaxf(1.f, fabsf(lo)));
+	hi = fminf(PI_F, hi + 5e-7f * fmaxf(1.f, fabsf(hi)));
+	u = fminf(fmaxf(u, lo), hi);
+	bool fresh = false; // G, Gp belong to the current u
+	for (int b = 0; b < 40; ++b)
+	{
+		ncpq_eval(p, u, G, Gp);
+		fresh = true;
+		float f = G - target;
+		if (fabsf(f) <= 2.5e-7f * fmaxf(1.f, fabsf(target)))
+			break;
+		if (f < 0.f)
+			lo = u;
+		else
+			hi = u;
+		if (hi - lo <= 2e-7f * fmaxf(1.f, fabsf(u)))
+			break;
+		float un = u - f / Gp;
+		fresh = false;
+		if (un > lo && un < hi)
+		{
+			bool done = fabsf(un - u) <= 2e-7f * fmaxf(1.f, fabsf(un));
+			u = un;
+			if (done)
+				break;
+		}
+		else
+			u = 0.5f * (lo + hi);
+	}
+	if (!fresh)
+		ncpq_eval(p, u, G, Gp);
 	return u;
 }
 

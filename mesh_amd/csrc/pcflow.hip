@@ -44,9 +44,11 @@ constexpr int NI = 32; // features
 constexpr int NP_LAYER = FM::Layout<NI>::NP;
 constexpr int MAX_LAYERS = 64;
 constexpr int THREADS = 256;
+constexpr bool FAST = true; // tanh_fast in every kernel of this flow
 constexpr uint32_t TAG_PCFZ = 0x5043465au; // prior, holonomies, gauge
 constexpr uint32_t TAG_PCFI = 0x50434649u; // initial weights
 constexpr double LOG_2PI = 1.8378770664093454836;
+using Weights = FM::WeightRegs<NI, THREADS>;
 
 // offset of ring position k = 0..7 around a plaquette, in order around it
 __device__ __forceinline__ void ring(int k, int &d0, int &d1)
@@ -64,10 +66,11 @@ __device__ __forceinline__ int sitew(PGeo const &g, int x0, int x1)
 }
 
 // active plaquette r of mask class m
-__device__ __forceinline__ int active_site(PGeo const &g, int m, int r)
+__device__ __forceinline__ void active_xy(PGeo const &g, int m, int r, int &x0, int &x1)
 {
 	const int h = g.L1 >> 1, i = r / h, j = r - i * h;
-	return ((m & 1) + 2 * i) * g.L1 + (m >> 1) + 2 * j;
+	x0 = (m & 1) + 2 * i;
+	x1 = (m >> 1) + 2 * j;
 }
 __device__ __forceinline__ int n_active(PGeo const &g, int m) { return (g.V >> 2) - (m == 3 ? 1 : 0); }
 __device__ __forceinline__ int class_of(PGeo const &g, int x) { return ((x / g.L1) & 1) | (((x % g.L1) & 1) << 1); }
@@ -76,41 +79,47 @@ __device__ __forceinline__ int active_index(PGeo const &g, int x)
 	return ((x / g.L1) >> 1) * (g.L1 >> 1) + ((x % g.L1) >> 1);
 }
 
-// the context value of plaquette y: its angle, 0 for the last
-__device__ __forceinline__ float ctx_angle(PGeo const &g, const float *phi, int y) { return y == g.V - 1 ? 0.f : phi[y]; }
-
-// context loop k of the active plaquette x: k < 8 the ring plaquette k, k >= 8
-// the 1x2 loop of ring plaquettes k - 8 and k - 7 (mod 8)
-__device__ __forceinline__ float loop_angle(PGeo const &g, const float *phi, int x0, int x1, int k)
+// (cos, sin) of context loop k of the plaquette at (x0, x1): k < 8 ring
+// plaquette k, k >= 8 the 1x2 loop of ring plaquettes k - 8 and k - 7 (mod 8)
+// by angle addition.  cs holds (cos, sin) of every plaquette, (1, 0) for the
+// last, which enters the context as the angle 0.
+__device__ __forceinline__ float2 loop_cs(PGeo const &g, const float2 *cs, int x0, int x1, int k)
 {
 	int d0, d1;
 	ring(k & 7, d0, d1);
-	float a = ctx_angle(g, phi, sitew(g, x0 + d0, x1 + d1));
-	if (k >= 8)
-	{
-		ring((k + 1) & 7, d0, d1);
-		a += ctx_angle(g, phi, sitew(g, x0 + d0, x1 + d1));
-	}
-	return a;
+	const float2 a = cs[sitew(g, x0 + d0, x1 + d1)];
+	if (k < 8)
+		return a;
+	ring((k + 1) & 7, d0, d1);
+	const float2 b = cs[sitew(g, x0 + d0, x1 + d1)];
+	return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.y, b.x, a.x * b.y));
 }
 
-// features of the 16 active plaquettes r0..r0+15 of class m: column 2k, 2k+1 =
-// (cos, sin) of context loop k (rows beyond NA are zero)
-__device__ void features(PGeo const &g, const float *phi, int m, int r0, int NA, FM::Tile<NI> &T)
+// features of the 16 active plaquettes r0..r0+15 of class m (rows beyond NA
+// zero): columns 2k, 2k+1 = (cos, sin) of context loop k; lane l forms loop
+// l % 16 of rows l / 16 + 4q
+__device__ void features(PGeo const &g, const float2 *cs, int m, int r0, int NA, FM::Tile<NI> &T)
 {
-	const int l = threadIdx.x & 63;
-	for (int e = l; e < 16 * NI; e += 64)
+	const int l = threadIdx.x & 63, k = l & 15;
+	for (int q = 0; q < 4; ++q)
 	{
-		const int row = e / NI, col = e - row * NI, r = r0 + row;
-		float v = 0.f;
+		const int row = (l >> 4) + 4 * q, r = r0 + row;
+		float2 v = make_float2(0.f, 0.f);
 		if (r < NA)
 		{
-			const int x = active_site(g, m, r), x0 = x / g.L1, x1 = x - x0 * g.L1;
-			const float a = loop_angle(g, phi, x0, x1, col >> 1);
-			v = (col & 1) ? sinf(a) : cosf(a);
+			int x0, x1;
+			active_xy(g, m, r, x0, x1);
+			v = loop_cs(g, cs, x0, x1, k);
 		}
-		T.x[row * NI + col] = v;
+		*reinterpret_cast<float2 *>(&T.x[row * NI + 2 * k]) = v;
 	}
+}
+
+__device__ __forceinline__ float2 sincos2(float a)
+{
+	float s, c;
+	sincosf(a, &s, &c);
+	return make_float2(c, s);
 }
 
 struct PCShared
@@ -120,22 +129,42 @@ struct PCShared
 	double red[4][4];
 };
 
-// MLP outputs of tiles it .. it+15 (four per wavefront) into Ys[tile][row*16+k]
-__device__ void mlp_round(PGeo const &g, const float *phi, int m, int NA, int it, int NT, PCShared &sh,
-                          float (*Ys)[256])
+// One layer on the workgroup: wavefront w takes tiles w, w + 4, ... of the
+// NA active plaquettes: features, MLP, then the circle map on quads (lane
+// 4i + k: row i of the tile, mixture component k).  Features read only
+// frozen plaquettes, so a wavefront may move its active ones while the others
+// still read theirs.  INV: apply g^-1 (the inverse pass).  Adds log g' to ld.
+template <bool INV>
+__device__ void layer_apply(PGeo const &g, float *phi, float2 *cs, int m, PCShared &sh, double &ld)
 {
-	const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+	const int w = threadIdx.x >> 6, l = threadIdx.x & 63, NA = n_active(g, m), NT = (NA + 15) / 16;
 	FM::Tile<NI> &T = sh.tile[w];
-	for (int q = 0; q < 4; ++q)
+	for (int tix = w; tix < NT; tix += 4)
 	{
-		const int tl = q * 4 + w, tix = it + tl;
-		if (tix >= NT)
-			break;
-		features(g, phi, m, tix * 16, NA, T);
+		features(g, cs, m, tix * 16, NA, T);
 		FM::wave_sync();
-		FM::mlp_forward<NI>(sh.lw, T);
-		for (int e = l; e < 256; e += 64)
-			Ys[tl][e] = T.y[e];
+		FM::mlp_forward<NI, FAST>(sh.lw, T);
+		const int row = l >> 2, r = tix * 16 + row;
+		if (r < NA)
+		{
+			int x0, x1;
+			active_xy(g, m, r, x0, x1);
+			const int x = x0 * g.L1 + x1;
+			const FM::NCPq p = FM::ncpq_params(&T.y[row * 16]);
+			const float a = FM::wrapf(phi[x] - p.t);
+			float v, Gp;
+			if (INV)
+				v = FM::ncpq_inverse(p, a, Gp);
+			else
+				FM::ncpq_eval(p, a, v, Gp);
+			if ((l & 3) == 0)
+			{
+				const float nx = FM::wrapf(v + p.t);
+				phi[x] = nx;
+				cs[x] = sincos2(nx);
+				ld += (double)logf(Gp);
+			}
+		}
 		FM::wave_sync();
 	}
 }
@@ -144,52 +173,39 @@ __device__ void mlp_round(PGeo const &g, const float *phi, int m, int NA, int it
 // every layer; phi_final (optional): the output; theta_out (optional): the
 // links.  out[2c + {0,1}] = {log q, S/beta}, S of the links when they are
 // formed, else of the plaquettes (the last one = -sum of the free ones).
-__global__ __launch_bounds__(THREADS) void k_pc_forward(PGeo g, int nl, const float *__restrict__ params,
-                                                        uint64_t seed, uint32_t step, float *theta_out,
-                                                        float *phi_layers, float *phi_final, double *out)
+// LDS: phi[V], cs[V] (then the links' scratch), al[V].
+__device__ void pc_forward(PGeo const &g, int nl, const float *__restrict__ params, uint64_t seed, uint32_t step,
+                           float *theta_out, float *phi_layers, float *phi_final, double *out, int c, float *dyn,
+                           PCShared &sh)
 {
-	extern __shared__ float dyn[];
-	float *phi = dyn;          // V plaquettes (the last unused)
-	float *al = dyn + g.V;     // V gauge angles
-	float *t = dyn + 2 * g.V;  // 2V link scratch
-	__shared__ PCShared sh;
-	__shared__ float Ys[16][256];
-	const int c = blockIdx.x;
+	float *phi = dyn;
+	float2 *cs = reinterpret_cast<float2 *>(dyn + g.V);
+	float *al = dyn + 3 * g.V;
 	const uint32_t cc = (uint32_t)c << 1;
+	Weights wr;
+	wr.load(params);
+	FM::zero_pad<NI>(sh.lw);
 	for (int x = threadIdx.x; x < g.V; x += THREADS)
 	{
-		phi[x] = x < g.V - 1 ? u1_plaq::uniform_angle(seed, step, (uint32_t)x, cc, TAG_PCFZ) : 0.f;
+		const float z = x < g.V - 1 ? u1_plaq::uniform_angle(seed, step, (uint32_t)x, cc, TAG_PCFZ) : 0.f;
+		phi[x] = z;
+		cs[x] = sincos2(z);
 		al[x] = u1_plaq::uniform_angle(seed, step, (uint32_t)x, cc | 1u, TAG_PCFZ);
 	}
 	double ld = 0.0;
 	for (int layer = 0; layer < nl; ++layer)
 	{
-		const int m = layer & 3, NA = n_active(g, m), NT = (NA + 15) / 16;
-		__syncthreads();
+		__syncthreads(); // the previous layer is done with the weights and the plaquettes
+		wr.store(sh.lw);
 		if (phi_layers)
 			for (int x = threadIdx.x; x < g.V; x += THREADS)
 				phi_layers[((long)c * nl + layer) * g.V + x] = phi[x];
-		FM::load_weights<NI>(sh.lw, params + (long)layer * NP_LAYER);
 		__syncthreads();
-		// the MLP inputs (frozen plaquettes) never change within a layer, so
-		// all MLPs of a round run first and then every thread moves one plaquette
-		for (int it = 0; it < NT; it += 16)
-		{
-			mlp_round(g, phi, m, NA, it, NT, sh, Ys);
-			__syncthreads();
-			const int r = it * 16 + threadIdx.x;
-			if (r < NA)
-			{
-				const int x = active_site(g, m, r);
-				FM::NCP p = FM::ncp_params(&Ys[threadIdx.x >> 4][(threadIdx.x & 15) * 16]);
-				float G, Gp;
-				FM::ncp_eval(p, FM::wrapf(phi[x] - p.t), G, Gp);
-				phi[x] = FM::wrapf(G + p.t);
-				ld += (double)logf(Gp);
-			}
-			__syncthreads();
-		}
+		if (layer + 1 < nl)
+			wr.load(params + (long)(layer + 1) * NP_LAYER);
+		layer_apply<false>(g, phi, cs, layer & 3, sh, ld);
 	}
+	__syncthreads();
 	if (phi_final)
 		for (int x = threadIdx.x; x < g.V; x += THREADS)
 			phi_final[(long)c * g.V + x] = phi[x];
@@ -206,7 +222,7 @@ __global__ __launch_bounds__(THREADS) void k_pc_forward(PGeo g, int nl, const fl
 		float *o = theta_out + (long)c * 2 * g.V;
 		const float h0 = u1_plaq::uniform_angle(seed, step, (uint32_t)g.V, cc, TAG_PCFZ);
 		const float h1 = u1_plaq::uniform_angle(seed, step, (uint32_t)g.V + 1u, cc, TAG_PCFZ);
-		u1_plaq::links_from_plaquettes(g, phi, h0, h1, al, t, o, THREADS);
+		u1_plaq::links_from_plaquettes(g, phi, h0, h1, al, reinterpret_cast<float *>(cs), o, THREADS);
 		__syncthreads();
 		double s[1] = {0.0};
 		for (int x = threadIdx.x; x < g.V; x += THREADS)
@@ -224,18 +240,18 @@ __global__ __launch_bounds__(THREADS) void k_pc_forward(PGeo g, int nl, const fl
 	}
 }
 
-// log q of given links: plaquettes, then the layers in reverse with g^-1
-// (flow_mlp::ncp_inverse).  out[2c + {0,1}] = {log q(U), S(U)/beta};
-// z_out (optional): the prior draws, V per configuration (the last 0).
-__global__ __launch_bounds__(THREADS) void k_pc_inverse(PGeo g, int nl, const float *__restrict__ params,
-                                                        const float *__restrict__ theta, float *z_out, double *out)
+// log q of given links: plaquettes, then the layers in reverse with g^-1.
+// out[2c + {0,1}] = {log q(U), S(U)/beta}; z_out (optional): the prior draws,
+// V per configuration (the last 0).  LDS: the links (then cs[V]), phi[V].
+__device__ void pc_inverse(PGeo const &g, int nl, const float *__restrict__ params, const float *__restrict__ theta,
+                           float *z_out, double *out, int c, float *dyn, PCShared &sh)
 {
-	extern __shared__ float dyn[];
-	float *th = dyn;          // 2V links
-	float *phi = dyn + 2 * g.V; // V plaquettes
-	__shared__ PCShared sh;
-	__shared__ float Ys[16][256];
-	const int c = blockIdx.x;
+	float *th = dyn;
+	float2 *cs = reinterpret_cast<float2 *>(dyn);
+	float *phi = dyn + 2 * g.V;
+	Weights wr;
+	wr.load(params + (long)(nl - 1) * NP_LAYER);
+	FM::zero_pad<NI>(sh.lw);
 	for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
 		th[i] = theta[(long)c * 2 * g.V + i];
 	__syncthreads();
@@ -247,29 +263,19 @@ __global__ __launch_bounds__(THREADS) void k_pc_inverse(PGeo g, int nl, const fl
 		v[1] += 1.0 - cos((double)tp);
 		phi[x] = x < g.V - 1 ? FM::wrapf(tp) : 0.f;
 	}
+	__syncthreads(); // the links are read; cs takes their place
+	for (int x = threadIdx.x; x < g.V; x += THREADS)
+		cs[x] = sincos2(phi[x]);
 	for (int layer = nl - 1; layer >= 0; --layer)
 	{
-		const int m = layer & 3, NA = n_active(g, m), NT = (NA + 15) / 16;
 		__syncthreads();
-		FM::load_weights<NI>(sh.lw, params + (long)layer * NP_LAYER);
+		wr.store(sh.lw);
 		__syncthreads();
-		for (int it = 0; it < NT; it += 16)
-		{
-			mlp_round(g, phi, m, NA, it, NT, sh, Ys);
-			__syncthreads();
-			const int r = it * 16 + threadIdx.x;
-			if (r < NA)
-			{
-				const int x = active_site(g, m, r);
-				FM::NCP p = FM::ncp_params(&Ys[threadIdx.x >> 4][(threadIdx.x & 15) * 16]);
-				float Gp;
-				const float u = FM::ncp_inverse(p, FM::wrapf(phi[x] - p.t), Gp);
-				phi[x] = FM::wrapf(u + p.t);
-				v[0] += (double)logf(Gp);
-			}
-			__syncthreads();
-		}
+		if (layer > 0)
+			wr.load(params + (long)(layer - 1) * NP_LAYER);
+		layer_apply<true>(g, phi, cs, layer & 3, sh, v[0]);
 	}
+	__syncthreads();
 	block_sum<2>(v, sh.red);
 	if (threadIdx.x == 0)
 	{
@@ -281,23 +287,59 @@ __global__ __launch_bounds__(THREADS) void k_pc_inverse(PGeo g, int nl, const fl
 			z_out[(long)c * g.V + x] = phi[x];
 }
 
+__global__ __launch_bounds__(THREADS) void k_pc_forward(PGeo g, int nl, const float *__restrict__ params,
+                                                        uint64_t seed, uint32_t step, float *theta_out,
+                                                        float *phi_layers, float *phi_final, double *out)
+{
+	extern __shared__ float dyn[];
+	__shared__ PCShared sh;
+	pc_forward(g, nl, params, seed, step, theta_out, phi_layers, phi_final, out, blockIdx.x, dyn, sh);
+}
+
+__global__ __launch_bounds__(THREADS) void k_pc_inverse(PGeo g, int nl, const float *__restrict__ params,
+                                                        const float *__restrict__ theta, float *z_out, double *out)
+{
+	extern __shared__ float dyn[];
+	__shared__ PCShared sh;
+	pc_inverse(g, nl, params, theta, z_out, out, blockIdx.x, dyn, sh);
+}
+
+// one MH proposal round in one launch: workgroups 0..batch-1 take log q of
+// the current configurations, batch..2 batch-1 draw the proposals, so that on
+// small lattices (two workgroups' LDS per CU) the two passes share the CUs
+__global__ __launch_bounds__(THREADS) void k_pc_propose(PGeo g, int nl, const float *__restrict__ params, int batch,
+                                                        const float *__restrict__ theta, double *state,
+                                                        uint64_t seed, uint32_t step, float *theta_prop,
+                                                        double *prop_out)
+{
+	extern __shared__ float dyn[];
+	__shared__ PCShared sh;
+	if ((int)blockIdx.x < batch)
+		pc_inverse(g, nl, params, theta, nullptr, state, blockIdx.x, dyn, sh);
+	else
+		pc_forward(g, nl, params, seed, step, theta_prop, nullptr, nullptr, prop_out, blockIdx.x - batch, dyn, sh);
+}
+
 // gradient of the per-configuration loss beta S - log det (S with the last
 // plaquette = -sum of the free ones) with respect to every weight:
 // partial[c][layer * NP_LAYER + i].  phi_layers / phi_final from k_pc_forward;
 // dxs: scratch of (V/4) * NI floats per configuration.
+// LDS: cs[V] (of the current layer's input), phi[V], a[V], gact[V/4].
 __global__ __launch_bounds__(THREADS) void k_pc_backward(PGeo g, int nl, const float *__restrict__ params,
                                                          const float *__restrict__ phi_layers,
                                                          const float *__restrict__ phi_final, float beta,
                                                          float *__restrict__ dxs, float *__restrict__ partial)
 {
 	extern __shared__ float dyn[];
-	float *phi = dyn;            // V: input of the current layer
-	float *a = dyn + g.V;        // V: adjoint w.r.t. the current layer's output
-	float *gact = dyn + 2 * g.V; // V/4: adjoint w.r.t. the active plaquettes' input
+	float2 *cs = reinterpret_cast<float2 *>(dyn);
+	float *phi = dyn + 2 * g.V;  // input of the current layer
+	float *a = dyn + 3 * g.V;    // adjoint w.r.t. the current layer's output
+	float *gact = dyn + 4 * g.V; // adjoint w.r.t. the active plaquettes' input
 	__shared__ PCShared sh;
 	const int c = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
 	float *dx = dxs + (long)c * (g.V >> 2) * NI;
 	float *part = partial + (long)c * nl * NP_LAYER;
+	FM::zero_pad<NI>(sh.lw);
 	for (int x = threadIdx.x; x < g.V; x += THREADS)
 		phi[x] = phi_final[(long)c * g.V + x];
 	double s[1] = {0.0};
@@ -312,40 +354,47 @@ __global__ __launch_bounds__(THREADS) void k_pc_backward(PGeo g, int nl, const f
 		const int m = layer & 3, NA = n_active(g, m), NT = (NA + 15) / 16;
 		__syncthreads();
 		for (int x = threadIdx.x; x < g.V; x += THREADS)
-			phi[x] = phi_layers[((long)c * nl + layer) * g.V + x];
-		FM::load_weights<NI>(sh.lw, params + (long)layer * NP_LAYER);
+		{
+			const float p = phi_layers[((long)c * nl + layer) * g.V + x];
+			phi[x] = p;
+			cs[x] = sincos2(p);
+		}
+		{
+			// no prefetch here: the gradient accumulators need the registers
+			Weights wr;
+			wr.load(params + (long)layer * NP_LAYER);
+			wr.store(sh.lw);
+		}
 		__syncthreads();
 		FM::Grad<NI> gr;
 		gr.zero();
-		for (int it = 0; it < (NT + 3) / 4; ++it)
+		FM::Tile<NI> &T = sh.tile[w];
+		for (int tix = w; tix < NT; tix += 4)
 		{
-			const int tix = it * 4 + w;
-			FM::Tile<NI> &T = sh.tile[w];
-			if (tix < NT)
+			features(g, cs, m, tix * 16, NA, T);
+			FM::wave_sync();
+			FM::mlp_forward<NI, FAST>(sh.lw, T);
+			if (l < 16)
 			{
-				features(g, phi, m, tix * 16, NA, T);
-				FM::wave_sync();
-				FM::mlp_forward<NI>(sh.lw, T);
-				if (l < 16)
+				float dy[16];
+				for (int k = 0; k < 16; ++k)
+					dy[k] = 0.f;
+				const int r = tix * 16 + l;
+				if (r < NA)
 				{
-					float dy[16];
-					for (int k = 0; k < 16; ++k)
-						dy[k] = 0.f;
-					const int r = tix * 16 + l;
-					if (r < NA)
-					{
-						const int x = active_site(g, m, r);
-						const float *y = T.y + l * 16;
-						gact[r] = FM::ncp_adjoint(y, FM::wrapf(phi[x] - FM::ncp_params(y).t), a[x], dy);
-					}
-					for (int k = 0; k < 16; ++k)
-						T.y[l * 16 + k] = dy[k];
+					int x0, x1;
+					active_xy(g, m, r, x0, x1);
+					const int x = x0 * g.L1 + x1;
+					const float *y = T.y + l * 16;
+					gact[r] = FM::ncp_adjoint(y, FM::wrapf(phi[x] - FM::ncp_params(y).t), a[x], dy);
 				}
-				FM::wave_sync();
-				FM::mlp_backward<NI>(sh.lw, T, gr, dx, tix * 16, NA);
+				for (int k = 0; k < 16; ++k)
+					T.y[l * 16 + k] = dy[k];
 			}
-			__syncthreads();
+			FM::wave_sync();
+			FM::mlp_backward<NI>(sh.lw, T, gr, dx, tix * 16, NA);
 		}
+		__syncthreads();
 		FM::store_grad<NI>(gr, part + (long)layer * NP_LAYER);
 		// adjoints w.r.t. the layer's input: the active plaquettes from gact,
 		// the frozen ones gain their features' adjoints
@@ -357,7 +406,7 @@ __global__ __launch_bounds__(THREADS) void k_pc_backward(PGeo g, int nl, const f
 				continue;
 			}
 			const int y0 = y / g.L1, y1 = y - y0 * g.L1;
-			const float cy = cosf(phi[y]), sy = sinf(phi[y]);
+			const float2 cy = cs[y];
 			float acc = 0.f;
 			for (int k = 0; k < 8; ++k)
 			{
@@ -368,14 +417,14 @@ __global__ __launch_bounds__(THREADS) void k_pc_backward(PGeo g, int nl, const f
 					continue;
 				const int x0 = x / g.L1, x1 = x - x0 * g.L1;
 				const float *dr = dx + (long)active_index(g, x) * NI;
-				acc = fmaf(-sy, dr[2 * k], acc);
-				acc = fmaf(cy, dr[2 * k + 1], acc);
+				acc = fmaf(-cy.y, dr[2 * k], acc);
+				acc = fmaf(cy.x, dr[2 * k + 1], acc);
 				// the two 1x2 loops that contain ring position k
 				for (int j : {(k + 7) & 7, k})
 				{
-					const float ps = loop_angle(g, phi, x0, x1, 8 + j);
-					acc = fmaf(-sinf(ps), dr[16 + 2 * j], acc);
-					acc = fmaf(cosf(ps), dr[16 + 2 * j + 1], acc);
+					const float2 ps = loop_cs(g, cs, x0, x1, 8 + j);
+					acc = fmaf(-ps.y, dr[16 + 2 * j], acc);
+					acc = fmaf(ps.x, dr[16 + 2 * j + 1], acc);
 				}
 			}
 			a[y] += acc;
@@ -402,7 +451,7 @@ __global__ void k_pc_init(int n, float *p, uint64_t seed)
 		FM::init_param<NI>(i, p, seed, TAG_PCFI);
 }
 
-int pc_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, PGeo *pg)
+int pc_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, PGeo *pg, const float *params = nullptr)
 {
 	if (layers < 1 || layers > MAX_LAYERS)
 		return mesh_fail(ctx, MESH_E_ARG, "the plaquette coupling flow has 1 to 64 layers");
@@ -413,6 +462,8 @@ int pc_setup(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, PGeo *pg)
 		return mesh_fail(ctx, MESH_E_SIZE, "the plaquette coupling flow holds a configuration in LDS: volume <= 4096");
 	if (batch < 1)
 		return mesh_fail(ctx, MESH_E_ARG, "batch must be >= 1");
+	if ((uintptr_t)params % 8)
+		return mesh_fail(ctx, MESH_E_ARG, "the flow's parameters must be 8-byte aligned");
 	pg->L0 = g->dims[0];
 	pg->L1 = g->dims[1];
 	pg->V = (int)V;
@@ -447,7 +498,7 @@ int mesh_pcflow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers,
 {
 	MESH_CHECK_CTX(ctx);
 	PGeo pg;
-	if (int rc = pc_setup(ctx, g, batch, layers, &pg))
+	if (int rc = pc_setup(ctx, g, batch, layers, &pg, params))
 		return rc;
 	if (!params || !theta_out || !out_dev)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to pcflow_sample");
@@ -465,7 +516,7 @@ int mesh_pcflow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, c
 {
 	MESH_CHECK_CTX(ctx);
 	PGeo pg;
-	if (int rc = pc_setup(ctx, g, batch, layers, &pg))
+	if (int rc = pc_setup(ctx, g, batch, layers, &pg, params))
 		return rc;
 	if (!params || !theta || !out_dev)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to pcflow_logq");
@@ -473,6 +524,27 @@ int mesh_pcflow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, c
 	if (int rc = set_lds(ctx, k_pc_inverse, lds))
 		return rc;
 	k_pc_inverse<<<batch, THREADS, lds, ctx->stream>>>(pg, layers, params, theta, z_out, out_dev);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_pcflow_propose(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params,
+                        const float *theta, double *state_dev, uint64_t seed, uint32_t step, float *theta_prop,
+                        double *prop_dev)
+{
+	MESH_CHECK_CTX(ctx);
+	PGeo pg;
+	if (int rc = pc_setup(ctx, g, batch, layers, &pg, params))
+		return rc;
+	if (!params || !theta || !state_dev || !theta_prop || !prop_dev)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to pcflow_propose");
+	if (theta == theta_prop)
+		return mesh_fail(ctx, MESH_E_ARG, "pcflow_propose: the proposals need their own buffer");
+	const size_t lds = 4 * (size_t)pg.V * sizeof(float);
+	if (int rc = set_lds(ctx, k_pc_propose, lds))
+		return rc;
+	k_pc_propose<<<2 * batch, THREADS, lds, ctx->stream>>>(pg, layers, params, batch, theta, state_dev, seed, step,
+	                                                       theta_prop, prop_dev);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }
@@ -494,7 +566,7 @@ int mesh_pcflow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, int layer
 {
 	MESH_CHECK_CTX(ctx);
 	PGeo pg;
-	if (int rc = pc_setup(ctx, g, batch, layers, &pg))
+	if (int rc = pc_setup(ctx, g, batch, layers, &pg, params))
 		return rc;
 	if (!params || !workspace || !grad || !out_dev)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to pcflow_gradient");
@@ -507,7 +579,7 @@ int mesh_pcflow_gradient(mesh_ctx *ctx, const mesh_grid *g, int batch, int layer
 		return rc;
 	k_pc_forward<<<batch, THREADS, lds_f, ctx->stream>>>(pg, layers, params, seed, step, nullptr, pl, pf, out_dev);
 	MESH_LAUNCHED(ctx);
-	const size_t lds_b = (2 * (size_t)pg.V + pg.V / 4) * sizeof(float);
+	const size_t lds_b = (4 * (size_t)pg.V + pg.V / 4) * sizeof(float);
 	if (int rc = set_lds(ctx, k_pc_backward, lds_b))
 		return rc;
 	k_pc_backward<<<batch, THREADS, lds_b, ctx->stream>>>(pg, layers, params, pl, pf, (float)beta, dxs, part);

@@ -196,6 +196,12 @@ class PlaqCouplingFlow(Flow):
                       C.c_uint32(step), tgt.ptr, out.ptr)
         return out
 
+    def propose(self, ens, state, seed, step, theta_out, out):
+        """logq(ens) into `state` and sample(...) into `theta_out` / `out`,
+        in one launch (mesh_pcflow_propose)."""
+        self.ctx.call("mesh_pcflow_propose", self.grid.ref, ens.batch, self.layers, self.params.ptr, ens.theta.ptr,
+                      state.ptr, C.c_uint64(seed), C.c_uint32(step), theta_out.ptr, out.ptr)
+
 
 class PlaqFlow:
     """Plaquette flow on an L0 x L1 lattice: V - 1 independent plaquettes
@@ -324,8 +330,11 @@ def run_chains(ens, beta, deltas, n_steps, seed, flow=None, n_therm=0, record_ev
         ctx.call("mesh_u1_hmc", ens.grid.ref, B, ens.theta.ptr, dptr, len(d), _d(beta), C.c_uint64(seed),
                  C.c_uint32(it), 1, hout.ptr.value + it * 4 * B * 8)
         if flow is not None:
-            flow.logq(ens, out=state)
-            flow.sample(ens, seed + 7919, it, theta_out=prop, out=pout)
+            if hasattr(flow, "propose"):
+                flow.propose(ens, state, seed + 7919, it, prop, pout)
+            else:
+                flow.logq(ens, out=state)
+                flow.sample(ens, seed + 7919, it, theta_out=prop, out=pout)
             ctx.call("mesh_flow_mh", ens.grid.ref, B, ens.theta.ptr, state.ptr, prop.ptr, pout.ptr,
                      _d(beta), C.c_uint64(seed), C.c_uint32(it), accb.ptr.value + it * B * 8)
         if it >= n_therm and (it - n_therm) % record_every == 0:

@@ -110,6 +110,32 @@ def test_pcflow_inverse_roundtrip(ctx):
     np.testing.assert_allclose(inv[:, 1], fwd[:, 1], rtol=1e-5)
 
 
+def test_pcflow_inverse_of_a_steep_map(ctx):
+    """a steeper map (the s_k biases spread over +-1; log det -22 .. -12 on
+    these draws): the inverse still recovers the prior draws and the
+    sampler's log q.  Steeper still, the composition itself becomes
+    ill-conditioned: with the last layers' weights doubled as well, the exact
+    float64 inverse of outputs moved by one FP32 ulp misses the draws by up
+    to 1.5e-2, so no FP32 inverse could pass this check there."""
+    L, B, seed, step = 8, 8, 109, 6
+    p = _params(110, 0.2)
+    rng = np.random.default_rng(111)
+    for layer in range(8):
+        o = (layer + 1) * U.PC_NP_LAYER - U.NO - U.NH * U.NO
+        p[o + U.NH * U.NO:o + U.NH * U.NO + U.NK] = rng.uniform(-1, 1, U.NK)
+    f = U1.PlaqCouplingFlow(L, L, ctx)
+    f.set_params(p.astype(np.float32))
+    ens = U1.U1Ensemble(L, L, B, ctx)
+    fwd = f.sample(ens, seed, step).download().reshape(B, 2)
+    z_out = U1.Buffer(B * L * L, np.float32, ctx)
+    inv = f.logq(ens, z_out=z_out).download().reshape(B, 2)
+    z = z_out.download().reshape(B, L * L)
+    assert np.abs(fwd[:, 0] - U.log_prior(L, L)).max() > 5  # a steep map
+    for c in range(B):
+        assert _wrapdiff(z[c, :-1], U.pcflow_draws(seed, step, c, L, L)[0]).max() < 2e-3, c
+    np.testing.assert_allclose(inv[:, 0], fwd[:, 0], atol=5e-3)
+
+
 @pytest.mark.parametrize("L0,L1", [(8, 8), (4, 12)])
 def test_pcflow_logq_of_given_configurations(ctx, L0, L1):
     """log q of uniform random links (not the flow's samples) against the
@@ -218,3 +244,27 @@ def test_pcflow_errors(ctx):
         U1.PlaqCouplingFlow(8, 8, ctx, layers=0)
     with pytest.raises(mesh_amd.MeshError):
         U1.PlaqCouplingFlow(8, 8, ctx, layers=65)
+
+
+def test_pcflow_propose_is_logq_and_sample(ctx):
+    """mesh_pcflow_propose = mesh_pcflow_logq of the current configurations
+    and mesh_pcflow_sample of the proposals, bit for bit, on a lattice where
+    the two halves share CUs (8x8) and on one where they do not (32x32)"""
+    for L in (8, 32):
+        B, seed, step = 16, 106, 3
+        f = U1.PlaqCouplingFlow(L, L, ctx)
+        f.set_params(_params(107, 0.2))
+        ens = U1.U1Ensemble(L, L, B, ctx)
+        ens.random_start(108)
+        st = U1.Buffer(2 * B, np.float64, ctx)
+        prop = U1.Buffer(B * 2 * L * L, np.float32, ctx)
+        pout = U1.Buffer(2 * B, np.float64, ctx)
+        f.propose(ens, st, seed, step, prop, pout)
+        ref_st = f.logq(ens).download()
+        ref_prop = U1.Buffer(B * 2 * L * L, np.float32, ctx)
+        ref_out = f.sample(ens, seed, step, theta_out=ref_prop).download()
+        np.testing.assert_array_equal(st.download(), ref_st)
+        np.testing.assert_array_equal(pout.download(), ref_out)
+        np.testing.assert_array_equal(prop.download(), ref_prop.download())
+        with pytest.raises(mesh_amd.MeshError):
+            f.propose(ens, st, seed, step, ens.theta, pout)
