@@ -268,3 +268,32 @@ def test_pcflow_propose_is_logq_and_sample(ctx):
         np.testing.assert_array_equal(prop.download(), ref_prop.download())
         with pytest.raises(mesh_amd.MeshError):
             f.propose(ens, st, seed, step, ens.theta, pout)
+
+
+def test_pcflow_gain_over_hmc_L16(ctx):
+    """At 16^2, beta 4: HMC + MH steps proposed by the trained 8-layer
+    plaquette coupling flow decorrelate Q within a few steps where plain HMC
+    needs tens of trajectories, with the exact <cos theta_P> and chi_t."""
+    L, beta, B = 16, 4.0, 256
+    d = gauge.makeDeltas("2lf", 0.5, 20)
+    f = U1.PlaqCouplingFlow(L, L, ctx, seed=112)
+    hist = f.train(beta, 4000, batch=B, lr=1e-3, lr_final=1e-4, seed=113)
+    assert hist[-50:].mean() > -_log_z(beta, L * L) - 0.5
+    runs = {}
+    for name, flow, n in (("hmc", None, 2000), ("flow", f, 500)):
+        e = U1.U1Ensemble(L, L, B, ctx)
+        e.random_start(114)
+        runs[name] = U1.run_chains(e, beta, d, n, 115, flow=flow, n_therm=300)
+    t_hmc, _ = U1.tau_int(runs["hmc"]["Q"])
+    t_flow, _ = U1.tau_int(runs["flow"]["Q"])
+    r = runs["flow"]
+    print(f"16^2 beta 4: tau_int(Q) HMC {t_hmc:.1f}, HMC + pcflow {t_flow:.2f}, flow acceptance {r['acc_flow']:.3f}")
+    assert r["acc_flow"] > 0.15, r["acc_flow"]
+    assert t_flow < t_hmc / 8, (t_hmc, t_flow)
+    chain_means = r["plaq"].mean(axis=1)
+    est, err = chain_means.mean(), chain_means.std(ddof=1) / math.sqrt(B)
+    exact = U.exact_mean_cos(beta, L * L)
+    assert abs(est - exact) < 5 * err + 1e-4, (est, exact, err)
+    chi = float(np.mean(r["Q"] ** 2)) / (L * L)
+    chi_exact = U.exact_chi_top(beta, L * L)
+    assert abs(chi - chi_exact) < 0.1 * chi_exact, (chi, chi_exact)
