@@ -19,6 +19,10 @@
 
 using namespace mesh_dev;
 
+// FP64 row-staged staples software-pipelined (staple_rows_pipe; same bits)
+#ifndef MESH_PIPE_F64
+#define MESH_PIPE_F64 1
+#endif
 // waves per SIMD the force kernels are compiled for (0: the compiler's choice)
 #ifndef MESH_FORCE_WAVES
 #define MESH_FORCE_WAVES 0
@@ -33,9 +37,12 @@ using namespace mesh_dev;
 #define MESH_FORCE_WAVES_F32 0
 #endif
 // ... and for the FP64 kernel with the row-staged staple path (0: as above;
-// 3 fits in 168 VGPRs but measured slower than the compiler's two waves)
+// 3 fits in 168 VGPRs but measured slower than two).  The pipelined staples
+// would take 260 registers and one wave; 2 holds them at two waves with three
+// VGPRs spilled, 1.1% faster than the unpipelined kernel at two waves and 16%
+// faster than the pipelined one at one wave (32^4, one box)
 #ifndef MESH_FORCE_WAVES_ROWS
-#define MESH_FORCE_WAVES_ROWS 0
+#define MESH_FORCE_WAVES_ROWS (MESH_PIPE_F64 ? 2 : 0)
 #endif
 // upper bound of waves per SIMD for the FP64 kernel with the row-staged staple
 // path (8: none): it runs at two either way (170 VGPRs), and with the bound the
@@ -343,6 +350,60 @@ __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict
 			madd_nn(S, T2, nu3 ? own_at(nu, sm) : load(U, nu, g, xmn));
 		}
 	}
+	return S;
+}
+
+// staple_4d_own for whole x3-rows in the block (ROWS) with shared upper
+// factors, software-pipelined: every product's phase first issues the global
+// loads the next phase needs, and __builtin_amdgcn_sched_barrier keeps them
+// ahead of the arithmetic, so that a wavefront multiplies while its next links
+// are in flight (the compiler's own schedule waits on one link's loads at a
+// time).  The operations and their order are those of staple_4d_own, so the
+// bits are the same; at most five links are live.
+template <int MU, int SH, class R>
+__device__ __forceinline__ m3<R> staple_rows_pipe(Geo const &g, const R *__restrict__ U, long x, int const *c,
+                                                  vec2_t<R> (*own)[9][64], int lane, vec2_t<R> (*sh)[9][64])
+{
+	int sp, sm;
+	x3_slots(g, x, c, sp, sm);
+	const long xpm = shift_c(g, x, MU, c[MU], 1);
+	m3<R> S = zero_m3<R>(), A, B, La, Lb, Lc;
+	// operands of the upper staple's first product (planes not shared)
+	auto fetch_upper = [&](auto NU) {
+		constexpr int nu = decltype(NU)::value;
+		if constexpr (!plane_shared(SH, nu < MU ? nu : MU))
+		{
+			A = MU == 3 ? lds_m3<R>(own, nu, sp) : load(U, nu, g, xpm);
+			B = nu == 3 ? lds_m3<R>(own, MU, sp) : load(U, MU, g, shift_c(g, x, nu, c[nu], 1));
+		}
+	};
+	// the upper and lower staples of plane (MU, nu); NEXT: the next nu (-1: none)
+	auto step = [&](auto NU, auto NEXT) {
+		constexpr int nu = decltype(NU)::value, nx = decltype(NEXT)::value;
+		const long xmn = nu == 3 ? 0 : shift_c(g, x, nu, c[nu], -1);
+		La = load(U, nu, g, shift_c(g, xpm, nu, c[nu], -1));
+		__builtin_amdgcn_sched_barrier(0);
+		m3<R> T1;
+		if constexpr (plane_shared(SH, nu < MU ? nu : MU))
+			T1 = nu < MU ? adj_m3(lds_m3<R>(sh, plane_index(nu, MU), lane)) : lds_m3<R>(sh, plane_index(MU, nu), lane);
+		else
+			T1 = upper_factor(A, B, nu < MU);
+		Lb = nu == 3 ? lds_m3<R>(own, MU, sm) : load(U, MU, g, xmn);
+		__builtin_amdgcn_sched_barrier(0);
+		madd_nd(S, T1, lds_m3<R>(own, nu, lane));
+		Lc = nu == 3 ? lds_m3<R>(own, nu, sm) : load(U, nu, g, xmn);
+		__builtin_amdgcn_sched_barrier(0);
+		m3<R> T2 = mul_dd(La, Lb);
+		if constexpr (nx >= 0)
+			fetch_upper(std::integral_constant<int, nx>{});
+		__builtin_amdgcn_sched_barrier(0);
+		madd_nn(S, T2, Lc);
+	};
+	constexpr int n0 = MU == 0 ? 1 : 0, n1 = MU <= 1 ? 2 : 1, n2 = MU <= 2 ? 3 : 2;
+	fetch_upper(std::integral_constant<int, n0>{});
+	step(std::integral_constant<int, n0>{}, std::integral_constant<int, n1>{});
+	step(std::integral_constant<int, n1>{}, std::integral_constant<int, n2>{});
+	step(std::integral_constant<int, n2>{}, std::integral_constant<int, -1>{});
 	return S;
 }
 
@@ -680,19 +741,24 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 				break;
 			}
 			__syncthreads();
+			constexpr bool PIPE = RW && MESH_PIPE_F64 && std::is_same<R, double>::value;
 			switch (mu)
 			{
 			case 0:
-				S = staple_4d_own<0, RW, R, SH>(g, U, x, c, own, lane, sh);
+				S = PIPE ? staple_rows_pipe<0, SH>(g, U, x, c, own, lane, sh)
+				         : staple_4d_own<0, RW, R, SH>(g, U, x, c, own, lane, sh);
 				break;
 			case 1:
-				S = staple_4d_own<1, RW, R, SH>(g, U, x, c, own, lane, sh);
+				S = PIPE ? staple_rows_pipe<1, SH>(g, U, x, c, own, lane, sh)
+				         : staple_4d_own<1, RW, R, SH>(g, U, x, c, own, lane, sh);
 				break;
 			case 2:
-				S = staple_4d_own<2, RW, R, SH>(g, U, x, c, own, lane, sh);
+				S = PIPE ? staple_rows_pipe<2, SH>(g, U, x, c, own, lane, sh)
+				         : staple_4d_own<2, RW, R, SH>(g, U, x, c, own, lane, sh);
 				break;
 			default:
-				S = staple_4d_own<3, RW, R, SH>(g, U, x, c, own, lane, sh);
+				S = PIPE ? staple_rows_pipe<3, SH>(g, U, x, c, own, lane, sh)
+				         : staple_4d_own<3, RW, R, SH>(g, U, x, c, own, lane, sh);
 				break;
 			}
 		}
