@@ -154,8 +154,9 @@ __device__ __forceinline__ int logical_block(int nb, int remap)
 //      of the lattice (4 | L0, 2 | L1); also order 3's fallback where 8 does
 //      not divide L0.
 // Returns the memory-order block index (-1: padding workgroup).  The order
-// only permutes workgroups; every site's arithmetic is unchanged.
-__device__ __forceinline__ int ordered_block_of(Geo const &g, int nb, int sbk, int arg, int b)
+// only permutes workgroups; every site's arithmetic is unchanged.  GEO: any
+// geometry with nd and L[4] (the SU(3) and SU(2) ones).
+template <class GEO> __device__ __forceinline__ int ordered_block_of(GEO const &g, int nb, int sbk, int arg, int b)
 {
 	const int order = arg & 15, T = arg >> 4;
 	if (order == 0)

@@ -37,6 +37,13 @@
 #include <cstdint>
 #include <initializer_list>
 
+// fused momentum + link step with the block's links staged in LDS
+// (k_su2_force_link_rows) where the geometry allows it; 0: always the
+// thread-per-link kernel
+#ifndef MESH_SU2_ROWS
+#define MESH_SU2_ROWS 1
+#endif
+
 using namespace mesh_dev;
 
 namespace {
@@ -257,6 +264,61 @@ __global__ __launch_bounds__(256) void k_su2_force_link(G2 g, const R *__restric
 	long x = i - (long)mu * g.V;
 	qt<R> u = ldq(U, g, mu, x);
 	qt<double> M = q64(qmul(u, staple(g, U, mu, x)));
+	qt<double> p = q64(ldq(P, g, mu, x));
+	qt<R> pn = qround<R>({p.w, fma(-k, M.x, p.x), fma(-k, M.y, p.y), fma(-k, M.z, p.z)});
+	stq(P, g, mu, x, pn);
+	qt<R> D = q_expm1(pn.x * t, pn.y * t, pn.z * t);
+	qt<R> un = qadd(u, qmul(D, u));
+	if (reunit)
+		un = q_reunit(un);
+	stq(Unext, g, mu, x, un);
+}
+
+// k_su2_force_link for 4-d lattices whose x3-rows tile the 64-site blocks
+// (V % 64 == 0, 64 % L3 == 0): a workgroup takes one block's 64 sites in
+// their four directions (wavefront mu = direction mu), stages the block's
+// 4 x 64 links in LDS, and reads there the link itself and the staple links
+// at x and at its x3-neighbours (7 of 18); the others come from global memory.
+// Workgroups follow the XCD-aware block order of the SU(3) stencil kernels
+// (ordered_block_of).  The arithmetic is that of k_su2_force_link (staple():
+// the same products in the same order), so the bits are the same.
+template <class R>
+__global__ __launch_bounds__(256) void k_su2_force_link_rows(G2 g, const R *__restrict__ U, R *__restrict__ P,
+                                                             double k, R *__restrict__ Unext, R t, int reunit, int nb,
+                                                             int order)
+{
+	__shared__ v4<R> own[4][64];
+	const int lb = ordered_block_of(g, nb, 64, order, blockIdx.x);
+	if (lb < 0)
+		return;
+	const int mu = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const long x = (long)lb * 64 + lane;
+	own[mu][lane] = *reinterpret_cast<const v4<R> *>(U + 4 * ((long)mu * g.stride + x));
+	__syncthreads();
+	auto O = [&](int d, int s) {
+		const v4<R> v = own[d][s];
+		return qt<R>{v.x, v.y, v.z, v.w};
+	};
+	// slots of the x3-neighbours (the block starts a row: 64 % L3 == 0)
+	const int L3 = g.L[3], c3 = lane % L3, o0 = lane - c3;
+	const int sp = o0 + (c3 + 1 == L3 ? 0 : c3 + 1), sm = o0 + (c3 == 0 ? L3 - 1 : c3 - 1);
+	qt<R> S = {R(0), R(0), R(0), R(0)};
+	const long xpm = nbr(g, x, mu, 1);
+	for (int nu = 0; nu < 4; ++nu)
+	{
+		if (nu == mu)
+			continue;
+		const long xpn = nbr(g, x, nu, 1), xmn = nbr(g, x, nu, -1), xpmmn = nbr(g, xpm, nu, -1);
+		const qt<R> a = mu == 3 ? O(nu, sp) : ldq(U, g, nu, xpm); // U_nu(x+mu)
+		const qt<R> b = nu == 3 ? O(mu, sp) : ldq(U, g, mu, xpn); // U_mu(x+nu)
+		qt<R> up = qmul(qmul(a, qconj(b)), qconj(O(nu, lane)));
+		const qt<R> d = nu == 3 ? O(mu, sm) : ldq(U, g, mu, xmn); // U_mu(x-nu)
+		const qt<R> e = nu == 3 ? O(nu, sm) : ldq(U, g, nu, xmn); // U_nu(x-nu)
+		qt<R> lo = qmul(qmul(qconj(ldq(U, g, nu, xpmmn)), qconj(d)), e);
+		S = qadd(qadd(S, up), lo);
+	}
+	const qt<R> u = O(mu, lane);
+	qt<double> M = q64(qmul(u, S));
 	qt<double> p = q64(ldq(P, g, mu, x));
 	qt<R> pn = qround<R>({p.w, fma(-k, M.x, p.x), fma(-k, M.y, p.y), fma(-k, M.z, p.z)});
 	stq(P, g, mu, x, pn);
@@ -579,10 +641,18 @@ int launch_force_link(mesh_ctx *ctx, G2 const &g, int prec, const void *U, void 
 {
 	const double k = delta * (beta / 4.0);
 	int ph = mesh_prof_begin(ctx, PROF_FORCE_LINK);
+	const bool rows = MESH_SU2_ROWS && g.nd == 4 && g.V % 64 == 0 && 64 % g.L[3] == 0;
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
-		k_su2_force_link<R><<<nblk(g.V * g.nd), 256, 0, ctx->stream>>>(g, as<R>(U), as<R>(P), k, as<R>(Unext),
-		                                                               (R)t, reunit);
+		if (rows)
+		{
+			const int nb = (int)(g.V / 64);
+			k_su2_force_link_rows<R><<<mesh_grid_blocks(ctx, nb), 256, 0, ctx->stream>>>(
+			    g, as<R>(U), as<R>(P), k, as<R>(Unext), (R)t, reunit, nb, mesh_order(ctx));
+		}
+		else
+			k_su2_force_link<R><<<nblk(g.V * g.nd), 256, 0, ctx->stream>>>(g, as<R>(U), as<R>(P), k,
+			                                                               as<R>(Unext), (R)t, reunit);
 	});
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
