@@ -518,9 +518,21 @@ template <class R> __device__ __forceinline__ qt<R> or_step(qt<R> K, R omega)
 // overrelaxation of the sites of one parity (sum of coordinates mod 2) on
 // the transformed links U, with h = G h when h is given; the t-th site of the
 // parity lies in row t / (L_f / 2) of the fastest dimension f
-template <class R> __global__ __launch_bounds__(256) void k_su2_or_half(G2 g, R *U, R *h, int parity, R omega)
+// Workgroups of 512 consecutive sites (256 of the parity) follow the
+// XCD-aware block order of the stencil kernels when V % 512 == 0 (nb > 0);
+// each thread reads its site's 8 links once (in this half-sweep no other
+// thread touches them: a link joins one site of each parity).
+template <class R>
+__global__ __launch_bounds__(256) void k_su2_or_half(G2 g, R *U, R *h, int parity, R omega, int nb, int order)
 {
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (nb > 0)
+	{
+		const int lb = ordered_block_of(g, nb, 512, order, blockIdx.x);
+		if (lb < 0)
+			return;
+		t = (long)lb * 256 + threadIdx.x;
+	}
 	if (t >= (g.V >> 1))
 		return;
 	const int f = g.nd - 1, hl = g.L[f] >> 1;
@@ -530,16 +542,26 @@ template <class R> __global__ __launch_bounds__(256) void k_su2_or_half(G2 g, R 
 	for (int mu = 0; mu < f; ++mu)
 		ps += (int)((x0 / g.st[mu]) % g.L[mu]);
 	const long x = x0 + 2 * (t - row * hl) + ((parity + ps) & 1);
+	qt<R> fw[4], bw[4];
+	long xm[4];
 	qt<R> K = {R(0), R(0), R(0), R(0)};
-	for (int mu = 0; mu < g.nd; ++mu)
-		K = qadd(qadd(K, ldq(U, g, mu, x)), qconj(ldq(U, g, mu, nbr(g, x, mu, -1))));
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu) // unrolled: the arrays stay in registers
+		if (mu < g.nd)
+		{
+			xm[mu] = nbr(g, x, mu, -1);
+			fw[mu] = ldq(U, g, mu, x);
+			bw[mu] = ldq(U, g, mu, xm[mu]);
+			K = qadd(qadd(K, fw[mu]), qconj(bw[mu]));
+		}
 	qt<R> G = or_step(K, omega);
-	for (int mu = 0; mu < g.nd; ++mu)
-	{
-		const long xm = nbr(g, x, mu, -1);
-		stq(U, g, mu, x, qmul(G, ldq(U, g, mu, x)));
-		stq(U, g, mu, xm, qmul(ldq(U, g, mu, xm), qconj(G)));
-	}
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
+		if (mu < g.nd)
+		{
+			stq(U, g, mu, x, qmul(G, fw[mu]));
+			stq(U, g, mu, xm[mu], qmul(bw[mu], qconj(G)));
+		}
 	if (h)
 		stq(h, g, 0, x, qmul(G, ldq(h, g, 0, x)));
 }
@@ -1096,7 +1118,9 @@ int mesh_su2_landau_or_halfsweep(mesh_ctx *ctx, const mesh_grid *g, int prec, vo
 	int ph = mesh_prof_begin(ctx, PROF_LANDAU_OR);
 	by_prec(prec, [&](auto tag) {
 		using R = decltype(tag);
-		k_su2_or_half<R><<<nblk(geo.V / 2), 256, 0, ctx->stream>>>(geo, as<R>(U), as<R>(h), parity, (R)omega);
+		const int nb = geo.V % 512 == 0 ? (int)(geo.V / 512) : 0;
+		k_su2_or_half<R><<<nb > 0 ? mesh_grid_blocks(ctx, nb) : nblk(geo.V / 2), 256, 0, ctx->stream>>>(
+		    geo, as<R>(U), as<R>(h), parity, (R)omega, nb, mesh_order(ctx));
 	});
 	mesh_prof_end(ctx, ph);
 	MESH_LAUNCHED(ctx);
