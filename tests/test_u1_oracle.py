@@ -390,3 +390,31 @@ def test_u1_nd_force_finite_difference_and_integrator():
     t1, q1, _ = U.nd_hmc_trajectory(th, q, beta, d)
     t2, _, _ = U.nd_hmc_trajectory(t1, -q1, beta, d)
     assert np.abs(U.wrap(t2 - th)).max() < 1e-12
+
+
+def test_factorized_plaquette_proposal_acceptance_ceiling():
+    """An independence sampler that proposes the V - 1 free plaquettes i.i.d.
+    from the exact one-plaquette law e^{beta cos}/(2 pi I0(beta)) -- the best a
+    proposal can do that does not see the constraint on their sum -- has
+    log-weights beta cos(P_last) + const, and accepts at
+    E_{u ~ vM(beta), v ~ U(-pi, pi)} min(1, e^{beta (cos v - cos u)})
+    (0.426 at beta 2, 0.272 at beta 4): the ceiling of the plaquette flows.
+    The integral against an MH simulation with this proposal on an 8x8 torus."""
+    from scipy import integrate
+
+    rng = np.random.default_rng(7)
+    V, n = 64, 40000
+    for beta in (2.0, 4.0):
+        z = 2 * np.pi * sp.i0(beta)
+        ceil, _ = integrate.dblquad(
+            lambda v, u: np.exp(beta * np.cos(u)) / z / (2 * np.pi) * min(1.0, np.exp(beta * (np.cos(v) - np.cos(u)))),
+            -np.pi, np.pi, -np.pi, np.pi, epsabs=1e-7)
+        lw = beta * np.cos(-rng.vonmises(0.0, beta, size=(n, V - 1)).sum(axis=1))
+        logu = np.log(rng.uniform(size=n))
+        acc, cur = 0, lw[0]
+        for i in range(1, n):
+            if logu[i] < lw[i] - cur:
+                acc += 1
+                cur = lw[i]
+        assert abs(acc / (n - 1) - ceil) < 0.012, (beta, acc / (n - 1), ceil)
+    assert abs(ceil - 0.272) < 1e-3
