@@ -518,7 +518,7 @@ int mesh_pcflow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, c
 /* one MH proposal round in one launch: state_dev[2c + {0,1}] = {log q, S/beta}
  * of the current configurations theta (as mesh_pcflow_logq) and proposals
  * theta_prop with prop_dev (as mesh_pcflow_sample); theta_prop must not alias
- * theta.  The two halves of the launch run side by side where two workgroups
+ * theta, nor prop_dev state_dev.  The two halves of the launch run side by side where two workgroups
  * fit a CU (V <= 256). */
 int mesh_pcflow_propose(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers, const float *params,
                         const float *theta, double *state_dev, uint64_t seed, uint32_t step, float *theta_prop,

@@ -538,8 +538,8 @@ int mesh_pcflow_propose(mesh_ctx *ctx, const mesh_grid *g, int batch, int layers
 		return rc;
 	if (!params || !theta || !state_dev || !theta_prop || !prop_dev)
 		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to pcflow_propose");
-	if (theta == theta_prop)
-		return mesh_fail(ctx, MESH_E_ARG, "pcflow_propose: the proposals need their own buffer");
+	if (theta == theta_prop || state_dev == prop_dev)
+		return mesh_fail(ctx, MESH_E_ARG, "pcflow_propose: the proposals and their log q need their own buffers");
 	const size_t lds = 4 * (size_t)pg.V * sizeof(float);
 	if (int rc = set_lds(ctx, k_pc_propose, lds))
 		return rc;

@@ -268,6 +268,8 @@ def test_pcflow_propose_is_logq_and_sample(ctx):
         np.testing.assert_array_equal(prop.download(), ref_prop.download())
         with pytest.raises(mesh_amd.MeshError):
             f.propose(ens, st, seed, step, ens.theta, pout)
+        with pytest.raises(mesh_amd.MeshError):
+            f.propose(ens, st, seed, step, prop, st)
 
 
 def test_pcflow_gain_over_hmc_L16(ctx):
