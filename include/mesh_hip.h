@@ -498,12 +498,14 @@ int mesh_pflow_adam(mesh_ctx *ctx, float *params, const float *grad, float *m, f
  * transformation).  Layer l moves the plaquettes of parity class l % 4,
  * (x0 mod 2, x1 mod 2) = (m % 2, m / 2), by h(a) = g(wrap(a - t)) + t, g a
  * mixture of 4 non-compact projections whose parameters an MLP
- * 32 -> 64 -> 64 -> 10 (tanh) computes from (cos, sin) of the 8 surrounding
- * plaquettes and of the 8 1x2 loops formed by neighbouring pairs of them (the
- * last plaquette enters as 0).  Exact log-density:
+ * 40 -> 64 -> 64 -> 10 (tanh) computes from (cos, sin) of the 8 surrounding
+ * plaquettes, of the 8 1x2 loops formed by neighbouring pairs of them and of
+ * the 4 L-shaped loops that the 2x2 squares of the 3x3 neighbourhood leave
+ * without their (active) centre (the last plaquette enters as 0).  Exact
+ * log-density:
  *   log q(U) = -sum_{layers, active} log g' - 2V log 2 pi.
  * Even extents >= 4, V <= 4096, 1 <= layers <= 64; weights as mesh_flow_*
- * with 32 input features (mesh_pcflow_num_params(layers) floats, device). */
+ * with 40 input features (mesh_pcflow_num_params(layers) floats, device). */
 int mesh_pcflow_num_params(int layers);
 /* initial weights (Philox(seed)): the identity map plus 1e-2 noise */
 int mesh_pcflow_init(mesh_ctx *ctx, int layers, float *params, uint64_t seed);

@@ -144,7 +144,7 @@ __device__ void mlp_round(FGeo const &g, const float *th, int mu, int a, int it,
 		wave_sync();
 		mlp_forward<NI>(sh.lw, T);
 		for (int e = l; e < 256; e += 64)
-			Ys[tl][e] = T.y[e];
+			Ys[tl][e] = T.y()[e];
 		wave_sync();
 	}
 }
@@ -352,13 +352,13 @@ __global__ __launch_bounds__(THREADS) void k_flow_backward(FGeo g, int nl, const
 					{
 						int y0, y1;
 						active_corner(g, mu, a, r, y0, y1);
-						const float *y = T.y + l * 16;
+						const float *y = T.y() + l * 16;
 						float tp = plaqf(g, th, y0, y1);
 						float A = ab[updated_link(g, mu, y0, y1)];
 						gact[r] = ncp_adjoint(y, wrapf(tp - ncp_params(y).t), A, dy) - A;
 					}
 					for (int k = 0; k < 16; ++k)
-						T.y[l * 16 + k] = dy[k];
+						T.y()[l * 16 + k] = dy[k];
 				}
 				wave_sync();
 				mlp_backward<NI>(sh.lw, T, gr, dx, tix * 16, g.NA);

@@ -30,8 +30,8 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <int NI> struct Layout
 {
-	static_assert(NI % 4 == 0 && NI <= 32, "features: a multiple of 4, at most 32");
-	static constexpr int XS = NI <= 16 ? 16 : 32; // feature row stride of a tile
+	static_assert(NI % 4 == 0 && NI <= 48, "features: a multiple of 4, at most 48");
+	static constexpr int XS = (NI + 15) / 16 * 16; // feature row stride of a tile
 	static constexpr int OW1 = 0, OB1 = OW1 + NI * NH, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
 	                     OB3 = OW3 + NH * NO, NP = OB3 + NO;
 };
@@ -90,10 +90,15 @@ template <int NI> __device__ void load_weights(LW<NI> &lw, const float *__restri
 		lw.b3[i] = i < NO ? p[Y::OB3 + i] : 0.f;
 }
 
-// per-wavefront tile scratch: features (row stride XS), two hidden layers, outputs
-template <int NI> struct Tile
+// per-wavefront tile scratch: features (row stride XS; the columns from NI on
+// must be finite), two hidden layers and the outputs y().  SEP_Y = false puts
+// the outputs over the features, which the first layer has consumed by then
+// (kernels without a backward pass: 1 KB less LDS per tile).
+template <int NI, bool SEP_Y = true> struct Tile
 {
-	float x[16 * Layout<NI>::XS], h1[16 * NH], h2[16 * NH], y[16 * 16];
+	float x[16 * Layout<NI>::XS], h1[16 * NH], h2[16 * NH];
+	float ys[SEP_Y ? 16 * 16 : 1];
+	__device__ float *y() { return SEP_Y ? ys : x; }
 };
 
 // One layer's parameters held in registers (two floats a lane per chunk) so
@@ -163,7 +168,7 @@ template <bool FAST> __device__ __forceinline__ float act(float x) { return FAST
 
 // MLP forward of one tile: h1 = tanh(x W1 + b1), h2 = tanh(h1 W2 + b2), y = h2 W3 + b3
 // (FAST: tanh_fast; a flow uses one choice in all of its kernels)
-template <int NI, bool FAST = false> __device__ void mlp_forward(LW<NI> const &lw, Tile<NI> &T)
+template <int NI, bool FAST = false, bool SY = true> __device__ void mlp_forward(LW<NI> const &lw, Tile<NI, SY> &T)
 {
 	constexpr int XS = Layout<NI>::XS;
 	int l = threadIdx.x & 63, c = l & 15, rq = (l >> 4) * 4;
@@ -183,7 +188,7 @@ template <int NI, bool FAST = false> __device__ void mlp_forward(LW<NI> const &l
 	wave_sync();
 	f4 d = tile_mm(T.h2, NH, 1, lw.w3, 16, 1, NH);
 	for (int j = 0; j < 4; ++j)
-		T.y[(rq + j) * 16 + c] = d[j] + lw.b3[c];
+		T.y()[(rq + j) * 16 + c] = d[j] + lw.b3[c];
 	wave_sync();
 }
 
@@ -425,7 +430,7 @@ template <int NI> struct Grad
 	}
 };
 
-// MLP backward of one tile whose T.y holds dL/dy (and T.x, T.h1, T.h2 the
+// MLP backward of one tile whose T.y() holds dL/dy (and T.x, T.h1, T.h2 the
 // forward values): accumulates the weight gradients into G and writes
 // dL/dx of rows r0 .. r0+15 (r < nrows) to dx[r * NI + feature]
 template <int NI>
@@ -435,14 +440,14 @@ __device__ void mlp_backward(LW<NI> const &lw, Tile<NI> &T, Grad<NI> &G, float *
 	const int l = threadIdx.x & 63, lc = l & 15, rq = (l >> 4) * 4;
 	// dW3 += H2^T dY, db3
 	for (int i0 = 0; i0 < NH; i0 += 16)
-		G.w3[i0 / 16] += tile_mm(T.h2 + i0, 1, NH, T.y, 16, 1, 16);
+		G.w3[i0 / 16] += tile_mm(T.h2 + i0, 1, NH, T.y(), 16, 1, 16);
 	if (l < 16)
 		for (int rr = 0; rr < 16; ++rr)
-			G.b3 += T.y[rr * 16 + l];
+			G.b3 += T.y()[rr * 16 + l];
 	// dA2 = (dY W3^T) * (1 - h2^2), into h2
 	f4 dh2[4];
 	for (int n0 = 0; n0 < NH; n0 += 16)
-		dh2[n0 / 16] = tile_mm(T.y, 16, 1, lw.w3 + n0 * 16, 1, 16, 16);
+		dh2[n0 / 16] = tile_mm(T.y(), 16, 1, lw.w3 + n0 * 16, 1, 16, 16);
 	wave_sync();
 	for (int n0 = 0; n0 < NH; n0 += 16)
 		for (int j = 0; j < 4; ++j)

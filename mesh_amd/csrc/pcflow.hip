@@ -12,11 +12,13 @@
 //    (x0 mod 2, x1 mod 2) = (m % 2, m / 2) are active (the last excluded),
 //    every other one is frozen.  Each active plaquette a moves to
 //    h(a) = g(wrap(a - t)) + t with the mixture g of flow_mlp.h, its
-//    parameters from an MLP 32 -> 64 -> 64 -> 10 of the frozen 3x3
-//    neighbourhood: (cos, sin) of the 8 surrounding plaquettes (1x1 loops)
-//    and of the 8 1x2 loops formed by neighbouring pairs of them (the loop
-//    angle is the sum of its two plaquettes); the last plaquette enters the
-//    context as 0 (it is not a variable of the flow);
+//    parameters from an MLP 40 -> 64 -> 64 -> 10 of the frozen 3x3
+//    neighbourhood: (cos, sin) of the 8 surrounding plaquettes (1x1 loops),
+//    of the 8 1x2 loops formed by neighbouring pairs of them and of the 4
+//    L-shaped loops that each 2x2 square of the neighbourhood leaves when its
+//    (active) centre is taken out (a loop's angle is the sum of its
+//    plaquettes); the last plaquette enters the context as 0 (it is not a
+//    variable of the flow);
 //  * log q(U) = -sum over layers and active plaquettes of log g'
 //    - (V - 1) log 2 pi (prior) - (V + 1) log 2 pi (holonomies, gauge).
 // Within a layer the active plaquettes see only frozen ones, so the map is
@@ -40,7 +42,8 @@ using u1_plaq::wrapd;
 
 namespace {
 
-constexpr int NI = 32; // features
+constexpr int NI = 40; // features: 20 loops, (cos, sin) each
+constexpr int XS = FM::Layout<NI>::XS;
 constexpr int NP_LAYER = FM::Layout<NI>::NP;
 constexpr int MAX_LAYERS = 64;
 constexpr int THREADS = 256;
@@ -80,39 +83,69 @@ __device__ __forceinline__ int active_index(PGeo const &g, int x)
 }
 
 // (cos, sin) of context loop k of the plaquette at (x0, x1): k < 8 ring
-// plaquette k, k >= 8 the 1x2 loop of ring plaquettes k - 8 and k - 7 (mod 8)
-// by angle addition.  cs holds (cos, sin) of every plaquette, (1, 0) for the
-// last, which enters the context as the angle 0.
-__device__ __forceinline__ float2 loop_cs(PGeo const &g, const float2 *cs, int x0, int x1, int k)
+// plaquette k; 8 <= k < 16 the 1x2 loop of ring plaquettes k - 8 and k - 7
+// (mod 8); 16 <= k < 20 the L-shaped loop of ring plaquettes 2j - 1, 2j and
+// 2j + 1 (mod 8), j = k - 16 (the 2x2 square at corner 2j without its
+// centre); by angle addition.  cs holds (cos, sin) of every plaquette, (1, 0)
+// for the last, which enters the context as the angle 0.
+__device__ __forceinline__ float2 ring_cs(PGeo const &g, const float2 *cs, int x0, int x1, int k)
 {
 	int d0, d1;
 	ring(k & 7, d0, d1);
-	const float2 a = cs[sitew(g, x0 + d0, x1 + d1)];
-	if (k < 8)
-		return a;
-	ring((k + 1) & 7, d0, d1);
-	const float2 b = cs[sitew(g, x0 + d0, x1 + d1)];
+	return cs[sitew(g, x0 + d0, x1 + d1)];
+}
+__device__ __forceinline__ float2 cs_add(float2 a, float2 b)
+{
 	return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.y, b.x, a.x * b.y));
+}
+__device__ __forceinline__ float2 loop_cs(PGeo const &g, const float2 *cs, int x0, int x1, int k)
+{
+	if (k < 8)
+		return ring_cs(g, cs, x0, x1, k);
+	if (k < 16)
+		return cs_add(ring_cs(g, cs, x0, x1, k - 8), ring_cs(g, cs, x0, x1, k - 7));
+	const int j = k - 16;
+	return cs_add(cs_add(ring_cs(g, cs, x0, x1, 2 * j + 7), ring_cs(g, cs, x0, x1, 2 * j)),
+	              ring_cs(g, cs, x0, x1, 2 * j + 1));
 }
 
 // features of the 16 active plaquettes r0..r0+15 of class m (rows beyond NA
-// zero): columns 2k, 2k+1 = (cos, sin) of context loop k; lane l forms loop
-// l % 16 of rows l / 16 + 4q
-__device__ void features(PGeo const &g, const float2 *cs, int m, int r0, int NA, FM::Tile<NI> &T)
+// zero): columns 2k, 2k+1 = (cos, sin) of context loop k, the row's columns
+// from NI on zero.  First the 8 ring plaquettes of every row (the only
+// gathers from the configuration), then the 1x2 and L-shaped loops from those
+// columns, by the angle additions of loop_cs in its order (the same bits).
+template <class TILE> __device__ void features(PGeo const &g, const float2 *cs, int m, int r0, int NA, TILE &T)
 {
-	const int l = threadIdx.x & 63, k = l & 15;
-	for (int q = 0; q < 4; ++q)
+	const int l = threadIdx.x & 63;
+	for (int e = l; e < 16 * 8; e += 64)
 	{
-		const int row = (l >> 4) + 4 * q, r = r0 + row;
+		const int row = e >> 3, k = e & 7, r = r0 + row;
 		float2 v = make_float2(0.f, 0.f);
 		if (r < NA)
 		{
 			int x0, x1;
 			active_xy(g, m, r, x0, x1);
-			v = loop_cs(g, cs, x0, x1, k);
+			v = ring_cs(g, cs, x0, x1, k);
 		}
-		*reinterpret_cast<float2 *>(&T.x[row * NI + 2 * k]) = v;
+		*reinterpret_cast<float2 *>(&T.x[row * XS + 2 * k]) = v;
 	}
+	FM::wave_sync();
+	for (int e = l; e < 16 * 12; e += 64)
+	{
+		const int row = e / 12, q = e - row * 12;
+		const float2 *rg = reinterpret_cast<const float2 *>(&T.x[row * XS]);
+		float2 v;
+		if (q < 8)
+			v = cs_add(rg[q], rg[(q + 1) & 7]);
+		else
+		{
+			const int j = q - 8;
+			v = cs_add(cs_add(rg[(2 * j + 7) & 7], rg[2 * j]), rg[2 * j + 1]);
+		}
+		*reinterpret_cast<float2 *>(&T.x[row * XS + 16 + 2 * q]) = v;
+	}
+	for (int e = l; e < 16 * (XS - NI); e += 64)
+		T.x[(e / (XS - NI)) * XS + NI + e % (XS - NI)] = 0.f;
 }
 
 __device__ __forceinline__ float2 sincos2(float a)
@@ -122,12 +155,16 @@ __device__ __forceinline__ float2 sincos2(float a)
 	return make_float2(c, s);
 }
 
-struct PCShared
+// LDS of the forward and inverse kernels (MLP outputs over the features) and
+// of the backward kernel (outputs apart: they carry dL/dy)
+template <bool SEP_Y> struct PCSharedT
 {
 	FM::LW<NI> lw;
-	FM::Tile<NI> tile[4];
+	FM::Tile<NI, SEP_Y> tile[4];
 	double red[4][4];
 };
+using PCShared = PCSharedT<false>;
+using PCSharedB = PCSharedT<true>;
 
 // One layer on the workgroup: wavefront w takes tiles w, w + 4, ... of the
 // NA active plaquettes: features, MLP, then the circle map on quads (lane
@@ -138,7 +175,7 @@ template <bool INV>
 __device__ void layer_apply(PGeo const &g, float *phi, float2 *cs, int m, PCShared &sh, double &ld)
 {
 	const int w = threadIdx.x >> 6, l = threadIdx.x & 63, NA = n_active(g, m), NT = (NA + 15) / 16;
-	FM::Tile<NI> &T = sh.tile[w];
+	auto &T = sh.tile[w];
 	for (int tix = w; tix < NT; tix += 4)
 	{
 		features(g, cs, m, tix * 16, NA, T);
@@ -150,7 +187,7 @@ __device__ void layer_apply(PGeo const &g, float *phi, float2 *cs, int m, PCShar
 			int x0, x1;
 			active_xy(g, m, r, x0, x1);
 			const int x = x0 * g.L1 + x1;
-			const FM::NCPq p = FM::ncpq_params(&T.y[row * 16]);
+			const FM::NCPq p = FM::ncpq_params(&T.y()[row * 16]);
 			const float a = FM::wrapf(phi[x] - p.t);
 			float v, Gp;
 			if (INV)
@@ -335,7 +372,7 @@ __global__ __launch_bounds__(THREADS) void k_pc_backward(PGeo g, int nl, const f
 	float *phi = dyn + 2 * g.V;  // input of the current layer
 	float *a = dyn + 3 * g.V;    // adjoint w.r.t. the current layer's output
 	float *gact = dyn + 4 * g.V; // adjoint w.r.t. the active plaquettes' input
-	__shared__ PCShared sh;
+	__shared__ PCSharedB sh;
 	const int c = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
 	float *dx = dxs + (long)c * (g.V >> 2) * NI;
 	float *part = partial + (long)c * nl * NP_LAYER;
@@ -368,7 +405,7 @@ __global__ __launch_bounds__(THREADS) void k_pc_backward(PGeo g, int nl, const f
 		__syncthreads();
 		FM::Grad<NI> gr;
 		gr.zero();
-		FM::Tile<NI> &T = sh.tile[w];
+		auto &T = sh.tile[w];
 		for (int tix = w; tix < NT; tix += 4)
 		{
 			features(g, cs, m, tix * 16, NA, T);
@@ -385,11 +422,11 @@ __global__ __launch_bounds__(THREADS) void k_pc_backward(PGeo g, int nl, const f
 					int x0, x1;
 					active_xy(g, m, r, x0, x1);
 					const int x = x0 * g.L1 + x1;
-					const float *y = T.y + l * 16;
+					const float *y = T.y() + l * 16;
 					gact[r] = FM::ncp_adjoint(y, FM::wrapf(phi[x] - FM::ncp_params(y).t), a[x], dy);
 				}
 				for (int k = 0; k < 16; ++k)
-					T.y[l * 16 + k] = dy[k];
+					T.y()[l * 16 + k] = dy[k];
 			}
 			FM::wave_sync();
 			FM::mlp_backward<NI>(sh.lw, T, gr, dx, tix * 16, NA);
@@ -425,6 +462,16 @@ __global__ __launch_bounds__(THREADS) void k_pc_backward(PGeo g, int nl, const f
 					const float2 ps = loop_cs(g, cs, x0, x1, 8 + j);
 					acc = fmaf(-ps.y, dr[16 + 2 * j], acc);
 					acc = fmaf(ps.x, dr[16 + 2 * j + 1], acc);
+				}
+				// the L-shaped loops (ring positions 2j - 1, 2j, 2j + 1) that
+				// contain ring position k: one for a corner, two for an edge
+				for (int j : {(k & 1) ? (k - 1) >> 1 : k >> 1, (k & 1) ? ((k + 1) >> 1) & 3 : -1})
+				{
+					if (j < 0)
+						continue;
+					const float2 ps = loop_cs(g, cs, x0, x1, 16 + j);
+					acc = fmaf(-ps.y, dr[32 + 2 * j], acc);
+					acc = fmaf(ps.x, dr[32 + 2 * j + 1], acc);
 				}
 			}
 			a[y] += acc;

@@ -183,9 +183,9 @@ class Flow:
 class PlaqCouplingFlow(Flow):
     """Coupling layers on the plaquettes of an L0 x L1 lattice (pcflow.hip):
     layer l moves the plaquettes of parity class l % 4 through a circle map
-    whose parameters an MLP computes from the surrounding plaquettes and 1x2
-    loops; the links follow by a maximal-tree gauge with uniform holonomies
-    and gauge.  Same interface as Flow (even extents >= 4)."""
+    whose parameters an MLP computes from the surrounding plaquettes, 1x2
+    loops and L-shaped loops; the links follow by a maximal-tree gauge with
+    uniform holonomies and gauge.  Same interface as Flow (even extents >= 4)."""
 
     _api = "mesh_pcflow"
 

@@ -446,10 +446,12 @@ def pflow_loss(params, u, beta):
 # (L0-1, L1-1), enters contexts as 0) start uniform; layer l moves the
 # plaquettes of parity class m = l % 4, (x0 % 2, x1 % 2) = (m % 2, m // 2), by
 # h(a) = g(wrap(a - t)) + t, the parameters from an MLP of (cos, sin) of the 8
-# ring plaquettes and the 8 1x2 loops of neighbouring ring pairs; the links
-# follow as for the plaquette flow (pflow_links).
+# ring plaquettes, the 8 1x2 loops of neighbouring ring pairs and the 4
+# L-shaped loops of ring positions (2j-1, 2j, 2j+1) (each 2x2 square of the
+# 3x3 neighbourhood without its centre); the links follow as for the
+# plaquette flow (pflow_links).
 
-PC_NI = 32
+PC_NI = 40
 PC_NP_LAYER = PC_NI * NH + NH + NH * NH + NH + NH * NO + NO
 TAG_PCFZ = 0x5043465A
 # ring offsets (d0, d1), in order around the plaquette
@@ -501,6 +503,9 @@ def _pc_mlp_y(W, phi, mask):
         feats += [torch.cos(ring[k]), torch.sin(ring[k])]
     for k in range(8):
         lp = ring[k] + ring[(k + 1) % 8]
+        feats += [torch.cos(lp), torch.sin(lp)]
+    for j in range(4):
+        lp = ring[(2 * j + 7) % 8] + ring[2 * j] + ring[2 * j + 1]
         feats += [torch.cos(lp), torch.sin(lp)]
     X = torch.stack(feats, dim=-1)
     H1 = torch.tanh(X @ W1 + b1)
