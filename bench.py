@@ -287,7 +287,7 @@ def kernel_only(name, prec, L, reps, variant=-1):
                       "launches": n, "copy_bytes_per_launch": 2 * U.bytes}), flush=True)
 
 
-def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pflow", "coupling"), substeps=20,
+def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pflow", "pcflow", "coupling"), substeps=20,
            start="hot", n_therm=500, layers=8):
     """U(1) 2-d at L x L: ESS/s of the topological charge Q for plain HMC
     (2lf, tau = 1, `substeps` steps) and for HMC interleaved 1:1 with
@@ -593,7 +593,7 @@ def main():
     ap.add_argument("--u1-train", type=int, default=4000)
     ap.add_argument("--u1-hmc-steps", type=int, default=4000)
     ap.add_argument("--u1-flow-steps", type=int, default=1000)
-    ap.add_argument("--u1-flows", default="pflow,coupling", help="flows to measure (pflow, coupling, pcflow)")
+    ap.add_argument("--u1-flows", default="pflow,pcflow,coupling", help="flows to measure (pflow, pcflow, coupling)")
     ap.add_argument("--u1-substeps", type=int, default=20, help="HMC steps per unit trajectory")
     ap.add_argument("--u1-start", choices=("hot", "cold"), default="hot")
     ap.add_argument("--u1-therm", type=int, default=500, help="HMC thermalisation trajectories")
