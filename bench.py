@@ -287,8 +287,8 @@ def kernel_only(name, prec, L, reps, variant=-1):
                       "launches": n, "copy_bytes_per_launch": 2 * U.bytes}), flush=True)
 
 
-def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pflow", "pcflow", "coupling"), substeps=20,
-           start="hot", n_therm=500, layers=8):
+def u1_ess(L, beta, batch, train_steps, hmc_steps, flow_steps, seed, flows=("pflow", "pcflow", "coupling"),
+           substeps=20, start="hot", n_therm=500, layers=8):
     """U(1) 2-d at L x L: ESS/s of the topological charge Q for plain HMC
     (2lf, tau = 1, `substeps` steps) and for HMC interleaved 1:1 with
     independence MH steps proposed by each trained flow ("pflow": plaquette

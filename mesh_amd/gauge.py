@@ -241,8 +241,9 @@ class Hmc:
     def summary(self):
         """Same quantities as Hmc::print_summary (hmc.h:88-100)."""
         p = np.array(self.plaq_history)
+        err = float(p.std(ddof=1) / math.sqrt(len(p))) if len(p) > 1 else 0.0
         return dict(time_per_step=float(np.mean(self.time_history)),
-                    plaquette=float(p.mean()), plaquette_err=float(p.std(ddof=1) / math.sqrt(len(p))) if len(p) > 1 else 0.0,
+                    plaquette=float(p.mean()), plaquette_err=err,
                     acceptance=float(np.mean(self.accept_history)),
                     exp_mdH=float(np.mean(np.exp(-np.array(self.deltaH_history)))))
 
