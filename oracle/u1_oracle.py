@@ -1,8 +1,10 @@
 """CPU oracle of the U(1) 2-d path: observables, HMC, exact finite-volume
-results, and the coupling-layer flow.
+results, Landau gauge on angles, and the learned proposals (the coupling-layer
+flow on links, the plaquette flow, the coupling-layer flow on plaquettes).
 
 TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's cpu_baseline;
-never by mesh_amd).  Conventions follow mesh_amd/csrc/u1.hip and flow.hip,
+never by mesh_amd).  Conventions follow mesh_amd/csrc/u1.hip, flow.hip,
+pflow.hip and pcflow.hip,
 which follow the reference's U(1) algebra (src/gauge/groups.h:29-108):
 angles theta[mu][x0][x1], theta_P = theta_0(x) + theta_1(x+0) - theta_0(x+1)
 - theta_1(x), S = beta sum (1 - cos theta_P), momenta q ~ N(0, 1/2)
