@@ -23,6 +23,13 @@ using namespace mesh_dev;
 #ifndef MESH_PIPE_F64
 #define MESH_PIPE_F64 1
 #endif
+// FP32: the global operands of the shared upper-staple factors loaded before
+// the staging barrier (share_fetch; same bits); 0: after it.  Same-box A/B at
+// 32^4: FP32 0.3443 -> 0.3401 ms; in FP64 (one plane's two links per
+// wavefront) it measured 0.8% slower and is not used.
+#ifndef MESH_SHARE_PREFETCH
+#define MESH_SHARE_PREFETCH 1
+#endif
 // waves per SIMD the force kernels are compiled for (0: the compiler's choice)
 #ifndef MESH_FORCE_WAVES
 #define MESH_FORCE_WAVES 0
@@ -264,22 +271,10 @@ __host__ __device__ constexpr int plane_index(int p, int q) { return p == 0 ? q 
 // the factor of direction p in plane q (direction q uses A_pq^+).  Mode 2:
 // the six planes split 2, 2, 1, 1 over the wavefronts of directions 0..3;
 // mode 1: planes (0, w) formed by wavefront w = 1, 2, 3.
-template <int W, bool ROWS, int MODE, class R>
-__device__ __forceinline__ void share_upper_factors(Geo const &g, const R *__restrict__ U, long x, int const *c,
-                                                    vec2_t<R> (*own)[9][64], vec2_t<R> (*sh)[9][64], int lane)
+// the planes (p, q) whose factors wavefront W forms, f(P, Q, I) with I its
+// index among them
+template <int W, int MODE, class F> __device__ __forceinline__ void for_shared_planes(F &&f)
 {
-	int sp = 0, sm = 0;
-	if constexpr (ROWS)
-		x3_slots(g, x, c, sp, sm);
-	auto one = [&](auto P, auto Q) {
-		constexpr int p = decltype(P)::value, q = decltype(Q)::value;
-		m3<R> a = (ROWS && p == 3) ? lds_m3<R>(own, q, sp) : load(U, q, g, shift_c(g, x, p, c[p], 1));
-		m3<R> b = (ROWS && q == 3) ? lds_m3<R>(own, p, sp) : load(U, p, g, shift_c(g, x, q, c[q], 1));
-		m3<R> A = mul_nd(a, b);
-#pragma unroll
-		for (int k = 0; k < 9; ++k)
-			sh[plane_index(p, q)][k][lane] = vec2_t<R>{A.e[k].re, A.e[k].im};
-	};
 	using I0 = std::integral_constant<int, 0>;
 	using I1 = std::integral_constant<int, 1>;
 	using I2 = std::integral_constant<int, 2>;
@@ -287,26 +282,73 @@ __device__ __forceinline__ void share_upper_factors(Geo const &g, const R *__res
 	if constexpr (MODE == 1)
 	{
 		if constexpr (W == 1)
-			one(I0{}, I1{});
+			f(I0{}, I1{}, I0{});
 		else if constexpr (W == 2)
-			one(I0{}, I2{});
+			f(I0{}, I2{}, I0{});
 		else if constexpr (W == 3)
-			one(I0{}, I3{});
+			f(I0{}, I3{}, I0{});
 	}
 	else if constexpr (W == 0)
 	{
-		one(I0{}, I1{});
-		one(I0{}, I2{});
+		f(I0{}, I1{}, I0{});
+		f(I0{}, I2{}, I1{});
 	}
 	else if constexpr (W == 1)
 	{
-		one(I1{}, I2{});
-		one(I1{}, I3{});
+		f(I1{}, I2{}, I0{});
+		f(I1{}, I3{}, I1{});
 	}
 	else if constexpr (W == 2)
-		one(I2{}, I3{});
+		f(I2{}, I3{}, I0{});
 	else
-		one(I0{}, I3{});
+		f(I0{}, I3{}, I0{});
+}
+
+// the factors' operands that come from global memory (with ROWS the x3-
+// neighbour ones come from the staged block): loaded by share_fetch before
+// the staging barrier, since they do not depend on it, so that their latency
+// overlaps the staging's
+template <class R> struct ShareOps
+{
+	m3<R> a[2], b[2];
+};
+
+template <int W, bool ROWS, int MODE, class R>
+__device__ __forceinline__ void share_fetch(Geo const &g, const R *__restrict__ U, long x, int const *c,
+                                            ShareOps<R> &F)
+{
+	for_shared_planes<W, MODE>([&](auto P, auto Q, auto I) {
+		constexpr int p = decltype(P)::value, q = decltype(Q)::value, i = decltype(I)::value;
+		if constexpr (!(ROWS && p == 3))
+			F.a[i] = load(U, q, g, shift_c(g, x, p, c[p], 1));
+		if constexpr (!(ROWS && q == 3))
+			F.b[i] = load(U, p, g, shift_c(g, x, q, c[q], 1));
+	});
+}
+
+template <int W, bool ROWS, int MODE, class R>
+__device__ __forceinline__ void share_upper_factors(Geo const &g, long x, int const *c, vec2_t<R> (*own)[9][64],
+                                                    vec2_t<R> (*sh)[9][64], int lane, ShareOps<R> const &F)
+{
+	int sp = 0, sm = 0;
+	if constexpr (ROWS)
+		x3_slots(g, x, c, sp, sm);
+	for_shared_planes<W, MODE>([&](auto P, auto Q, auto I) {
+		constexpr int p = decltype(P)::value, q = decltype(Q)::value, i = decltype(I)::value;
+		m3<R> a, b;
+		if constexpr (ROWS && p == 3)
+			a = lds_m3<R>(own, q, sp);
+		else
+			a = F.a[i];
+		if constexpr (ROWS && q == 3)
+			b = lds_m3<R>(own, p, sp);
+		else
+			b = F.b[i];
+		m3<R> A = mul_nd(a, b);
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+			sh[plane_index(p, q)][k][lane] = vec2_t<R>{A.e[k].re, A.e[k].im};
+	});
 }
 
 // SH: share mode of the first upper-staple factors (share_upper_factors)
@@ -710,6 +752,28 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 		// block's own links, then the staples read them from LDS
 		__shared__ vec2_t<R> own[4][9][64];
 		const int lane = threadIdx.x & 63;
+		int c[4];
+		coords<R>(g, x, c);
+		constexpr bool RW = PATH == STAPLE_ROWS; // the block holds whole x3-rows
+		constexpr int SH = share_mode<R>();
+		ShareOps<R> F;
+		constexpr bool PF = MESH_SHARE_PREFETCH && std::is_same<R, float>::value;
+		if constexpr (SH > 0 && PF)
+			switch (mu)
+			{
+			case 0:
+				share_fetch<0, RW, SH>(g, U, x, c, F);
+				break;
+			case 1:
+				share_fetch<1, RW, SH>(g, U, x, c, F);
+				break;
+			case 2:
+				share_fetch<2, RW, SH>(g, U, x, c, F);
+				break;
+			default:
+				share_fetch<3, RW, SH>(g, U, x, c, F);
+				break;
+			}
 		{
 			const vec2_t<R> *p = reinterpret_cast<const vec2_t<R> *>(U) + site_off(g, mu, x);
 #pragma unroll
@@ -717,27 +781,39 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 				own[mu][k][lane] = p[k * plane_step(g)];
 		}
 		__syncthreads();
-		int c[4];
-		coords<R>(g, x, c);
-		constexpr bool RW = PATH == STAPLE_ROWS; // the block holds whole x3-rows
-		constexpr int SH = share_mode<R>();
 		if constexpr (SH > 0)
 		{
 			// planes (0, q) are indices 0..2, so mode 1 needs three
 			__shared__ vec2_t<R> sh[SH == 2 ? 6 : 3][9][64];
+			if constexpr (!PF)
+				switch (mu)
+				{
+				case 0:
+					share_fetch<0, RW, SH>(g, U, x, c, F);
+					break;
+				case 1:
+					share_fetch<1, RW, SH>(g, U, x, c, F);
+					break;
+				case 2:
+					share_fetch<2, RW, SH>(g, U, x, c, F);
+					break;
+				default:
+					share_fetch<3, RW, SH>(g, U, x, c, F);
+					break;
+				}
 			switch (mu)
 			{
 			case 0:
-				share_upper_factors<0, RW, SH>(g, U, x, c, own, sh, lane);
+				share_upper_factors<0, RW, SH>(g, x, c, own, sh, lane, F);
 				break;
 			case 1:
-				share_upper_factors<1, RW, SH>(g, U, x, c, own, sh, lane);
+				share_upper_factors<1, RW, SH>(g, x, c, own, sh, lane, F);
 				break;
 			case 2:
-				share_upper_factors<2, RW, SH>(g, U, x, c, own, sh, lane);
+				share_upper_factors<2, RW, SH>(g, x, c, own, sh, lane, F);
 				break;
 			default:
-				share_upper_factors<3, RW, SH>(g, U, x, c, own, sh, lane);
+				share_upper_factors<3, RW, SH>(g, x, c, own, sh, lane, F);
 				break;
 			}
 			__syncthreads();
