@@ -481,6 +481,13 @@ int mesh_pflow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, const float 
 /* out_dev[2c + {0,1}] = {log q(U), S(U)/beta} of configurations theta */
 int mesh_pflow_logq(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, const float *theta,
                     double *out_dev);
+/* one MH proposal round in one launch: state_dev[2c + {0,1}] = {log q, S/beta}
+ * of the current configurations theta (as mesh_pflow_logq) and proposals
+ * theta_prop (as mesh_pflow_sample, the same links) with prop_dev = their
+ * {log q, S/beta}, log q from the draws (the inverse map's value up to
+ * rounding); theta_prop must not alias theta, nor prop_dev state_dev */
+int mesh_pflow_propose(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, const float *theta,
+                       double *state_dev, uint64_t seed, uint32_t step, float *theta_prop, double *prop_dev);
 /* bytes of the workspace mesh_pflow_gradient needs */
 size_t mesh_pflow_workspace_bytes(const mesh_grid *g, int batch);
 /* reverse-KL gradient grad = (1/batch) sum_c d/dparams [log q + beta S] over a

@@ -126,6 +126,7 @@ _SIGS = {
     "mesh_pflow_init": (i32, [vp, vp]),
     "mesh_pflow_sample": (i32, [vp, gp, i32, vp, u64, u32, vp]),
     "mesh_pflow_logq": (i32, [vp, gp, i32, vp, vp, vp]),
+    "mesh_pflow_propose": (i32, [vp, gp, i32, vp, vp, vp, u64, u32, vp, vp]),
     "mesh_pflow_workspace_bytes": (C.c_size_t, [gp, i32]),
     "mesh_pflow_gradient": (i32, [vp, gp, i32, vp, u64, u32, f64, vp, vp, vp]),
     "mesh_pflow_adam": (i32, [vp, vp, vp, vp, vp, i32, f64]),

@@ -175,43 +175,60 @@ __device__ __forceinline__ float uniform_angle(uint64_t seed, uint32_t step, uin
 	return u1_plaq::uniform_angle(seed, step, a, b, TAG_PFLOW);
 }
 
-// Proposals: links of configuration c into theta_out[c][2V].
+// Proposals: links of configuration c into theta_out[c][2V]; with out,
+// out[2c + {0,1}] = {log q, S/beta} of the proposal, log q from the draws
+// (-log g' of each free plaquette, the inverse map's value up to rounding)
+// and S from the links' plaquettes.
 // Draws: plaquette x (x != V-1) counter (x, 2c, step<<5, tag); holonomies
 // (V, 2c, ..) and (V+1, 2c, ..); gauge angle a(x) (x, 2c+1, ..).
-__global__ __launch_bounds__(THREADS) void k_pflow_sample(PGeo g, const float *__restrict__ params, uint64_t seed,
-                                                          uint32_t step, float *__restrict__ theta_out)
+// LDS: 4V floats.
+__device__ void pflow_sample(PGeo const &g, const float *__restrict__ params, uint64_t seed, uint32_t step,
+                             float *__restrict__ theta_out, double *out, int c, float *lds, double (*red)[4])
 {
-	extern __shared__ float lds[];
 	float *P = lds;          // V plaquette angles (the last unused)
 	float *t = lds + g.V;    // 2V links
 	float *al = lds + 3 * g.V; // V gauge angles
-	const int c = blockIdx.x;
 	const Map p = map_params(params);
 	const uint32_t cc = (uint32_t)c << 1;
+	double v[1] = {0.0};
 	for (int x = threadIdx.x; x < g.V; x += THREADS)
 	{
 		if (x < g.V - 1)
 		{
-			float v = wrapf(uniform_angle(seed, step, (uint32_t)x, cc) - p.t), G, Gp;
-			g_eval(p, v, G, Gp);
+			float u = wrapf(uniform_angle(seed, step, (uint32_t)x, cc) - p.t), G, Gp;
+			g_eval(p, u, G, Gp);
 			P[x] = wrapf(G + p.t);
+			v[0] -= (double)logf(Gp);
 		}
 		al[x] = uniform_angle(seed, step, (uint32_t)x, cc | 1u);
 	}
 	__syncthreads();
 	const float h0 = uniform_angle(seed, step, (uint32_t)g.V, cc);
 	const float h1 = uniform_angle(seed, step, (uint32_t)g.V + 1u, cc);
-	u1_plaq::links_from_plaquettes(g, P, h0, h1, al, t, theta_out + (long)c * 2 * g.V, THREADS);
+	float *o = theta_out + (long)c * 2 * g.V;
+	u1_plaq::links_from_plaquettes(g, P, h0, h1, al, t, o, THREADS);
+	if (!out)
+		return;
+	__syncthreads();
+	double w[2] = {v[0], 0.0};
+	for (int x = threadIdx.x; x < g.V; x += THREADS)
+	{
+		const int x0 = x / g.L1, x1 = x - x0 * g.L1;
+		w[1] += 1.0 - cos((double)plaqp(g, o, x0, x1));
+	}
+	block_sum<2>(w, red);
+	if (threadIdx.x == 0)
+	{
+		out[2 * c + 0] = w[0] - (double)(g.V + 1) * LOG_2PI - (double)(g.V - 1) * LOG_2PI;
+		out[2 * c + 1] = w[1];
+	}
 }
 
-// out[2c + {0,1}] = {log q(U), S(U)/beta} of configurations theta[c][2V]
-__global__ __launch_bounds__(THREADS) void k_pflow_logq(PGeo g, const float *__restrict__ params,
-                                                        const float *__restrict__ theta, double *out)
+// out[2c + {0,1}] = {log q(U), S(U)/beta} of configurations theta[c][2V];
+// LDS: 2V floats
+__device__ void pflow_logq(PGeo const &g, const float *__restrict__ params, const float *__restrict__ theta,
+                           double *out, int c, float *t, double (*red)[4], float *tab)
 {
-	extern __shared__ float t[]; // 2V
-	__shared__ double red[2][4];
-	__shared__ float tab[NTAB + 1];
-	const int c = blockIdx.x;
 	const Map p = map_params(params);
 	for (int i = threadIdx.x; i < 2 * g.V; i += THREADS)
 		t[i] = theta[(long)c * 2 * g.V + i];
@@ -240,6 +257,39 @@ __global__ __launch_bounds__(THREADS) void k_pflow_logq(PGeo g, const float *__r
 		out[2 * c + 0] = v[0] - (double)(g.V + 1) * LOG_2PI - (double)(g.V - 1) * LOG_2PI;
 		out[2 * c + 1] = v[1];
 	}
+}
+
+__global__ __launch_bounds__(THREADS) void k_pflow_sample(PGeo g, const float *__restrict__ params, uint64_t seed,
+                                                          uint32_t step, float *__restrict__ theta_out)
+{
+	extern __shared__ float lds[];
+	pflow_sample(g, params, seed, step, theta_out, nullptr, blockIdx.x, lds, nullptr);
+}
+
+__global__ __launch_bounds__(THREADS) void k_pflow_logq(PGeo g, const float *__restrict__ params,
+                                                        const float *__restrict__ theta, double *out)
+{
+	extern __shared__ float t[]; // 2V
+	__shared__ double red[2][4];
+	__shared__ float tab[NTAB + 1];
+	pflow_logq(g, params, theta, out, blockIdx.x, t, red, tab);
+}
+
+// one MH proposal round in one launch: workgroups 0..batch-1 take log q of
+// the current configurations, batch..2 batch-1 draw the proposals with their
+// log q and S/beta, side by side on the CUs
+__global__ __launch_bounds__(THREADS) void k_pflow_propose(PGeo g, const float *__restrict__ params, int batch,
+                                                           const float *__restrict__ theta, double *state,
+                                                           uint64_t seed, uint32_t step, float *theta_prop,
+                                                           double *prop_out)
+{
+	extern __shared__ float lds[];
+	__shared__ double red[2][4];
+	__shared__ float tab[NTAB + 1];
+	if ((int)blockIdx.x < batch)
+		pflow_logq(g, params, theta, state, blockIdx.x, lds, red, tab);
+	else
+		pflow_sample(g, params, seed, step, theta_prop, prop_out, blockIdx.x - batch, lds, red);
 }
 
 // Reverse-KL gradient of one configuration: loss = log q + beta S with
@@ -387,6 +437,26 @@ int mesh_pflow_sample(mesh_ctx *ctx, const mesh_grid *g, int batch, const float 
 	MESH_HIP(ctx, hipFuncSetAttribute((const void *)k_pflow_sample, hipFuncAttributeMaxDynamicSharedMemorySize,
 	                                  (int)lds));
 	k_pflow_sample<<<batch, THREADS, lds, ctx->stream>>>(pg, params, seed, step, theta_out);
+	MESH_LAUNCHED(ctx);
+	return MESH_OK;
+}
+
+int mesh_pflow_propose(mesh_ctx *ctx, const mesh_grid *g, int batch, const float *params, const float *theta,
+                       double *state_dev, uint64_t seed, uint32_t step, float *theta_prop, double *prop_dev)
+{
+	MESH_CHECK_CTX(ctx);
+	PGeo pg;
+	if (int rc = pflow_setup(ctx, g, batch, &pg))
+		return rc;
+	if (!params || !theta || !state_dev || !theta_prop || !prop_dev)
+		return mesh_fail(ctx, MESH_E_ARG, "NULL argument to pflow_propose");
+	if (theta == theta_prop || state_dev == prop_dev)
+		return mesh_fail(ctx, MESH_E_ARG, "pflow_propose: the proposals and their log q need their own buffers");
+	size_t lds = 4 * (size_t)pg.V * sizeof(float);
+	MESH_HIP(ctx, hipFuncSetAttribute((const void *)k_pflow_propose, hipFuncAttributeMaxDynamicSharedMemorySize,
+	                                  (int)lds));
+	k_pflow_propose<<<2 * batch, THREADS, lds, ctx->stream>>>(pg, params, batch, theta, state_dev, seed, step,
+	                                                          theta_prop, prop_dev);
 	MESH_LAUNCHED(ctx);
 	return MESH_OK;
 }

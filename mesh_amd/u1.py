@@ -269,6 +269,12 @@ class PlaqFlow:
         self.ctx.call("mesh_pflow_logq", self.grid.ref, ens.batch, self.params.ptr, ens.theta.ptr, out.ptr)
         return out
 
+    def propose(self, ens, state, seed, step, theta_out, out):
+        """logq(ens) into `state` and proposals into `theta_out` with their
+        (log q, S/beta) in `out`, in one launch (mesh_pflow_propose)."""
+        self.ctx.call("mesh_pflow_propose", self.grid.ref, ens.batch, self.params.ptr, ens.theta.ptr, state.ptr,
+                      C.c_uint64(seed), C.c_uint32(step), theta_out.ptr, out.ptr)
+
 
 def tau_int(series, S=1.5):
     """Integrated autocorrelation time by the Gamma method with Wolff's

@@ -383,6 +383,38 @@ def test_pflow_logq_of_given_configurations(ctx):
     np.testing.assert_allclose(out[:, 1], S, rtol=1e-5)
 
 
+def test_pflow_propose(ctx):
+    """mesh_pflow_propose: the states' (log q, S/beta) are mesh_pflow_logq's
+    bit for bit, the proposals mesh_pflow_sample's, and the proposals' log q
+    (from the draws) that of the inverse map to rounding; aliased outputs are
+    refused"""
+    for L in (8, 32):
+        B, seed, step = 16, 201, 4
+        f = U1.PlaqFlow(L, L, ctx)
+        f.set_params(_pparams(202, 0.5))
+        ens = U1.U1Ensemble(L, L, B, ctx)
+        ens.random_start(203)
+        st = U1.Buffer(2 * B, np.float64, ctx)
+        prop = U1.Buffer(B * 2 * L * L, np.float32, ctx)
+        pout = U1.Buffer(2 * B, np.float64, ctx)
+        f.propose(ens, st, seed, step, prop, pout)
+        np.testing.assert_array_equal(st.download(), f.logq(ens).download())
+        ref = U1.Buffer(B * 2 * L * L, np.float32, ctx)
+        ctx.call("mesh_pflow_sample", ens.grid.ref, B, f.params.ptr, U1.C.c_uint64(seed), U1.C.c_uint32(step),
+                 ref.ptr)
+        np.testing.assert_array_equal(prop.download(), ref.download())
+        e2 = U1.U1Ensemble(L, L, B, ctx)
+        e2.set_angles(prop.download().reshape(B, 2, L, L))
+        inv = f.logq(e2).download().reshape(B, 2)
+        po = pout.download().reshape(B, 2)
+        np.testing.assert_allclose(po[:, 0], inv[:, 0], atol=2e-3)
+        np.testing.assert_array_equal(po[:, 1], inv[:, 1])
+        with pytest.raises(mesh_amd.MeshError):
+            f.propose(ens, st, seed, step, ens.theta, pout)
+        with pytest.raises(mesh_amd.MeshError):
+            f.propose(ens, st, seed, step, prop, st)
+
+
 def test_pflow_gradient_against_autograd(ctx):
     torch = pytest.importorskip("torch")
     L, B, beta, seed, step = 8, 6, 2.0, 65, 3
