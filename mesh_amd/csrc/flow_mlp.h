@@ -91,14 +91,17 @@ template <int NI> __device__ void load_weights(LW<NI> &lw, const float *__restri
 }
 
 // per-wavefront tile scratch: features (row stride XS; the columns from NI on
-// must be finite), two hidden layers and the outputs y().  SEP_Y = false puts
-// the outputs over the features, which the first layer has consumed by then
-// (kernels without a backward pass: 1 KB less LDS per tile).
-template <int NI, bool SEP_Y = true> struct Tile
+// must be finite), two hidden layers h1, h2() and the outputs y().  SEP = false
+// (kernels without a backward pass, which needs every layer's values) puts h2
+// over h1 and the outputs over the features, each consumed by then: 5 KB less
+// LDS per tile.
+template <int NI, bool SEP = true> struct Tile
 {
-	float x[16 * Layout<NI>::XS], h1[16 * NH], h2[16 * NH];
-	float ys[SEP_Y ? 16 * 16 : 1];
-	__device__ float *y() { return SEP_Y ? ys : x; }
+	float x[16 * Layout<NI>::XS], h1[16 * NH];
+	float h2s[SEP ? 16 * NH : 1];
+	float ys[SEP ? 16 * 16 : 1];
+	__device__ float *h2() { return SEP ? h2s : h1; }
+	__device__ float *y() { return SEP ? ys : x; }
 };
 
 // One layer's parameters held in registers (two floats a lane per chunk) so
@@ -179,14 +182,26 @@ template <int NI, bool FAST = false, bool SY = true> __device__ void mlp_forward
 			T.h1[(rq + j) * NH + n0 + c] = act<FAST>(d[j] + lw.b1[n0 + c]);
 	}
 	wave_sync();
-	for (int n0 = 0; n0 < NH; n0 += 16)
+	if constexpr (SY)
+		for (int n0 = 0; n0 < NH; n0 += 16)
+		{
+			f4 d = tile_mm(T.h1, NH, 1, lw.w2 + n0, NH, 1, NH);
+			for (int j = 0; j < 4; ++j)
+				T.h2()[(rq + j) * NH + n0 + c] = act<FAST>(d[j] + lw.b2[n0 + c]);
+		}
+	else
 	{
-		f4 d = tile_mm(T.h1, NH, 1, lw.w2 + n0, NH, 1, NH);
-		for (int j = 0; j < 4; ++j)
-			T.h2[(rq + j) * NH + n0 + c] = act<FAST>(d[j] + lw.b2[n0 + c]);
+		// h2() is h1's storage: all four column blocks of h1 W2 first
+		f4 d2[NH / 16];
+		for (int n0 = 0; n0 < NH; n0 += 16)
+			d2[n0 / 16] = tile_mm(T.h1, NH, 1, lw.w2 + n0, NH, 1, NH);
+		wave_sync();
+		for (int n0 = 0; n0 < NH; n0 += 16)
+			for (int j = 0; j < 4; ++j)
+				T.h2()[(rq + j) * NH + n0 + c] = act<FAST>(d2[n0 / 16][j] + lw.b2[n0 + c]);
 	}
 	wave_sync();
-	f4 d = tile_mm(T.h2, NH, 1, lw.w3, 16, 1, NH);
+	f4 d = tile_mm(T.h2(), NH, 1, lw.w3, 16, 1, NH);
 	for (int j = 0; j < 4; ++j)
 		T.y()[(rq + j) * 16 + c] = d[j] + lw.b3[c];
 	wave_sync();
@@ -430,7 +445,7 @@ template <int NI> struct Grad
 	}
 };
 
-// MLP backward of one tile whose T.y() holds dL/dy (and T.x, T.h1, T.h2 the
+// MLP backward of one tile whose T.y() holds dL/dy (and T.x, T.h1, T.h2() the
 // forward values): accumulates the weight gradients into G and writes
 // dL/dx of rows r0 .. r0+15 (r < nrows) to dx[r * NI + feature]
 template <int NI>
@@ -440,7 +455,7 @@ __device__ void mlp_backward(LW<NI> const &lw, Tile<NI> &T, Grad<NI> &G, float *
 	const int l = threadIdx.x & 63, lc = l & 15, rq = (l >> 4) * 4;
 	// dW3 += H2^T dY, db3
 	for (int i0 = 0; i0 < NH; i0 += 16)
-		G.w3[i0 / 16] += tile_mm(T.h2 + i0, 1, NH, T.y(), 16, 1, 16);
+		G.w3[i0 / 16] += tile_mm(T.h2() + i0, 1, NH, T.y(), 16, 1, 16);
 	if (l < 16)
 		for (int rr = 0; rr < 16; ++rr)
 			G.b3 += T.y()[rr * 16 + l];
@@ -452,20 +467,20 @@ __device__ void mlp_backward(LW<NI> const &lw, Tile<NI> &T, Grad<NI> &G, float *
 	for (int n0 = 0; n0 < NH; n0 += 16)
 		for (int j = 0; j < 4; ++j)
 		{
-			float hv = T.h2[(rq + j) * NH + n0 + lc];
-			T.h2[(rq + j) * NH + n0 + lc] = dh2[n0 / 16][j] * (1.f - hv * hv);
+			float hv = T.h2()[(rq + j) * NH + n0 + lc];
+			T.h2()[(rq + j) * NH + n0 + lc] = dh2[n0 / 16][j] * (1.f - hv * hv);
 		}
 	wave_sync();
 	// dW2 += H1^T dA2, db2
 	for (int i0 = 0; i0 < NH; i0 += 16)
 		for (int n0 = 0; n0 < NH; n0 += 16)
-			G.w2[(i0 / 16) * 4 + n0 / 16] += tile_mm(T.h1 + i0, 1, NH, T.h2 + n0, NH, 1, 16);
+			G.w2[(i0 / 16) * 4 + n0 / 16] += tile_mm(T.h1 + i0, 1, NH, T.h2() + n0, NH, 1, 16);
 	for (int rr = 0; rr < 16; ++rr)
-		G.b2 += T.h2[rr * NH + l];
+		G.b2 += T.h2()[rr * NH + l];
 	// dA1 = (dA2 W2^T) * (1 - h1^2), into h1
 	f4 dh1[4];
 	for (int n0 = 0; n0 < NH; n0 += 16)
-		dh1[n0 / 16] = tile_mm(T.h2, NH, 1, lw.w2 + n0 * NH, 1, NH, NH);
+		dh1[n0 / 16] = tile_mm(T.h2(), NH, 1, lw.w2 + n0 * NH, 1, NH, NH);
 	wave_sync();
 	for (int n0 = 0; n0 < NH; n0 += 16)
 		for (int j = 0; j < 4; ++j)
