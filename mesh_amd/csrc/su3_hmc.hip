@@ -72,20 +72,28 @@ template <class R, int PATH = 0> constexpr int force_min_waves()
 // Wavefront-per-direction kernel: the first upper-staple factor of a plane
 // formed once per workgroup and shared through LDS (the same bits either
 // way).  MESH_SHARE_UPPER (FP32) / MESH_SHARE_UPPER_F64: 0 each direction
-// forms its own, 1 the three planes (0, nu) are shared, 2 all six.  FP64 with
-// all six would need 91 KB of LDS per workgroup, one workgroup per CU.
+// forms its own, 1 the three planes (0, nu) are shared, 2 all six, 3 the four
+// planes other than (0, 1) and (2, 3), one formed by each wavefront (every
+// wavefront then forms two factors, as the busiest ones of mode 2 do, in 4 KB
+// less LDS per shared plane).  FP64 with all six would need 91 KB of LDS per
+// workgroup, one workgroup per CU.
 #ifndef MESH_SHARE_UPPER
-#define MESH_SHARE_UPPER 2
+#define MESH_SHARE_UPPER 3
 #endif
 #ifndef MESH_SHARE_UPPER_F64
-#define MESH_SHARE_UPPER_F64 1
+#define MESH_SHARE_UPPER_F64 3
 #endif
 template <class R> constexpr int share_mode()
 {
 	return std::is_same<R, float>::value ? MESH_SHARE_UPPER : MESH_SHARE_UPPER_F64;
 }
 // is plane (p, q), p < q, shared in mode m
-__host__ __device__ constexpr bool plane_shared(int m, int p) { return m == 2 || (m == 1 && p == 0); }
+__host__ __device__ constexpr bool plane_shared(int m, int p, int q)
+{
+	return m == 2 || (m == 1 && p == 0) || (m == 3 && p + q != 1 && p + q != 5);
+}
+// number of shared planes in mode m
+__host__ __device__ constexpr int shared_planes(int m) { return m == 2 ? 6 : (m == 3 ? 4 : 3); }
 
 // non-temporal hints on the once-touched fields of the force kernels
 #ifndef MESH_STREAM_HINTS
@@ -266,11 +274,18 @@ __device__ __forceinline__ void x3_slots(Geo const &g, long x, int const *c, int
 
 // index of plane (p, q), p < q, among the six of a 4-d lattice
 __host__ __device__ constexpr int plane_index(int p, int q) { return p == 0 ? q - 1 : (p == 1 ? q + 1 : 5); }
+// LDS slot of the shared factor of plane (p, q) in mode m (mode 3: the planes
+// (0, 2), (1, 3), (1, 2), (0, 3), formed by wavefronts 0..3)
+__host__ __device__ constexpr int share_slot(int m, int p, int q)
+{
+	return m != 3 ? plane_index(p, q) : (p == 0 ? (q == 2 ? 0 : 3) : (q == 3 ? 1 : 2));
+}
 
 // Shared first upper-staple factors: A_pq = U_q(x+p) U_p(x+q)^+ for p < q,
 // the factor of direction p in plane q (direction q uses A_pq^+).  Mode 2:
 // the six planes split 2, 2, 1, 1 over the wavefronts of directions 0..3;
-// mode 1: planes (0, w) formed by wavefront w = 1, 2, 3.
+// mode 1: planes (0, w) formed by wavefront w = 1, 2, 3; mode 3: one plane
+// each (share_slot).
 // the planes (p, q) whose factors wavefront W forms, f(P, Q, I) with I its
 // index among them
 template <int W, int MODE, class F> __device__ __forceinline__ void for_shared_planes(F &&f)
@@ -279,7 +294,18 @@ template <int W, int MODE, class F> __device__ __forceinline__ void for_shared_p
 	using I1 = std::integral_constant<int, 1>;
 	using I2 = std::integral_constant<int, 2>;
 	using I3 = std::integral_constant<int, 3>;
-	if constexpr (MODE == 1)
+	if constexpr (MODE == 3)
+	{
+		if constexpr (W == 0)
+			f(I0{}, I2{}, I0{});
+		else if constexpr (W == 1)
+			f(I1{}, I3{}, I0{});
+		else if constexpr (W == 2)
+			f(I1{}, I2{}, I0{});
+		else
+			f(I0{}, I3{}, I0{});
+	}
+	else if constexpr (MODE == 1)
 	{
 		if constexpr (W == 1)
 			f(I0{}, I1{}, I0{});
@@ -347,7 +373,7 @@ __device__ __forceinline__ void share_upper_factors(Geo const &g, long x, int co
 		m3<R> A = mul_nd(a, b);
 #pragma unroll
 		for (int k = 0; k < 9; ++k)
-			sh[plane_index(p, q)][k][lane] = vec2_t<R>{A.e[k].re, A.e[k].im};
+			sh[share_slot(MODE, p, q)][k][lane] = vec2_t<R>{A.e[k].re, A.e[k].im};
 	});
 }
 
@@ -373,9 +399,10 @@ __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict
 		long xpn = nu3 ? 0 : shift_c(g, x, nu, c[nu], 1);
 		long xmn = nu3 ? 0 : shift_c(g, x, nu, c[nu], -1);
 		long xpmmn = shift_c(g, xpm, nu, c[nu], -1);
-		if (plane_shared(SH, nu < MU ? nu : MU))
+		if (plane_shared(SH, nu < MU ? nu : MU, nu < MU ? MU : nu))
 		{
-			m3<R> T1 = nu < MU ? adj_m3(lds_m3<R>(sh, plane_index(nu, MU), lane)) : lds_m3<R>(sh, plane_index(MU, nu), lane);
+			m3<R> T1 = nu < MU ? adj_m3(lds_m3<R>(sh, share_slot(SH, nu, MU), lane))
+			                   : lds_m3<R>(sh, share_slot(SH, MU, nu), lane);
 			madd_nd(S, T1, own_at(nu, lane));
 		}
 		else
@@ -413,7 +440,7 @@ __device__ __forceinline__ m3<R> staple_rows_pipe(Geo const &g, const R *__restr
 	// operands of the upper staple's first product (planes not shared)
 	auto fetch_upper = [&](auto NU) {
 		constexpr int nu = decltype(NU)::value;
-		if constexpr (!plane_shared(SH, nu < MU ? nu : MU))
+		if constexpr (!plane_shared(SH, nu < MU ? nu : MU, nu < MU ? MU : nu))
 		{
 			A = MU == 3 ? lds_m3<R>(own, nu, sp) : load(U, nu, g, xpm);
 			B = nu == 3 ? lds_m3<R>(own, MU, sp) : load(U, MU, g, shift_c(g, x, nu, c[nu], 1));
@@ -426,8 +453,9 @@ __device__ __forceinline__ m3<R> staple_rows_pipe(Geo const &g, const R *__restr
 		La = load(U, nu, g, shift_c(g, xpm, nu, c[nu], -1));
 		__builtin_amdgcn_sched_barrier(0);
 		m3<R> T1;
-		if constexpr (plane_shared(SH, nu < MU ? nu : MU))
-			T1 = nu < MU ? adj_m3(lds_m3<R>(sh, plane_index(nu, MU), lane)) : lds_m3<R>(sh, plane_index(MU, nu), lane);
+		if constexpr (plane_shared(SH, nu < MU ? nu : MU, nu < MU ? MU : nu))
+			T1 = nu < MU ? adj_m3(lds_m3<R>(sh, share_slot(SH, nu, MU), lane))
+			             : lds_m3<R>(sh, share_slot(SH, MU, nu), lane);
 		else
 			T1 = upper_factor(A, B, nu < MU);
 		Lb = nu == 3 ? lds_m3<R>(own, MU, sm) : load(U, MU, g, xmn);
@@ -784,7 +812,7 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 		if constexpr (SH > 0)
 		{
 			// planes (0, q) are indices 0..2, so mode 1 needs three
-			__shared__ vec2_t<R> sh[SH == 2 ? 6 : 3][9][64];
+			__shared__ vec2_t<R> sh[shared_planes(SH)][9][64];
 			if constexpr (!PF)
 				switch (mu)
 				{
