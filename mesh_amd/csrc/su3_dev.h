@@ -610,7 +610,7 @@ struct Geo
 #define MESH_SITE_PAIRS 1
 #endif
 
-MD long site_off(Geo const &g, int f, long x)
+MHD long site_off(Geo const &g, int f, long x)
 {
 	long o = (long)(f * 9) * g.stride;
 	if (MESH_SITE_LB)
@@ -790,11 +790,80 @@ MD int coord(Geo const &g, long x, int mu) { return (int)(((unsigned)x / (unsign
 #else
 MD int coord(Geo const &g, long x, int mu) { return (int)((x / g.st[mu]) % g.L[mu]); }
 #endif
-MD long shift_c(Geo const &g, long x, int mu, int c, int s)
+MHD long shift_c(Geo const &g, long x, int mu, int c, int s)
 {
 	int n = c + s;
 	n = n >= g.L[mu] ? n - g.L[mu] : (n < 0 ? n + g.L[mu] : n);
 	return x + (long)(n - c) * g.st[mu];
+}
+
+// shift_c in whole blocks, for a direction whose site stride is a multiple
+// of the block length (g.st[mu] % 2^MESH_SITE_LB == 0): the blocks a site
+// moves by, which leaves its position within the block (and so its slot) as
+// it is
+MHD int blk_shift(Geo const &g, int mu, int c, int s)
+{
+	int n = c + s;
+	n = n >= g.L[mu] ? n - g.L[mu] : (n < 0 ? n + g.L[mu] : n);
+	return (n - c) * (g.st[mu] >> MESH_SITE_LB);
+}
+// element offset of matrix f, block blk, plane 0, slot l (site_off with the
+// site's block and slot given)
+MHD long blk_off(Geo const &g, int f, int blk, int l)
+{
+	return (long)(f * 9) * g.stride + (((long)blk * 9) << MESH_SITE_LB) + l;
+}
+
+// plane slot of offset o within a 64-site block (site_off's slot order)
+MHD int block_slot(int o) { return MESH_SITE_PAIRS ? (((o & 1) << 5) | (o >> 1)) : o; }
+
+// slots of the x3-neighbours x +/- e3 in x's block (the block holds whole
+// x3-rows: L3 | 64)
+MHD void x3_slots(Geo const &g, long x, int const *c, int &sp, int &sm)
+{
+	const int o = (int)(x & 63), c3 = c[3], L3 = g.L[3], o0 = o - c3;
+	sp = block_slot(o0 + (c3 + 1 == L3 ? 0 : c3 + 1));
+	sm = block_slot(o0 + (c3 == 0 ? L3 - 1 : c3 - 1));
+}
+
+// Rows path of the fused MD step (4-d, V % 64 == 0, L3 | 64 and
+// (L2 L3) % 64 == 0): a 64-site block holds whole x3-rows and lies within one
+// (x0, x1) plane.  A shift along x0 or x1 then moves every site of a block by
+// the same number of blocks and keeps its slot, and a shift along x3 stays in
+// the block, so a link displaced only along x0, x1 and x3 is found from the
+// block index, uniform over the block's wavefront, and a slot: its address is
+// formed in scalar registers but for the lane's slot.  RowsAt holds, for site
+// x, its block lb, its coordinates c0 and c1 (the block's), its slot and the
+// slots of x +/- e3.
+struct RowsAt
+{
+	int lb, c0, c1, slot, sp, sm;
+};
+MHD RowsAt rows_at_lane(Geo const &g, long x, int const *c)
+{
+	RowsAt a;
+	a.lb = (int)(x >> MESH_SITE_LB);
+	a.c0 = c[0];
+	a.c1 = c[1];
+	a.slot = block_slot((int)(x & 63));
+	x3_slots(g, x, c, a.sp, a.sm);
+	return a;
+}
+struct BlkSlot
+{
+	int blk, l;
+};
+// block and slot of the site x + s1 e_a1 + s2 e_a2 (a1, a2 != 2; a2 = -1: one
+// shift; |s| = 1)
+MHD BlkSlot rows_site(Geo const &g, RowsAt const &a, int a1, int s1, int a2, int s2)
+{
+	static_assert(MESH_SITE_LB == 6, "the rows path assumes 64-site blocks");
+	const int d0 = (a1 == 0 ? s1 : 0) + (a2 == 0 ? s2 : 0), d1 = (a1 == 1 ? s1 : 0) + (a2 == 1 ? s2 : 0);
+	const int d3 = (a1 == 3 ? s1 : 0) + (a2 == 3 ? s2 : 0);
+	BlkSlot b;
+	b.blk = a.lb + (d0 ? blk_shift(g, 0, a.c0, d0) : 0) + (d1 ? blk_shift(g, 1, a.c1, d1) : 0);
+	b.l = d3 == 0 ? a.slot : (d3 > 0 ? a.sp : a.sm);
+	return b;
 }
 
 #undef MD

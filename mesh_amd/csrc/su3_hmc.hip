@@ -250,7 +250,6 @@ __device__ __forceinline__ m3<R> staple_4d(Geo const &g, const R *__restrict__ U
 // (own[d][k][slot], staged once by the workgroup instead of once per
 // direction's wavefront): the four links of site x, and with ROWS (the block
 // holds whole x3-rows, L3 | 64) also those of its x3-neighbours.
-__device__ __forceinline__ int block_slot(int o) { return MESH_SITE_PAIRS ? (((o & 1) << 5) | (o >> 1)) : o; }
 
 template <class R> __device__ __forceinline__ m3<R> lds_m3(vec2_t<R> (*a)[9][64], int d, int slot)
 {
@@ -264,12 +263,38 @@ template <class R> __device__ __forceinline__ m3<R> lds_m3(vec2_t<R> (*a)[9][64]
 	return m;
 }
 
-// slots of the x3-neighbours x +/- e3 in the staged block (whole x3-rows)
-__device__ __forceinline__ void x3_slots(Geo const &g, long x, int const *c, int &sp, int &sm)
+// RowsAt of x with the block and its x0, x1 coordinates in scalar registers
+// (the same for every lane of the block's wavefront on the rows path)
+__device__ __forceinline__ RowsAt rows_at(Geo const &g, long x, int const *c)
 {
-	const int o = (int)(x & 63), c3 = c[3], L3 = g.L[3], o0 = o - c3;
-	sp = block_slot(o0 + (c3 + 1 == L3 ? 0 : c3 + 1));
-	sm = block_slot(o0 + (c3 == 0 ? L3 - 1 : c3 - 1));
+	RowsAt a = rows_at_lane(g, x, c);
+	a.lb = __builtin_amdgcn_readfirstlane(a.lb);
+	a.c0 = __builtin_amdgcn_readfirstlane(a.c0);
+	a.c1 = __builtin_amdgcn_readfirstlane(a.c1);
+	return a;
+}
+
+// link f at x + s1 e_a1 + s2 e_a2 (a2 = -1: one shift); xs is that site,
+// which the rows path needs only for shifts along x2
+template <bool ROWS, class R>
+__device__ __forceinline__ m3<R> load_disp(const R *__restrict__ U, int f, Geo const &g, RowsAt const &a, int a1,
+                                           int s1, int a2, int s2, long xs)
+{
+	if constexpr (ROWS)
+		if (a1 != 2 && a2 != 2)
+		{
+			const BlkSlot b = rows_site(g, a, a1, s1, a2, s2);
+			const vec2_t<R> *p = reinterpret_cast<const vec2_t<R> *>(U) + blk_off(g, f, b.blk, 0);
+			m3<R> m;
+#pragma unroll
+			for (int k = 0; k < 9; ++k)
+			{
+				vec2_t<R> v = p[b.l + (k << MESH_SITE_LB)];
+				m.e[k] = {v.x, v.y};
+			}
+			return m;
+		}
+	return load(U, f, g, xs);
 }
 
 // index of plane (p, q), p < q, among the six of a 4-d lattice
@@ -343,12 +368,15 @@ template <int W, bool ROWS, int MODE, class R>
 __device__ __forceinline__ void share_fetch(Geo const &g, const R *__restrict__ U, long x, int const *c,
                                             ShareOps<R> &F)
 {
+	RowsAt A{};
+	if constexpr (ROWS)
+		A = rows_at(g, x, c);
 	for_shared_planes<W, MODE>([&](auto P, auto Q, auto I) {
 		constexpr int p = decltype(P)::value, q = decltype(Q)::value, i = decltype(I)::value;
 		if constexpr (!(ROWS && p == 3))
-			F.a[i] = load(U, q, g, shift_c(g, x, p, c[p], 1));
+			F.a[i] = load_disp<ROWS>(U, q, g, A, p, 1, -1, 0, shift_c(g, x, p, c[p], 1));
 		if constexpr (!(ROWS && q == 3))
-			F.b[i] = load(U, p, g, shift_c(g, x, q, c[q], 1));
+			F.b[i] = load_disp<ROWS>(U, p, g, A, q, 1, -1, 0, shift_c(g, x, q, c[q], 1));
 	});
 }
 
@@ -384,9 +412,10 @@ __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict
 {
 	auto own_at = [&](int d, int slot) { return lds_m3<R>(own, d, slot); };
 	// slots of the x3-neighbours x +/- e3 (used with ROWS only)
-	int sp = 0, sm = 0;
+	RowsAt A{};
 	if constexpr (ROWS)
-		x3_slots(g, x, c, sp, sm);
+		A = rows_at(g, x, c);
+	const int sp = A.sp, sm = A.sm;
 	m3<R> S = zero_m3<R>();
 	long xpm = shift_c(g, x, MU, c[MU], 1);
 #pragma unroll
@@ -407,16 +436,16 @@ __device__ __forceinline__ m3<R> staple_4d_own(Geo const &g, const R *__restrict
 		}
 		else
 		{
-			m3<R> a = fwd3 ? own_at(nu, sp) : load(U, nu, g, xpm);
-			m3<R> b = nu3 ? own_at(MU, sp) : load(U, MU, g, xpn);
+			m3<R> a = fwd3 ? own_at(nu, sp) : load_disp<ROWS>(U, nu, g, A, MU, 1, -1, 0, xpm);
+			m3<R> b = nu3 ? own_at(MU, sp) : load_disp<ROWS>(U, MU, g, A, nu, 1, -1, 0, xpn);
 			m3<R> T1 = upper_factor(a, b, nu < MU);
 			madd_nd(S, T1, own_at(nu, lane));
 		}
 		{
-			m3<R> a = load(U, nu, g, xpmmn);
-			m3<R> b = nu3 ? own_at(MU, sm) : load(U, MU, g, xmn);
+			m3<R> a = load_disp<ROWS>(U, nu, g, A, MU, 1, nu, -1, xpmmn);
+			m3<R> b = nu3 ? own_at(MU, sm) : load_disp<ROWS>(U, MU, g, A, nu, -1, -1, 0, xmn);
 			m3<R> T2 = mul_dd(a, b);
-			madd_nn(S, T2, nu3 ? own_at(nu, sm) : load(U, nu, g, xmn));
+			madd_nn(S, T2, nu3 ? own_at(nu, sm) : load_disp<ROWS>(U, nu, g, A, nu, -1, -1, 0, xmn));
 		}
 	}
 	return S;
@@ -433,8 +462,8 @@ template <int MU, int SH, class R>
 __device__ __forceinline__ m3<R> staple_rows_pipe(Geo const &g, const R *__restrict__ U, long x, int const *c,
                                                   vec2_t<R> (*own)[9][64], int lane, vec2_t<R> (*sh)[9][64])
 {
-	int sp, sm;
-	x3_slots(g, x, c, sp, sm);
+	const RowsAt At = rows_at(g, x, c);
+	const int sp = At.sp, sm = At.sm;
 	const long xpm = shift_c(g, x, MU, c[MU], 1);
 	m3<R> S = zero_m3<R>(), A, B, La, Lb, Lc;
 	// operands of the upper staple's first product (planes not shared)
@@ -442,15 +471,16 @@ __device__ __forceinline__ m3<R> staple_rows_pipe(Geo const &g, const R *__restr
 		constexpr int nu = decltype(NU)::value;
 		if constexpr (!plane_shared(SH, nu < MU ? nu : MU, nu < MU ? MU : nu))
 		{
-			A = MU == 3 ? lds_m3<R>(own, nu, sp) : load(U, nu, g, xpm);
-			B = nu == 3 ? lds_m3<R>(own, MU, sp) : load(U, MU, g, shift_c(g, x, nu, c[nu], 1));
+			A = MU == 3 ? lds_m3<R>(own, nu, sp) : load_disp<true>(U, nu, g, At, MU, 1, -1, 0, xpm);
+			B = nu == 3 ? lds_m3<R>(own, MU, sp)
+			            : load_disp<true>(U, MU, g, At, nu, 1, -1, 0, shift_c(g, x, nu, c[nu], 1));
 		}
 	};
 	// the upper and lower staples of plane (MU, nu); NEXT: the next nu (-1: none)
 	auto step = [&](auto NU, auto NEXT) {
 		constexpr int nu = decltype(NU)::value, nx = decltype(NEXT)::value;
 		const long xmn = nu == 3 ? 0 : shift_c(g, x, nu, c[nu], -1);
-		La = load(U, nu, g, shift_c(g, xpm, nu, c[nu], -1));
+		La = load_disp<true>(U, nu, g, At, MU, 1, nu, -1, shift_c(g, xpm, nu, c[nu], -1));
 		__builtin_amdgcn_sched_barrier(0);
 		m3<R> T1;
 		if constexpr (plane_shared(SH, nu < MU ? nu : MU, nu < MU ? MU : nu))
@@ -458,10 +488,10 @@ __device__ __forceinline__ m3<R> staple_rows_pipe(Geo const &g, const R *__restr
 			             : lds_m3<R>(sh, share_slot(SH, MU, nu), lane);
 		else
 			T1 = upper_factor(A, B, nu < MU);
-		Lb = nu == 3 ? lds_m3<R>(own, MU, sm) : load(U, MU, g, xmn);
+		Lb = nu == 3 ? lds_m3<R>(own, MU, sm) : load_disp<true>(U, MU, g, At, nu, -1, -1, 0, xmn);
 		__builtin_amdgcn_sched_barrier(0);
 		madd_nd(S, T1, lds_m3<R>(own, nu, lane));
-		Lc = nu == 3 ? lds_m3<R>(own, nu, sm) : load(U, nu, g, xmn);
+		Lc = nu == 3 ? lds_m3<R>(own, nu, sm) : load_disp<true>(U, nu, g, At, nu, -1, -1, 0, xmn);
 		__builtin_amdgcn_sched_barrier(0);
 		m3<R> T2 = mul_dd(La, Lb);
 		if constexpr (nx >= 0)
@@ -745,7 +775,8 @@ __device__ __forceinline__ void fused_link_step(Geo const &g, R *__restrict__ Un
 // antihermitian layout (the fused kernels run only inside a trajectory).
 // Staple paths of k_force_update, one per kernel (a kernel is compiled for
 // the registers of its largest path, and the general ones need more):
-//   STAPLE_ROWS  4-d, V % 64 == 0, L3 | 64: the block's links staged in LDS,
+//   STAPLE_ROWS  4-d, V % 64 == 0, L3 | 64, (L2 L3) % 64 == 0: the block's
+//                links staged in LDS,
 //                own and x3-neighbour links read there;
 //   STAPLE_OWN   4-d, V % 64 == 0: the block's links staged, own links read there;
 //   STAPLE_ANY   any geometry: every link from global memory.
@@ -759,7 +790,7 @@ inline int staple_path(Geo const &g)
 {
 	if (!MESH_OWN_LDS || g.nd != 4 || (g.V & 63) != 0)
 		return STAPLE_ANY;
-	return (MESH_OWN_ROWS && 64 % g.L[3] == 0) ? STAPLE_ROWS : STAPLE_OWN;
+	return (MESH_OWN_ROWS && 64 % g.L[3] == 0 && (g.L[2] * g.L[3]) % 64 == 0) ? STAPLE_ROWS : STAPLE_OWN;
 }
 
 template <class R, bool FUSE, bool PC, int PATH>
