@@ -23,12 +23,16 @@ using namespace mesh_dev;
 #ifndef MESH_PIPE_F64
 #define MESH_PIPE_F64 1
 #endif
-// FP32: the global operands of the shared upper-staple factors loaded before
-// the staging barrier (share_fetch; same bits); 0: after it.  Same-box A/B at
-// 32^4: FP32 0.3443 -> 0.3401 ms; in FP64 (one plane's two links per
-// wavefront) it measured 0.8% slower and is not used.
+// The global operands of the shared upper-staple factors loaded before the
+// staging barrier (share_fetch; same bits), FP32 / FP64; 0: after it.  With
+// three FP32 workgroups per CU it gained 1.2% (0.3443 -> 0.3401 ms at 32^4);
+// with four (the four-plane mode) it costs 0.4% (0.3417 vs 0.3432 ms, three
+// same-box rounds), and in FP64 it is even (0.5626 vs 0.5626 ms).
 #ifndef MESH_SHARE_PREFETCH
-#define MESH_SHARE_PREFETCH 1
+#define MESH_SHARE_PREFETCH 0
+#endif
+#ifndef MESH_SHARE_PREFETCH_F64
+#define MESH_SHARE_PREFETCH_F64 0
 #endif
 // waves per SIMD the force kernels are compiled for (0: the compiler's choice)
 #ifndef MESH_FORCE_WAVES
@@ -816,7 +820,7 @@ __attribute__((amdgpu_waves_per_eu(force_min_waves<R, PATH>(), force_max_waves<R
 		constexpr bool RW = PATH == STAPLE_ROWS; // the block holds whole x3-rows
 		constexpr int SH = share_mode<R>();
 		ShareOps<R> F;
-		constexpr bool PF = MESH_SHARE_PREFETCH && std::is_same<R, float>::value;
+		constexpr bool PF = std::is_same<R, float>::value ? MESH_SHARE_PREFETCH : MESH_SHARE_PREFETCH_F64;
 		if constexpr (SH > 0 && PF)
 			switch (mu)
 			{
