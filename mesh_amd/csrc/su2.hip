@@ -140,6 +140,32 @@ __device__ __forceinline__ long nbr(G2 const &g, long x, int mu, int s)
 	return (long)((int)xs + (n - c) * g.st[mu]);
 }
 
+// coordinates of site x (c[mu] = (x / st[mu]) % L[mu], 0 beyond nd), with
+// one division per direction, and the neighbour of x along mu from its known
+// coordinate c = c[mu]: nbr's site without nbr's division
+__device__ __forceinline__ void coords2(G2 const &g, long x, int (&c)[4])
+{
+	unsigned t = (unsigned)x;
+#pragma unroll
+	for (int mu = 3; mu >= 0; --mu)
+	{
+		if (mu < g.nd)
+		{
+			const unsigned q = t / (unsigned)g.L[mu];
+			c[mu] = (int)(t - q * (unsigned)g.L[mu]);
+			t = q;
+		}
+		else
+			c[mu] = 0;
+	}
+}
+__device__ __forceinline__ long shc(G2 const &g, long x, int mu, int c, int s)
+{
+	int n = c + s;
+	n = n < 0 ? n + g.L[mu] : (n >= g.L[mu] ? n - g.L[mu] : n);
+	return (long)((int)x + (n - c) * g.st[mu]);
+}
+
 // quaternion (w, x, y, z) of direction f at site x: four consecutive reals,
 // one vector load (two for FP64)
 template <class R> using v4 = R __attribute__((ext_vector_type(4)));
@@ -172,15 +198,21 @@ template <class R> __device__ __forceinline__ qt<R> staple(G2 const &g, const R 
 	return S;
 }
 
-// sum over planes mu < nu of Re P_{mu nu}(x) (the quaternion's real part, Re tr / 2), in FP64
-template <class R> __device__ __forceinline__ double plaq_site(G2 const &g, const R *__restrict__ U, long x)
+// sum over planes mu < nu of Re P_{mu nu}(x) (the quaternion's real part, Re tr / 2), in FP64;
+// c: the coordinates of x (coords2)
+template <class R>
+__device__ __forceinline__ double plaq_site(G2 const &g, const R *__restrict__ U, long x, int const (&c)[4])
 {
 	double s = 0.0;
-	for (int mu = 0; mu < g.nd; ++mu)
-		for (int nu = mu + 1; nu < g.nd; ++nu)
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+		for (int nu = mu + 1; nu < 4; ++nu)
 		{
-			qt<double> A = qmul(q64(ldq(U, g, mu, x)), q64(ldq(U, g, nu, nbr(g, x, mu, 1))));
-			qt<double> B = qmul(q64(ldq(U, g, nu, x)), q64(ldq(U, g, mu, nbr(g, x, nu, 1))));
+			if (nu >= g.nd)
+				continue;
+			qt<double> A = qmul(q64(ldq(U, g, mu, x)), q64(ldq(U, g, nu, shc(g, x, mu, c[mu], 1))));
+			qt<double> B = qmul(q64(ldq(U, g, nu, x)), q64(ldq(U, g, mu, shc(g, x, nu, c[nu], 1))));
 			// Re (A B^+) = A . B
 			s += fma(A.z, B.z, fma(A.y, B.y, fma(A.x, B.x, A.w * B.w)));
 		}
@@ -303,12 +335,16 @@ __global__ __launch_bounds__(256) void k_su2_force_link_rows(G2 g, const R *__re
 	const int L3 = g.L[3], c3 = lane % L3, o0 = lane - c3;
 	const int sp = o0 + (c3 + 1 == L3 ? 0 : c3 + 1), sm = o0 + (c3 == 0 ? L3 - 1 : c3 - 1);
 	qt<R> S = {R(0), R(0), R(0), R(0)};
-	const long xpm = nbr(g, x, mu, 1);
+	int c[4];
+	coords2(g, x, c);
+	const long xpm = shc(g, x, mu, mu == 0 ? c[0] : (mu == 1 ? c[1] : (mu == 2 ? c[2] : c[3])), 1);
+#pragma unroll
 	for (int nu = 0; nu < 4; ++nu)
 	{
 		if (nu == mu)
 			continue;
-		const long xpn = nbr(g, x, nu, 1), xmn = nbr(g, x, nu, -1), xpmmn = nbr(g, xpm, nu, -1);
+		// x + mu has x's coordinate along nu
+		const long xpn = shc(g, x, nu, c[nu], 1), xmn = shc(g, x, nu, c[nu], -1), xpmmn = shc(g, xpm, nu, c[nu], -1);
 		const qt<R> a = mu == 3 ? O(nu, sp) : ldq(U, g, nu, xpm); // U_nu(x+mu)
 		const qt<R> b = nu == 3 ? O(mu, sp) : ldq(U, g, mu, xpn); // U_mu(x+nu)
 		qt<R> up = qmul(qmul(a, qconj(b)), qconj(O(nu, lane)));
@@ -363,7 +399,13 @@ template <class R> __global__ __launch_bounds__(256) void k_su2_plaq(G2 g, const
                                                                    int nb)
 {
 	long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	double v[1] = {x < g.V ? plaq_site(g, U, x) : 0.0};
+	double v[1] = {0.0};
+	if (x < g.V)
+	{
+		int c[4];
+		coords2(g, x, c);
+		v[0] = plaq_site(g, U, x, c);
+	}
 	block_sum_store<1>(v, partials, nb, blockIdx.x);
 }
 
@@ -377,7 +419,9 @@ __global__ __launch_bounds__(256) void k_su2_delta_h(G2 g, const R *__restrict__
 	double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 	if (x < g.V)
 	{
-		double pso = plaq_site(g, Uo, x), psn = plaq_site(g, Un, x);
+		int c[4];
+		coords2(g, x, c);
+		double pso = plaq_site(g, Uo, x, c), psn = plaq_site(g, Un, x, c);
 		double kn = 0.0;
 		for (int mu = 0; mu < g.nd; ++mu)
 			kn += kin(ldq(P, g, mu, x));
@@ -536,11 +580,19 @@ __global__ __launch_bounds__(256) void k_su2_or_half(G2 g, R *U, R *h, int parit
 	if (t >= (g.V >> 1))
 		return;
 	const int f = g.nd - 1, hl = g.L[f] >> 1;
-	const long row = t / hl;
+	// sites are below 2^31 (nbr's 32-bit arithmetic): 32-bit divisions
+	const long row = (long)((unsigned)t / (unsigned)hl);
 	const long x0 = row * g.L[f];
-	int ps = 0;
-	for (int mu = 0; mu < f; ++mu)
-		ps += (int)((x0 / g.st[mu]) % g.L[mu]);
+	// coordinates of the row's first site in the slower directions, then x's
+	// along the row
+	int cc[4] = {0, 0, 0, 0}, ps = 0;
+#pragma unroll
+	for (int mu = 0; mu < 3; ++mu)
+		if (mu < f)
+		{
+			cc[mu] = (int)(((unsigned)x0 / (unsigned)g.st[mu]) % (unsigned)g.L[mu]);
+			ps += cc[mu];
+		}
 	const long x = x0 + 2 * (t - row * hl) + ((parity + ps) & 1);
 	qt<R> fw[4], bw[4];
 	long xm[4];
@@ -549,7 +601,7 @@ __global__ __launch_bounds__(256) void k_su2_or_half(G2 g, R *U, R *h, int parit
 	for (int mu = 0; mu < 4; ++mu) // unrolled: the arrays stay in registers
 		if (mu < g.nd)
 		{
-			xm[mu] = nbr(g, x, mu, -1);
+			xm[mu] = shc(g, x, mu, mu == f ? (int)(x - x0) : cc[mu], -1);
 			fw[mu] = ldq(U, g, mu, x);
 			bw[mu] = ldq(U, g, mu, xm[mu]);
 			K = qadd(qadd(K, fw[mu]), qconj(bw[mu]));
