@@ -185,12 +185,20 @@ template <class R> __device__ __forceinline__ void or_site(Geo const &g, R *U, R
 {
 	int f = g.nd - 1;
 	int hl = g.L[f] >> 1;
-	long row = t / hl;
+	// sites are below 2^26: 32-bit divisions
+	long row = (long)((unsigned)t / (unsigned)hl);
 	int j = (int)(t - row * hl);
 	long x0 = row * g.L[f];
-	int ps = 0;
-	for (int mu = 0; mu < f; ++mu)
-		ps += coord(g, x0, mu);
+	// x's coordinates: the row start's in the slower directions, 2 j + the
+	// parity bit along the row (one division per direction in all)
+	int cc[4] = {0, 0, 0, 0}, ps = 0;
+#pragma unroll
+	for (int mu = 0; mu < 3; ++mu)
+		if (mu < f)
+		{
+			cc[mu] = coord(g, x0, mu);
+			ps += cc[mu];
+		}
 	long x = x0 + 2 * j + ((parity + ps) & 1);
 	m3<R> K = zero_m3<R>();
 	long xm[4];
@@ -204,7 +212,7 @@ template <class R> __device__ __forceinline__ void or_site(Geo const &g, R *U, R
 	{
 		if (mu >= g.nd)
 			break;
-		xm[mu] = shift_c(g, x, mu, coord(g, x, mu), -1);
+		xm[mu] = shift_c(g, x, mu, mu == f ? (int)(x - x0) : cc[mu], -1);
 		// FP64: the forward links stream past the L2 so that the backward
 		// ones, read again after the transformation, are still there
 		m3<R> a = (!keep && MESH_OR_STREAM) ? load_stream(U, mu, g, x) : load(U, mu, g, x);
