@@ -185,12 +185,16 @@ template <class R> __device__ __forceinline__ void stq(R *__restrict__ b, G2 con
 template <class R> __device__ __forceinline__ qt<R> staple(G2 const &g, const R *__restrict__ U, int mu, long x)
 {
 	qt<R> S = {R(0), R(0), R(0), R(0)};
-	const long xpm = nbr(g, x, mu, 1);
-	for (int nu = 0; nu < g.nd; ++nu)
+	int c[4];
+	coords2(g, x, c);
+	const long xpm = shc(g, x, mu, mu == 0 ? c[0] : (mu == 1 ? c[1] : (mu == 2 ? c[2] : c[3])), 1);
+#pragma unroll
+	for (int nu = 0; nu < 4; ++nu)
 	{
-		if (nu == mu)
+		if (nu == mu || nu >= g.nd)
 			continue;
-		const long xpn = nbr(g, x, nu, 1), xmn = nbr(g, x, nu, -1), xpmmn = nbr(g, xpm, nu, -1);
+		// x + mu has x's coordinate along nu
+		const long xpn = shc(g, x, nu, c[nu], 1), xmn = shc(g, x, nu, c[nu], -1), xpmmn = shc(g, xpm, nu, c[nu], -1);
 		qt<R> up = qmul(qmul(ldq(U, g, nu, xpm), qconj(ldq(U, g, mu, xpn))), qconj(ldq(U, g, nu, x)));
 		qt<R> lo = qmul(qmul(qconj(ldq(U, g, nu, xpmmn)), qconj(ldq(U, g, mu, xmn))), ldq(U, g, nu, xmn));
 		S = qadd(qadd(S, up), lo);
@@ -463,9 +467,14 @@ template <class R> __device__ __forceinline__ qt<R> landau_gd(G2 const &g, const
                                                               const R *__restrict__ h, long x)
 {
 	qt<R> d = {R(0), R(0), R(0), R(0)};
-	for (int mu = 0; mu < g.nd; ++mu)
+	int c[4];
+	coords2(g, x, c);
+#pragma unroll
+	for (int mu = 0; mu < 4; ++mu)
 	{
-		const long xp = nbr(g, x, mu, 1), xm = nbr(g, x, mu, -1);
+		if (mu >= g.nd)
+			break;
+		const long xp = shc(g, x, mu, c[mu], 1), xm = shc(g, x, mu, c[mu], -1);
 		qt<R> a = ldq(U, g, mu, x), b = ldq(U, g, mu, xm);
 		if (h)
 		{
