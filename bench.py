@@ -20,10 +20,11 @@ recorded around every launch on the library's stream during the timed region.
 Per-site algorithmic bytes of every kernel are in kernel_bytes(); the
 trajectory's total is the sum over the launches actually made.
 
-The acceptance, plaquette and <exp(-dH)> of the timed window belong to a
-chain only 50 + W trajectories from a Haar start; the "physics" objects come
-from the same chain continued untimed (--equil trajectories without
-Metropolis, then --measure recorded with it), near equilibrium.
+The acceptance, plaquette and <exp(-dH)> of the timed window ("timed_window")
+belong to a chain only --therm + W trajectories from a Haar start, still far
+from equilibrium; the "physics" objects come from the same chain continued
+untimed (--equil trajectories without Metropolis, then --measure recorded
+with it), near equilibrium, and are the ones to read as physics.
 
 cpu_baseline: the CPU oracle (FP64, OpenMP) running one trajectory of the
 same workload on the host cores, with the one-thread and reference-structured
@@ -240,6 +241,14 @@ def cpu_baseline(L, seed, scheme, eps, substeps):
                 modes=modes,
                 modes_note=("per trajectory from one timed pass of each stage at the same size: momenta, "
                             f"{n_force} momentum steps, {n_link} link steps, end (reunitarisation, Delta H)"))
+
+
+def timed_window(r, therm, warmup):
+    """Acceptance, plaquette and <exp(-dH)> of the timed trajectories, labelled
+    as what they are: a chain therm + warmup trajectories from a Haar start."""
+    return {"acceptance": r["acceptance"], "plaquette": r["plaquette"], "exp_minus_dH": r["exp_mdH"],
+            "note": f"timed trajectories {therm + warmup}.. of a chain from a Haar start, before "
+                    "equilibrium (exp_minus_dH far from 1); the 'physics' object holds the equilibrium values"}
 
 
 def kernel_only(name, prec, L, reps, variant=-1):
@@ -667,9 +676,7 @@ def main():
                      "kernel": dom, "avg_ms": fk["avg_ms"],
                      "algorithmic_bytes_per_launch": alg},
         "trajectory_roofline_frac": value * trajectory_bytes_per_site(r64["kernels"], 8, a.steps, compact) / 8e12,
-        "acceptance": r64["acceptance"],
-        "plaquette": r64["plaquette"],
-        "exp_minus_dH": r64["exp_mdH"],
+        "timed_window": timed_window(r64, a.therm, a.warmup),
         **({"physics": r64["physics"]} if "physics" in r64 else {}),
         "kernels_f64": r64["kernels"],
     }
@@ -683,7 +690,7 @@ def main():
         cmp = same_config_compare(r64["hmc"].U, a.seed, a.scheme, a.epsilon, a.substeps)
         line["f32_mixed"] = {
             "value": v32, "ms_per_step": 1e3 * r32["seconds"] / a.steps, "ratio_vs_f64": v32 / value,
-            "acceptance": r32["acceptance"], "plaquette": r32["plaquette"],
+            "timed_window": timed_window(r32, a.therm, a.warmup),
             **({"physics": r32["physics"]} if "physics" in r32 else {}),
             "dominant_kernel": dom32, "dominant_frac_of_hbm": ach32 / HBM_PEAK_GBS,
             "dominant_traffic": measured_traffic("f32", a.L) if dom32 == "force_link" and compact else None,
