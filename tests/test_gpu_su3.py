@@ -130,12 +130,14 @@ def test_force_variants_bitexact(ctx, shape, prec, variant, remap):
 # wavefront-per-direction kernel stages the block but reads x3-neighbours from
 # global memory (its "own links" path, with the shared upper-staple products)
 OWN_SHAPES = [(2, 4, 4, 6), (4, 2, 8, 12), (4, 4, 4, 3)]
+# ... and shapes whose blocks lie within one (x0, x1) plane ((L2 L3) % 64 == 0)
+OWN_UNI_SHAPES = [(4, 4, 16, 12), (2, 6, 16, 20)]
 
 
 @pytest.mark.parametrize("remap", [0, 3])
 @pytest.mark.parametrize("variant", [-1, 1])
 @pytest.mark.parametrize("prec", [F64, F32])
-@pytest.mark.parametrize("shape", OWN_SHAPES)
+@pytest.mark.parametrize("shape", OWN_SHAPES + OWN_UNI_SHAPES)
 def test_force_own_path_bitexact(ctx, shape, prec, variant, remap):
     U = O.random_gauge(list(shape), 23, _pname(prec))
     P, _ = O.momenta(list(shape), 23, 0, _pname(prec))
@@ -147,7 +149,7 @@ def test_force_own_path_bitexact(ctx, shape, prec, variant, remap):
 
 
 @pytest.mark.parametrize("prec", [F64, F32])
-@pytest.mark.parametrize("shape", OWN_SHAPES[:2])
+@pytest.mark.parametrize("shape", OWN_SHAPES[:2] + OWN_UNI_SHAPES)
 def test_fused_trajectory_own_path_bitexact(ctx, shape, prec):
     shape = list(shape)
     hmc = gauge.Hmc(mesh_amd.Grid(shape), prec, seed=8, ctx=ctx)
