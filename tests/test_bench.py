@@ -55,3 +55,12 @@ def test_cpu_baseline_fields():
     assert set(r["modes"]) == {"fused_1_thread", "reference_structured", "reference_structured_1_thread"}
     assert all(m["value"] > 0 and len(m["stage_seconds"]) == 4 for m in r["modes"].values())
     assert r["modes"]["fused_1_thread"]["threads"] == 1
+
+
+def test_timed_window_is_labelled():
+    """the acceptance, plaquette and <exp(-dH)> of the timed trajectories come
+    out in their own object, labelled as a chain before equilibrium, so that
+    they are not read as the equilibrium figures of the physics object"""
+    w = bench.timed_window({"acceptance": 0.5, "plaquette": 0.59, "exp_mdH": 3.0}, 50, 2)
+    assert set(w) == {"acceptance", "plaquette", "exp_minus_dH", "note"}
+    assert w["exp_minus_dH"] == 3.0 and "52" in w["note"] and "physics" in w["note"]
